@@ -1,0 +1,250 @@
+"""Seeded synthetic inputs for the dense-tracking hot path.
+
+Five workloads, named by what they are:
+
+* ``small``     -- 4 keyframes, all 12 directed edges, 48x64; two 128-d fmaps
+                   for a 4-level radius-3 CorrBlock lookup.
+* ``window``    -- frontend window: 12 keyframes, neighbours within +-3 plus 10
+                   long edges (70 edges), 4 GN iterations, t0 = 2.
+* ``global300`` -- global backend BA: 300 keyframes, 3000 edges with
+                   Poisson(10) out-degree (one frame with 1 edge, one with 40),
+                   8 GN iterations, t0 = 0, plus a radius-3 lookup for every edge.
+* ``diff16``    -- 16 keyframes, 80 edges (differentiation workload).
+* ``hires64``   -- 64 keyframes, 800 edges at 96x128 (on-the-fly correlation).
+
+Geometry: camera poses on a random walk (or exp of N(0, 0.05) for ``small``),
+inverse depth U(0.2, 1.0), targets = exact reprojection of the true scene plus
+N(0, 0.5 px), weights U(0, 1), eta = 1e-4.  The BA starts from a perturbed
+state (poses in the optimisation window get a left perturbation Exp(N(0,0.01)),
+disparities a 5 % multiplicative perturbation) so every GN iteration has work.
+Everything is float32 numpy except the index arrays (int64).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+INTRINSICS = (64.0, 64.0, 32.0, 24.0)
+
+
+@dataclass
+class BAProblem:
+    name: str
+    poses: np.ndarray          # [N, 7] float32, perturbed start
+    disps: np.ndarray          # [N, h, w] float32, perturbed start
+    intrinsics: np.ndarray     # [4] float32
+    targets: np.ndarray        # [E, 2, h, w] float32
+    weights: np.ndarray        # [E, 2, h, w] float32
+    eta: np.ndarray            # [F, h, w] float32 (F = unique frames of [t0,t1) ++ ii)
+    ii: np.ndarray             # [E] int64
+    jj: np.ndarray             # [E] int64
+    t0: int
+    t1: int
+    iterations: int
+    lm: float
+    ep: float
+    poses_true: np.ndarray = field(repr=False, default=None)
+    disps_true: np.ndarray = field(repr=False, default=None)
+
+
+def _quat_mul(a, b):
+    ax, ay, az, aw = a[..., 0], a[..., 1], a[..., 2], a[..., 3]
+    bx, by, bz, bw = b[..., 0], b[..., 1], b[..., 2], b[..., 3]
+    return np.stack([aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by + ay * bw + az * bx - ax * bz,
+                     aw * bz + az * bw + ax * by - ay * bx,
+                     aw * bw - ax * bx - ay * by - az * bz], -1)
+
+
+def _rotmat(q):
+    x, y, z, w = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                  2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                  2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)
+    return R.reshape(q.shape[:-1] + (3, 3))
+
+
+def _exp(xi):
+    """SE3 exponential of [tau, phi] as a [..., 7] pose (float64)."""
+    tau, phi = xi[..., :3], xi[..., 3:]
+    th = np.linalg.norm(phi, axis=-1, keepdims=True)
+    ths = np.where(th < 1e-12, 1.0, th)
+    imag = np.where(th < 1e-12, 0.5, np.sin(0.5 * ths) / ths)
+    q = np.concatenate([imag * phi, np.cos(0.5 * th)], -1)
+    a = np.where(th < 1e-12, 0.5, (1 - np.cos(ths)) / ths ** 2)
+    b = np.where(th < 1e-12, 1.0 / 6.0, (ths - np.sin(ths)) / ths ** 3)
+    c1 = np.cross(phi, tau)
+    t = tau + a * c1 + b * np.cross(phi, c1)
+    return np.concatenate([t, q], -1)
+
+
+def _compose(p, q):
+    """p * q for poses [..., 7]."""
+    t = np.einsum('...ij,...j->...i', _rotmat(p[..., 3:]), q[..., :3]) + p[..., :3]
+    return np.concatenate([t, _quat_mul(p[..., 3:], q[..., 3:])], -1)
+
+
+def _reproject(poses, disps, intr, ii, jj):
+    """Exact reprojection pi(G_j G_i^-1 X(u, v, d_i)) -> [E, 2, h, w] (float64)."""
+    fx, fy, cx, cy = intr
+    _, h, w = disps.shape
+    v, u = np.meshgrid(np.arange(h, dtype=np.float64), np.arange(w, dtype=np.float64),
+                       indexing='ij')
+    X0 = np.stack([(u - cx) / fx, (v - cy) / fy, np.ones_like(u)])
+    out = np.zeros((len(ii), 2, h, w))
+    for s in range(0, len(ii), 256):
+        a, b = ii[s:s + 256], jj[s:s + 256]
+        pi, pj = poses[a], poses[b]
+        qi_inv = np.concatenate([-pi[:, 3:6], pi[:, 6:]], -1)
+        qij = _quat_mul(pj[:, 3:], qi_inv)
+        R = _rotmat(qij)
+        tij = pj[:, :3] - np.einsum('eij,ej->ei', R, pi[:, :3])
+        X = np.einsum('eij,jhw->eihw', R, X0) + tij[:, :, None, None] * disps[a][:, None]
+        z = np.where(np.abs(X[:, 2]) < 1e-6, 1e-6, X[:, 2])
+        out[s:s + 256, 0] = fx * X[:, 0] / z + cx
+        out[s:s + 256, 1] = fy * X[:, 1] / z + cy
+    return out
+
+
+def _random_walk(rng, n, step_m=0.05, step_deg=2.0):
+    poses = np.zeros((n, 7))
+    poses[0, 6] = 1.0
+    for k in range(1, n):
+        d = rng.normal(size=3)
+        d *= step_m / np.linalg.norm(d)
+        a = rng.normal(size=3)
+        a *= np.deg2rad(step_deg) / np.linalg.norm(a)
+        poses[k] = _compose(_exp(np.concatenate([d, a])), poses[k - 1])
+    return poses
+
+
+def _finish(name, rng, poses_true, ii, jj, h, w, t0, t1, iterations, lm=1e-4, ep=0.1,
+            eta_val=1e-4):
+    n = poses_true.shape[0]
+    disps_true = rng.uniform(0.2, 1.0, size=(n, h, w))
+    intr = np.asarray(INTRINSICS, np.float64)
+    tgt = _reproject(poses_true, disps_true, intr, ii, jj)
+    tgt += rng.normal(scale=0.5, size=tgt.shape)
+    wts = rng.uniform(0.0, 1.0, size=tgt.shape)
+    poses = poses_true.copy()
+    pert = _exp(rng.normal(scale=0.01, size=(t1 - t0, 6)))
+    poses[t0:t1] = _compose(pert, poses[t0:t1])
+    disps = disps_true * (1.0 + 0.05 * rng.normal(size=disps_true.shape))
+    disps = np.clip(disps, 0.05, None)
+    frames = np.unique(np.concatenate([np.arange(t0, t1), ii]))
+    eta = np.full((len(frames), h, w), eta_val)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return BAProblem(name=name, poses=f32(poses), disps=f32(disps), intrinsics=f32(intr),
+                     targets=f32(tgt), weights=f32(wts), eta=f32(eta),
+                     ii=np.ascontiguousarray(ii, np.int64), jj=np.ascontiguousarray(jj, np.int64),
+                     t0=t0, t1=t1, iterations=iterations, lm=lm, ep=ep,
+                     poses_true=f32(poses_true), disps_true=f32(disps_true))
+
+
+def ba_small(seed=0, h=48, w=64):
+    """4 keyframes, all 12 directed edges, 2 GN iterations, t0 = 1."""
+    rng = np.random.default_rng(seed)
+    poses = _exp(rng.normal(scale=0.05, size=(4, 6)))
+    poses[0] = [0, 0, 0, 0, 0, 0, 1]
+    ii, jj = np.meshgrid(np.arange(4), np.arange(4), indexing='ij')
+    m = ii != jj
+    return _finish('small', rng, poses, ii[m], jj[m], h, w, t0=1, t1=4, iterations=2)
+
+
+def _window_edges(rng, n, radius, n_long):
+    ii, jj = [], []
+    for i in range(n):
+        for j in range(n):
+            if i != j and abs(i - j) <= radius:
+                ii.append(i)
+                jj.append(j)
+    have = set(zip(ii, jj))
+    while n_long > 0:
+        i, j = rng.integers(0, n, size=2)
+        if abs(int(i) - int(j)) > radius and (int(i), int(j)) not in have:
+            have.add((int(i), int(j)))
+            ii.append(int(i))
+            jj.append(int(j))
+            n_long -= 1
+    return np.asarray(ii), np.asarray(jj)
+
+
+def ba_window(seed=1, h=48, w=64):
+    """Frontend window: 12 keyframes, 60 neighbour + 10 long edges, 4 iterations, t0 = 2."""
+    rng = np.random.default_rng(seed)
+    poses = _random_walk(rng, 12)
+    ii, jj = _window_edges(rng, 12, 3, 10)
+    return _finish('window', rng, poses, ii, jj, h, w, t0=2, t1=12, iterations=4)
+
+
+def _poisson_edges(rng, n, total, mean_deg=10.0, lo=1, hi=40, temporal_frac=0.7):
+    deg = np.clip(rng.poisson(mean_deg, size=n), lo, hi)
+    deg[n // 3] = lo
+    deg[(2 * n) // 3] = hi
+    fixed = {n // 3, (2 * n) // 3}
+    free = [k for k in range(n) if k not in fixed]
+    while deg.sum() != total:
+        k = free[rng.integers(0, len(free))]
+        if deg.sum() > total and deg[k] > lo:
+            deg[k] -= 1
+        elif deg.sum() < total and deg[k] < hi:
+            deg[k] += 1
+    ii, jj = [], []
+    for i in range(n):
+        chosen = []
+        n_temp = int(round(temporal_frac * deg[i]))
+        step = 1
+        while len(chosen) < n_temp and step < n:
+            for j in (i + step, i - step):
+                if 0 <= j < n and len(chosen) < n_temp:
+                    chosen.append(j)
+            step += 1
+        while len(chosen) < deg[i]:
+            j = int(rng.integers(0, n))
+            if j != i and j not in chosen:
+                chosen.append(j)
+        ii.extend([i] * len(chosen))
+        jj.extend(chosen)
+    return np.asarray(ii), np.asarray(jj)
+
+
+def ba_global300(seed=2, h=48, w=64, n=300, total=3000):
+    """Global BA: 300 keyframes, 3000 edges, 8 iterations, t0 = 0, t1 = 300."""
+    rng = np.random.default_rng(seed)
+    poses = _random_walk(rng, n)
+    ii, jj = _poisson_edges(rng, n, total)
+    return _finish('global300', rng, poses, ii, jj, h, w, t0=0, t1=n, iterations=8)
+
+
+def ba_diff16(seed=3, h=48, w=64):
+    """16 keyframes, 80 edges (|i-j| <= 3 neighbours, trimmed/padded to 80), 2 iterations."""
+    rng = np.random.default_rng(seed)
+    poses = _random_walk(rng, 16)
+    ii, jj = _window_edges(rng, 16, 2, 80 - 2 * (15 + 14))
+    return _finish('diff16', rng, poses, ii, jj, h, w, t0=1, t1=16, iterations=2)
+
+
+def ba_hires64(seed=4, h=96, w=128):
+    """64 keyframes, 800 edges at 96x128, 4 iterations."""
+    rng = np.random.default_rng(seed)
+    poses = _random_walk(rng, 64)
+    ii, jj = _poisson_edges(rng, 64, 800, mean_deg=12.5, lo=1, hi=30)
+    return _finish('hires64', rng, poses, ii, jj, h, w, t0=1, t1=64, iterations=4)
+
+
+def fmaps(seed, n, c=128, h=48, w=64):
+    """Feature maps N(0,1)/sqrt(c), shape [1, n, c, h, w] float32."""
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(size=(1, n, c, h, w)) / np.sqrt(c)).astype(np.float32)
+
+
+def lookup_coords(seed, n, h=48, w=64, sigma=2.0):
+    """Pixel grid plus N(0, sigma) flow, shape [1, n, h, w, 2] (x, y) float32."""
+    rng = np.random.default_rng(seed)
+    v, u = np.meshgrid(np.arange(h, dtype=np.float32), np.arange(w, dtype=np.float32),
+                       indexing='ij')
+    grid = np.stack([u, v], -1)[None, None]
+    return (grid + sigma * rng.standard_normal(size=(1, n, h, w, 2))).astype(np.float32)
+
+
+PROBLEMS = {'small': ba_small, 'window': ba_window, 'global300': ba_global300,
+            'diff16': ba_diff16, 'hires64': ba_hires64}
