@@ -1,0 +1,233 @@
+#!/usr/bin/env python
+"""Benchmark of the dense-tracking hot path on one MI355X.
+
+Workload ``global300``: 300 keyframes at 48x64, 3000 edges (Poisson(10) out-degree),
+one ``ba`` call of 8 Gauss-Newton iterations (t0 = 0, t1 = 300, lm = 1e-4, ep = 0.1)
+plus one bf16 4-level radius-3 CorrBlock lookup over all 3000 edges per iteration.
+A step = that ba call + 8 lookups; ``value`` = GN iterations per second.
+
+Beside it, in the same process: BA-only iterations/s, lookup GB/s, the plain
+all-FP32 path (FP32 reduced Cholesky and back-substitution, FP32 volume) and the
+final weighted RMS reprojection error of both paths from the same start; the
+oracle timed on the host as the CPU baseline.  Inputs are synthetic and already
+resident in HBM when timing starts.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'hi-slam2_amd'))
+
+HBM_PEAK_GBPS = 8000.0
+LOOKUP_BYTES_BF16 = 912      # 4 levels * 8*8 taps * 2 B + 196 outputs * 2 B + 8 B coords
+LOOKUP_BYTES_F32 = 1816
+METRIC = 'dense-BA GN iters/s & corr-lookup GB/s vs plain GPU path at equal reproj error'
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
+    return ap.parse_args()
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU only; --gpus {args.gpus} is not supported '
+              '(the sharding design is described in DESIGN.md).', file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from hislam2_amd import backend, synth
+    from hislam2_amd import corr as hcorr
+    from hislam2_amd.metrics import reprojection_rms
+    db = backend()
+    dev = torch.device('cuda:0')
+    torch.cuda.set_device(dev)
+
+    log('generating global300')
+    p = synth.ba_global300()
+    E, iters = len(p.ii), p.iterations
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    poses0, disps0 = up(p.poses), up(p.disps)
+    intr, tg, wt, eta, ii, jj = up(p.intrinsics), up(p.targets), up(p.weights), up(p.eta), up(p.ii), up(p.jj)
+    fm32 = up(synth.fmaps(7, p.poses.shape[0]))             # [1, 300, 128, 48, 64]
+    coords = up(synth.lookup_coords(8, E))                    # [1, 3000, 48, 64, 2]
+    h, w = p.disps.shape[1:]
+    lookups = E * h * w
+
+    if args.copy_probe:
+        a = torch.empty(1 << 29, dtype=torch.float32, device=dev)   # 2 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    def ba_call(poses, disps, precision):
+        return db.ba_ex(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, iters, p.lm, p.ep,
+                        False, precision)
+
+    # ---- accuracy: both paths from the same start -------------------------------------
+    res = {}
+    for name, prec in (('split', 0), ('plain', 1)):
+        if name == 'plain' and args.no_plain:
+            continue
+        P_, D_ = poses0.clone(), disps0.clone()
+        out = ba_call(P_, D_, prec)
+        torch.cuda.synchronize()
+        res[name] = dict(ok=int(out[3].item()),
+                         rms=reprojection_rms(P_, D_, intr, tg, wt, ii, jj))
+    rms0 = reprojection_rms(poses0, disps0, intr, tg, wt, ii, jj)
+    log('reprojection rms start', rms0, res)
+
+    def time_path(dtype, precision):
+        f = fm32.to(dtype)
+        blk = hcorr.CorrBlock(f[:, p.ii], f[:, p.jj], dtype=dtype)   # volume build (untimed)
+        del f
+        poses, disps = poses0.clone(), disps0.clone()
+        stream = torch.cuda.current_stream()
+        ev = []
+
+        def step(record):
+            ba_call(poses, disps, precision)
+            for _ in range(iters):
+                if record:
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    blk(coords)
+                    e1.record(stream)
+                    ev.append((e0, e1))
+                else:
+                    blk(coords)
+
+        for _ in range(args.warmup):
+            step(False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(True)
+        torch.cuda.synchronize()
+        t_total = time.perf_counter() - t0
+        lookup_ms = [a.elapsed_time(b) for a, b in ev]
+        # BA alone
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            ba_call(poses, disps, precision)
+        torch.cuda.synchronize()
+        t_ba = time.perf_counter() - t1
+        del blk
+        torch.cuda.empty_cache()
+        return dict(t_total=t_total, t_ba=t_ba, lookup_ms=float(np.mean(lookup_ms)))
+
+    log('timing split path (bf16 volume, FP64 solve)')
+    split = time_path(torch.bfloat16, 0)
+    plain = None
+    if not args.no_plain:
+        log('timing plain path (fp32 volume, FP32 solve)')
+        plain = time_path(torch.float32, 1)
+
+    K = args.steps
+    value = iters * K / split['t_total']
+    lookup_s = split['lookup_ms'] / 1e3
+    achieved = LOOKUP_BYTES_BF16 * lookups / lookup_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, 'profiles', 'lookup_traffic.json')
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get('bytes_per_launch')
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, fm32, coords)
+
+    line = {
+        'metric': METRIC,
+        'value': round(value, 3),
+        'unit': 'GN it/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': args.warmup,
+        'ms_per_step': round(1e3 * split['t_total'] / K, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'bf16 corr / f32 jacobians+hessians / f64 reduced solve',
+        'data': 'synthetic (seeded random-walk poses, U(0.2,1) inverse depth, N(0,1)/sqrt(128) fmaps)',
+        'config': {'workload': 'global300: 300 keyframes 48x64, 3000 edges, ba(8 GN iterations, '
+                               't0=0, t1=300, lm=1e-4, ep=0.1) + bf16 4-level r=3 CorrBlock lookup '
+                               'of all 3000 edges per iteration',
+                   'edges': E, 'keyframes': int(p.poses.shape[0]), 'iterations_per_call': iters},
+        'ba_only_it_s': round(iters * K / split['t_ba'], 3),
+        'lookup_ms': round(split['lookup_ms'], 4),
+        'lookup_GBps_algorithmic': round(achieved, 1),
+        'reproj_rms_start_px': round(rms0, 6),
+        'reproj_rms_split_px': round(res['split']['rms'], 6),
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
+                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': traffic,
+                     'kernel': 'corr_lookup_fwd_kernel<bf16> (4 levels, 9.216M lookups/launch, '
+                               '912 B/lookup algorithmic)'},
+        'cpu_baseline': cpu,
+    }
+    if plain is not None:
+        pv = iters * K / plain['t_total']
+        line.update({
+            'plain_fp32_it_s': round(pv, 3),
+            'plain_fp32_ba_only_it_s': round(iters * K / plain['t_ba'], 3),
+            'plain_fp32_lookup_ms': round(plain['lookup_ms'], 4),
+            'speedup_vs_plain': round(value / pv, 3),
+            'reproj_rms_plain_px': round(res['plain']['rms'], 6),
+            'reproj_diff_px': round(abs(res['split']['rms'] - res['plain']['rms']), 7),
+        })
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(p, fm32, coords):
+    """The float64 oracle on the host: one full GN iteration of global300 plus the
+    lookup of 10 of the 3000 edges, scaled to one GN iteration + one full lookup."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    from oracle import ba as oba
+    from oracle import corr as oc
+    log('cpu baseline: oracle GN iteration')
+    t0 = time.perf_counter()
+    oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta, p.ii, p.jj, p.t0, p.t1, 1,
+           p.lm, p.ep)
+    t_ba = time.perf_counter() - t0
+    n = 10
+    fm = fm32[0].float().cpu().numpy()
+    f1 = fm[p.ii[:n]][None].astype(np.float64)
+    f2 = fm[p.jj[:n]][None].astype(np.float64)
+    pyr = oc.pyramid(oc.corr_allpairs(f1, f2), 4)
+    c = coords[:, :n].cpu().numpy().astype(np.float64)
+    log('cpu baseline: oracle lookup of', n, 'edges')
+    t1 = time.perf_counter()
+    oc.corrblock_lookup(pyr, c, 3)
+    t_lk = time.perf_counter() - t1
+    per_iter = t_ba + t_lk * len(p.ii) / n
+    return {'value': round(1.0 / per_iter, 6), 'unit': 'GN it/s', 'cores': 1, 'kind': 'port',
+            'sample': f'float64 numpy oracle: 1 GN iteration of global300 ({t_ba:.1f} s) + '
+                      f'CorrBlock lookup of {n}/3000 edges ({t_lk:.2f} s, scaled x{len(p.ii) // n}); '
+                      f'host {platform.processor() or platform.machine()}, os.cpu_count()={os.cpu_count()}'}
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,101 @@
+// Shared helpers for the gfx950 kernels: status plumbing, element conversions,
+// wave64 reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <string>
+
+#include "hsl2.h"
+
+namespace hsl2 {
+
+void set_error(const std::string& msg);
+
+#define HSL2_REQUIRE(cond, code, msg)          \
+  do {                                         \
+    if (!(cond)) {                             \
+      ::hsl2::set_error(msg);                  \
+      return (code);                           \
+    }                                          \
+  } while (0)
+
+#define HSL2_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::hsl2::set_error(std::string(#call " failed: ") + hipGetErrorString(e_));    \
+      return HSL2_ERR_HIP;                                                          \
+    }                                                                               \
+  } while (0)
+
+#define HSL2_LAUNCH_CHECK() HSL2_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kWave = 64;
+
+// ---- element <-> float -------------------------------------------------------
+template <typename T> __device__ __forceinline__ float to_f(T v);
+template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ float to_f<double>(double v) { return static_cast<float>(v); }
+template <> __device__ __forceinline__ float to_f<__half>(__half v) { return __half2float(v); }
+template <> __device__ __forceinline__ float to_f<__hip_bfloat16>(__hip_bfloat16 v) {
+  return __bfloat162float(v);
+}
+
+template <typename T> __device__ __forceinline__ T from_f(float v);
+template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ double from_f<double>(float v) { return v; }
+template <> __device__ __forceinline__ __half from_f<__half>(float v) { return __float2half(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 from_f<__hip_bfloat16>(float v) {
+  return __float2bfloat16(v);
+}
+
+// ---- wave64 reductions ---------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Reduce V per-thread values over a block of NT threads (NT a multiple of 64, <= 1024)
+// through LDS in chunks of 16 values; thread t < V ends up with no value -- instead the
+// results are written by the block to out[0..V) (plain stores).  `lds` must hold
+// 16 * NT floats.  Every thread of the block must call this.
+template <int V, int NT>
+__device__ void block_reduce_store(const float (&vals)[V], float* lds, float* out) {
+  constexpr int C = 16;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int c0 = 0; c0 < V; c0 += C) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+      if (c0 + c < V) lds[c * NT + t] = vals[c0 + c];
+    __syncthreads();
+    // NT threads: value c = t / (NT/16), segment = t % (NT/16)
+    constexpr int SEG = NT / C;                 // threads per value
+    const int c = t / SEG;
+    const int s = t % SEG;
+    float acc = 0.f;
+    if (c0 + c < V) {
+      for (int k = s; k < NT; k += SEG) acc += lds[c * NT + k];
+    }
+    // reduce SEG consecutive lanes (SEG is a power of two <= 64)
+#pragma unroll
+    for (int o = SEG / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (s == 0 && c0 + c < V) out[c0 + c] = acc;
+    __syncthreads();
+  }
+}
+
+}  // namespace hsl2

@@ -1,0 +1,314 @@
+// Radius-r bilinear lookup into a stored correlation volume, and its adjoint.
+//
+// Semantics of corr_index_forward_kernel / corr_index_backward_kernel
+// (src/correlation_kernels.cu:19-124): for pixel (y, x) of item n, with
+// (x0, y0) = coords[n, :, y, x], the (2r+2)^2 integer taps at
+// (floor(x0) - r + i, floor(y0) - r + j) of the pixel's own h2 x w2 plane are
+// bilinearly combined into (2r+1)^2 outputs ordered [a = x-offset][b = y-offset];
+// taps outside the plane are zero.
+//
+// Mapping (not the reference's one-thread-per-pixel): a 256-thread block owns 64
+// consecutive pixels.  Phase 1 gathers the 64 tap windows into LDS with lanes
+// running along window rows (contiguous in memory); phase 2 combines taps with
+// pixels on the fast lane axis, so the [a][b][pixel] output rows are written fully
+// coalesced.  Taps and sums are kept in float (double for f64 volumes) and rounded
+// once on store.
+#include "common.hpp"
+
+namespace hsl2 {
+namespace {
+
+constexpr int kPix = 64;      // pixels per block
+constexpr int kThreads = 256;
+
+template <typename T> struct Acc { using type = float; };
+template <> struct Acc<double> { using type = double; };
+
+template <typename T> __device__ __forceinline__ typename Acc<T>::type load_acc(const T* p) {
+  return static_cast<typename Acc<T>::type>(to_f<T>(*p));
+}
+template <> __device__ __forceinline__ double load_acc<double>(const double* p) { return *p; }
+
+template <typename T> __device__ __forceinline__ T store_acc(float v) { return from_f<T>(v); }
+template <typename T> __device__ __forceinline__ T store_acc(double v) { return static_cast<T>(v); }
+template <> __device__ __forceinline__ __half store_acc<__half>(double v) {
+  return __float2half(static_cast<float>(v));
+}
+template <> __device__ __forceinline__ __hip_bfloat16 store_acc<__hip_bfloat16>(double v) {
+  return __float2bfloat16(static_cast<float>(v));
+}
+
+// Up to four pyramid levels looked up by one launch (blockIdx.y = level).
+struct LevelTable {
+  const void* vol[4];
+  int h2[4], w2[4];
+  float scale[4];          // coordinate scale of the level (1 / 2^level)
+  int64_t out_offset[4];   // element offset of the level's first output channel
+};
+
+// Coordinates of pixel p of item n: x = coords[n * cb + p * cp], y = x-address + cy.
+// Output: out + n * out_batch_stride + level offset + o * hw1 + pix, o = a * (2r+1) + b.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
+                       int64_t cy_off, T* __restrict__ out, int hw1, int r,
+                       int64_t out_batch_stride, int tiles_per_item) {
+  using A = typename Acc<T>::type;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  A* taps = reinterpret_cast<A*>(smem_raw);                 // [kPix][D*D]
+  __shared__ int s_x0[kPix], s_y0[kPix];
+  __shared__ float s_dx[kPix], s_dy[kPix];
+
+  const int level = blockIdx.y;
+  const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
+  const int h2 = lv.h2[level], w2 = lv.w2[level];
+  const float scale = lv.scale[level];
+  const int n = blockIdx.x / tiles_per_item;
+  const int pix0 = (blockIdx.x % tiles_per_item) * kPix;
+  const int D = 2 * r + 2;
+  const int D2 = D * D;
+  const int npix = min(kPix, hw1 - pix0);
+  const int t = threadIdx.x;
+
+  if (t < kPix) {
+    int x0 = 0, y0 = 0;
+    float fx = 0.f, fy = 0.f;
+    if (t < npix) {
+      const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
+      const float cx = cp_[0] * scale;
+      const float cy = cp_[cy_off] * scale;
+      const float flx = floorf(cx), fly = floorf(cy);
+      x0 = static_cast<int>(flx) - r;
+      y0 = static_cast<int>(fly) - r;
+      fx = cx - flx;
+      fy = cy - fly;
+    }
+    s_x0[t] = x0;
+    s_y0[t] = y0;
+    s_dx[t] = fx;
+    s_dy[t] = fy;
+  }
+  __syncthreads();
+
+  const int64_t plane = (int64_t)h2 * w2;
+  const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
+  for (int idx = t; idx < npix * D2; idx += kThreads) {
+    const int p = idx / D2;
+    const int tap = idx - p * D2;
+    const int ty = tap / D;
+    const int tx = tap - ty * D;
+    const int y1 = s_y0[p] + ty;
+    const int x1 = s_x0[p] + tx;
+    A v = 0;
+    if (y1 >= 0 && y1 < h2 && x1 >= 0 && x1 < w2) v = load_acc<T>(vbase + p * plane + y1 * w2 + x1);
+    taps[p * D2 + tap] = v;
+  }
+  __syncthreads();
+
+  const int R = 2 * r + 1;
+  T* obase = out + (int64_t)n * out_batch_stride + lv.out_offset[level] + pix0;
+  for (int idx = t; idx < npix * R * R; idx += kThreads) {
+    const int o = idx / npix;
+    const int p = idx - o * npix;
+    const int a = o / R;        // x offset
+    const int b = o - a * R;    // y offset
+    const A fx = s_dx[p], fy = s_dy[p];
+    const A* tp = taps + p * D2;
+    const A v = (1 - fx) * (1 - fy) * tp[b * D + a] + fx * (1 - fy) * tp[b * D + a + 1] +
+                (1 - fx) * fy * tp[(b + 1) * D + a] + fx * fy * tp[(b + 1) * D + a + 1];
+    obase[(int64_t)o * hw1 + p] = store_acc<T>(v);
+  }
+}
+
+// Adjoint: writes the complete volume_grad planes of the block's 64 pixels.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+corr_lookup_bwd_kernel(const float* __restrict__ coords, const T* __restrict__ grad,
+                       T* __restrict__ vgrad, int hw1, int h2, int w2, int r,
+                       int tiles_per_item) {
+  using A = typename Acc<T>::type;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  A* gt = reinterpret_cast<A*>(smem_raw);                   // [kPix][D*D]
+  __shared__ int s_x0[kPix], s_y0[kPix];
+  __shared__ float s_dx[kPix], s_dy[kPix];
+
+  const int n = blockIdx.x / tiles_per_item;
+  const int pix0 = (blockIdx.x % tiles_per_item) * kPix;
+  const int D = 2 * r + 2;
+  const int D2 = D * D;
+  const int R = 2 * r + 1;
+  const int npix = min(kPix, hw1 - pix0);
+  const int t = threadIdx.x;
+
+  if (t < kPix) {
+    int x0 = 0, y0 = 0;
+    float fx = 0.f, fy = 0.f;
+    if (t < npix) {
+      const float cx = coords[(int64_t)n * 2 * hw1 + pix0 + t];
+      const float cy = coords[(int64_t)n * 2 * hw1 + hw1 + pix0 + t];
+      const float flx = floorf(cx), fly = floorf(cy);
+      x0 = static_cast<int>(flx) - r;
+      y0 = static_cast<int>(fly) - r;
+      fx = cx - flx;
+      fy = cy - fly;
+    }
+    s_x0[t] = x0;
+    s_y0[t] = y0;
+    s_dx[t] = fx;
+    s_dy[t] = fy;
+  }
+  __syncthreads();
+
+  // tap gradients: tap (ty, tx) collects the up-to-four outputs that read it
+  const T* gbase = grad + (int64_t)n * R * R * hw1 + pix0;
+  for (int idx = t; idx < npix * D2; idx += kThreads) {
+    const int p = idx % npix;
+    const int tap = idx / npix;
+    const int ty = tap / D;
+    const int tx = tap - ty * D;
+    const A fx = s_dx[p], fy = s_dy[p];
+    A g = 0;
+    // output (a, b) reads tap (a + i, b + j) with weight w_ij
+    if (tx < R && ty < R) g += (1 - fx) * (1 - fy) * load_acc<T>(gbase + (int64_t)(tx * R + ty) * hw1 + p);
+    if (tx > 0 && ty < R) g += fx * (1 - fy) * load_acc<T>(gbase + (int64_t)((tx - 1) * R + ty) * hw1 + p);
+    if (tx < R && ty > 0) g += (1 - fx) * fy * load_acc<T>(gbase + (int64_t)(tx * R + ty - 1) * hw1 + p);
+    if (tx > 0 && ty > 0) g += fx * fy * load_acc<T>(gbase + (int64_t)((tx - 1) * R + ty - 1) * hw1 + p);
+    gt[p * D2 + tap] = g;
+  }
+  __syncthreads();
+
+  const int64_t plane = (int64_t)h2 * w2;
+  T* vbase = vgrad + ((int64_t)n * hw1 + pix0) * plane;
+  for (int p = 0; p < npix; ++p) {
+    const int xs = s_x0[p], ys = s_y0[p];
+    T* pl = vbase + p * plane;
+    for (int e = t; e < plane; e += kThreads) {
+      const int y2 = e / w2;
+      const int x2 = e - y2 * w2;
+      const int ty = y2 - ys, tx = x2 - xs;
+      A v = 0;
+      if (ty >= 0 && ty < D && tx >= 0 && tx < D) v = gt[p * D2 + ty * D + tx];
+      pl[e] = store_acc<T>(v);
+    }
+  }
+}
+
+template <typename T>
+int launch_fwd(const LevelTable& lv, int levels, const float* coords, int64_t cb, int cp,
+               int64_t cy_off, void* corr, int64_t batch, int64_t hw1, int r,
+               int64_t out_batch_stride, hipStream_t s) {
+  const int tiles = static_cast<int>((hw1 + kPix - 1) / kPix);
+  const int64_t blocks = batch * tiles;
+  HSL2_REQUIRE(blocks < (1ll << 31), HSL2_ERR_SHAPE, "corr lookup: grid too large");
+  const int D = 2 * r + 2;
+  const size_t lds = sizeof(typename Acc<T>::type) * kPix * D * D;
+  HSL2_REQUIRE(lds <= 64 * 1024, HSL2_ERR_UNSUPPORTED, "corr lookup: radius too large");
+  if (blocks == 0) return HSL2_OK;
+  hipLaunchKernelGGL(corr_lookup_fwd_kernel<T>, dim3(static_cast<unsigned>(blocks), levels),
+                     dim3(kThreads), lds, s, lv, coords, cb, cp, cy_off, static_cast<T*>(corr),
+                     static_cast<int>(hw1), r, out_batch_stride, tiles);
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+template <typename T>
+int launch_bwd(const float* coords, const void* grad, void* vgrad, int64_t batch, int64_t hw1,
+               int64_t h2, int64_t w2, int r, hipStream_t s) {
+  const int tiles = static_cast<int>((hw1 + kPix - 1) / kPix);
+  const int64_t blocks = batch * tiles;
+  HSL2_REQUIRE(blocks < (1ll << 31), HSL2_ERR_SHAPE, "corr lookup: grid too large");
+  const int D = 2 * r + 2;
+  const size_t lds = sizeof(typename Acc<T>::type) * kPix * D * D;
+  HSL2_REQUIRE(lds <= 64 * 1024, HSL2_ERR_UNSUPPORTED, "corr lookup: radius too large");
+  if (blocks == 0) return HSL2_OK;
+  hipLaunchKernelGGL(corr_lookup_bwd_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads),
+                     lds, s, coords, static_cast<const T*>(grad), static_cast<T*>(vgrad),
+                     static_cast<int>(hw1), static_cast<int>(h2), static_cast<int>(w2), r, tiles);
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+int check_lookup_args(const void* a, const void* b, const void* c, int64_t batch, int64_t h1,
+                      int64_t w1, int64_t h2, int64_t w2, int radius) {
+  HSL2_REQUIRE(a && b && c, HSL2_ERR_ARGUMENT, "corr lookup: null pointer");
+  HSL2_REQUIRE(batch >= 0 && h1 >= 0 && w1 >= 0 && h2 > 0 && w2 > 0, HSL2_ERR_SHAPE,
+               "corr lookup: bad sizes");
+  HSL2_REQUIRE(radius >= 0 && radius <= 8, HSL2_ERR_ARGUMENT, "corr lookup: radius must be in [0, 8]");
+  HSL2_REQUIRE(h1 * w1 < (1ll << 31) && h2 * w2 < (1ll << 31), HSL2_ERR_SHAPE,
+               "corr lookup: plane too large");
+  return HSL2_OK;
+}
+
+}  // namespace
+
+int corr_lookup_fwd(const LevelTable& lv, int levels, const float* coords, int64_t cb, int cp,
+                    int64_t cy_off, void* corr, int dtype, int64_t batch, int64_t hw1, int radius,
+                    int64_t out_batch_stride, hipStream_t s) {
+  switch (dtype) {
+    case HSL2_F32: return launch_fwd<float>(lv, levels, coords, cb, cp, cy_off, corr, batch, hw1, radius, out_batch_stride, s);
+    case HSL2_F16: return launch_fwd<__half>(lv, levels, coords, cb, cp, cy_off, corr, batch, hw1, radius, out_batch_stride, s);
+    case HSL2_BF16: return launch_fwd<__hip_bfloat16>(lv, levels, coords, cb, cp, cy_off, corr, batch, hw1, radius, out_batch_stride, s);
+    case HSL2_F64: return launch_fwd<double>(lv, levels, coords, cb, cp, cy_off, corr, batch, hw1, radius, out_batch_stride, s);
+    default: set_error("corr lookup: unknown dtype"); return HSL2_ERR_ARGUMENT;
+  }
+}
+
+}  // namespace hsl2
+
+using namespace hsl2;
+
+extern "C" int hsl2_corr_index_forward(const void* volume, const float* coords, void* corr,
+                                       int dtype, int64_t batch, int64_t h1, int64_t w1,
+                                       int64_t h2, int64_t w2, int radius, void* stream) {
+  int st = check_lookup_args(volume, coords, corr, batch, h1, w1, h2, w2, radius);
+  if (st) return st;
+  const int64_t R = 2 * radius + 1;
+  LevelTable lv{};
+  lv.vol[0] = volume;
+  lv.h2[0] = static_cast<int>(h2);
+  lv.w2[0] = static_cast<int>(w2);
+  lv.scale[0] = 1.0f;
+  lv.out_offset[0] = 0;
+  // planar coords [batch][2][h1*w1]
+  return corr_lookup_fwd(lv, 1, coords, 2 * h1 * w1, 1, h1 * w1, corr, dtype, batch, h1 * w1, radius,
+                         R * R * h1 * w1, as_stream(stream));
+}
+
+extern "C" int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels,
+                                        const float* coords, void* corr, int dtype, int64_t batch,
+                                        int64_t h1, int64_t w1, int radius, void* stream) {
+  HSL2_REQUIRE(levels && num_levels >= 1 && num_levels <= 4, HSL2_ERR_ARGUMENT,
+               "pyramid lookup: 1..4 levels");
+  LevelTable lv{};
+  const int64_t R = 2 * radius + 1;
+  for (int i = 0; i < num_levels; ++i) {
+    const int64_t h2 = h1 >> i, w2 = w1 >> i;
+    int st = check_lookup_args(levels[i], coords, corr, batch, h1, w1, h2, w2, radius);
+    if (st) return st;
+    lv.vol[i] = levels[i];
+    lv.h2[i] = static_cast<int>(h2);
+    lv.w2[i] = static_cast<int>(w2);
+    lv.scale[i] = 1.0f / static_cast<float>(1 << i);
+    lv.out_offset[i] = i * R * R * h1 * w1;
+  }
+  // interleaved coords [batch][h1*w1][2]
+  return corr_lookup_fwd(lv, num_levels, coords, 2 * h1 * w1, 2, 1, corr, dtype, batch, h1 * w1,
+                         radius, num_levels * R * R * h1 * w1, as_stream(stream));
+}
+
+extern "C" int hsl2_corr_index_backward(const float* coords, const void* corr_grad,
+                                        void* volume_grad, int dtype, int64_t batch, int64_t h1,
+                                        int64_t w1, int64_t h2, int64_t w2, int radius,
+                                        void* stream) {
+  int st = check_lookup_args(coords, corr_grad, volume_grad, batch, h1, w1, h2, w2, radius);
+  if (st) return st;
+  const int64_t hw1 = h1 * w1;
+  hipStream_t s = as_stream(stream);
+  switch (dtype) {
+    case HSL2_F32: return launch_bwd<float>(coords, corr_grad, volume_grad, batch, hw1, h2, w2, radius, s);
+    case HSL2_F16: return launch_bwd<__half>(coords, corr_grad, volume_grad, batch, hw1, h2, w2, radius, s);
+    case HSL2_BF16: return launch_bwd<__hip_bfloat16>(coords, corr_grad, volume_grad, batch, hw1, h2, w2, radius, s);
+    case HSL2_F64: return launch_bwd<double>(coords, corr_grad, volume_grad, batch, hw1, h2, w2, radius, s);
+    default: set_error("corr lookup: unknown dtype"); return HSL2_ERR_ARGUMENT;
+  }
+}

@@ -1,0 +1,360 @@
+// droid_backends: the torch extension module the reference's Python code imports
+// (hislam2/geom/ba.py:4, hislam2/depth_video.py:3, hislam2/modules/corr.py:4),
+// rebuilt over the C ABI of libhsl2 (hsl2.h).  Same function names, positional
+// argument order and return conventions as src/droid.cpp:141-168, 254-305, 325-341.
+//
+// This layer owns validation (device, dtype, shape, contiguity), allocation through
+// the torch caching allocator, and the current HIP stream; the kernels live behind
+// the C ABI.  A non-contiguous input raises RuntimeError exactly like the
+// reference's CHECK_INPUT; inconsistent shapes and out-of-range edge indices, which
+// are undefined behaviour in the reference, raise ValueError.
+#include <torch/extension.h>
+
+#include <ATen/hip/impl/HIPGuardImplMasqueradingAsCUDA.h>
+#include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
+
+#include <string>
+#include <vector>
+
+#include "hsl2.h"
+
+namespace py = pybind11;
+
+namespace {
+
+#define CHECK_CONTIGUOUS(x) TORCH_CHECK((x).is_contiguous(), #x " must be contiguous")
+
+void*current_stream() {
+  return reinterpret_cast<void*>(c10::hip::getCurrentHIPStreamMasqueradingAsCUDA().stream());
+}
+
+void raise_status(int st, const char* what) {
+  if (st == HSL2_OK) return;
+  std::string msg = std::string(what) + ": " + hsl2_status_string(st) + ": " + hsl2_last_error();
+  if (st == HSL2_ERR_ARGUMENT || st == HSL2_ERR_SHAPE || st == HSL2_ERR_INDEX) throw py::value_error(msg);
+  if (st == HSL2_ERR_UNSUPPORTED) throw py::type_error(msg);
+  throw std::runtime_error(msg);
+}
+
+void require(bool cond, const std::string& msg) {
+  if (!cond) throw py::value_error(msg);
+}
+
+void require_device(const torch::Tensor& t, const char* name) {
+  require(t.is_cuda(), std::string(name) + " must be a GPU tensor");
+}
+
+int dtype_code(const torch::Tensor& t) {
+  switch (t.scalar_type()) {
+    case torch::kFloat32: return HSL2_F32;
+    case torch::kFloat16: return HSL2_F16;
+    case torch::kBFloat16: return HSL2_BF16;
+    case torch::kFloat64: return HSL2_F64;
+    default: throw py::type_error("unsupported dtype");
+  }
+}
+
+// ---- ba (src/droid.cpp:141-168) ------------------------------------------------
+std::vector<torch::Tensor> ba_impl(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                   torch::Tensor targets, torch::Tensor weights, torch::Tensor eta,
+                                   torch::Tensor ii, torch::Tensor jj, int t0, int t1, int iterations,
+                                   double lm, double ep, bool motion_only, int precision,
+                                   bool want_status) {
+  CHECK_CONTIGUOUS(targets);
+  CHECK_CONTIGUOUS(weights);
+  CHECK_CONTIGUOUS(poses);
+  CHECK_CONTIGUOUS(disps);
+  CHECK_CONTIGUOUS(intrinsics);
+  CHECK_CONTIGUOUS(ii);
+  CHECK_CONTIGUOUS(jj);
+  for (auto* t : {&poses, &disps, &intrinsics, &targets, &weights}) {
+    require_device(*t, "ba tensors");
+    require(t->scalar_type() == torch::kFloat32, "ba: poses, disps, intrinsics, targets, weights must be float32");
+  }
+  require(poses.dim() == 2 && poses.size(1) == 7, "ba: poses must be [N, 7]");
+  require(disps.dim() == 3 && disps.size(0) == poses.size(0), "ba: disps must be [N, ht, wd]");
+  require(intrinsics.numel() >= 4, "ba: intrinsics must hold fx, fy, cx, cy");
+  require(ii.dim() == 1 && jj.dim() == 1 && ii.size(0) == jj.size(0), "ba: ii, jj must be [E]");
+  require(ii.scalar_type() == torch::kInt64 && jj.scalar_type() == torch::kInt64, "ba: ii, jj must be int64");
+  const int64_t E = ii.size(0), ht = disps.size(1), wd = disps.size(2);
+  require(targets.numel() == E * 2 * ht * wd && weights.numel() == E * 2 * ht * wd,
+          "ba: targets and weights must be [E, 2, ht, wd]");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+
+  // one host copy of the edge list per call (the reference copies it 16+ times per iteration)
+  torch::Tensor ii_h = ii.to(torch::kCPU), jj_h = jj.to(torch::kCPU);
+  hsl2_ba_plan* plan = nullptr;
+  raise_status(hsl2_ba_plan_create(ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E, poses.size(0),
+                                   ht, wd, t0, t1, motion_only ? 1 : 0, precision, &plan),
+               "ba");
+  std::unique_ptr<hsl2_ba_plan, void (*)(hsl2_ba_plan*)> guard_plan(plan, hsl2_ba_plan_destroy);
+  const int64_t F = hsl2_ba_plan_num_frames(plan);
+  int64_t eta_rows = 1;
+  const float* eta_ptr = nullptr;
+  torch::Tensor eta_c;
+  if (!motion_only) {
+    require_device(eta, "eta");
+    require(eta.scalar_type() == torch::kFloat32, "ba: eta must be float32");
+    eta_c = eta.contiguous();
+    require(eta_c.numel() % (ht * wd) == 0, "ba: eta must hold whole [ht, wd] rows");
+    eta_rows = eta_c.numel() / (ht * wd);
+    require(eta_rows == F || eta_rows == 1,
+            "ba: eta must have one [ht, wd] row per unique frame of [t0, t1) ++ ii (" + std::to_string(F) +
+                ") or a single row");
+    eta_ptr = eta_c.data_ptr<float>();
+  }
+  auto opts = poses.options();
+  torch::Tensor ws = torch::empty({static_cast<int64_t>(hsl2_ba_plan_workspace_bytes(plan)) + 256},
+                                  opts.dtype(torch::kUInt8));
+  uintptr_t base = reinterpret_cast<uintptr_t>(ws.data_ptr());
+  void* ws_ptr = reinterpret_cast<void*>((base + 255) & ~uintptr_t(255));
+  torch::Tensor dx = torch::zeros({t1 - t0, 6}, opts);
+  torch::Tensor dz = motion_only ? torch::Tensor() : torch::zeros({F, ht * wd}, opts);
+  int ok = 1;
+  raise_status(hsl2_ba_run(plan, poses.data_ptr<float>(), disps.data_ptr<float>(),
+                           intrinsics.data_ptr<float>(), targets.data_ptr<float>(),
+                           weights.data_ptr<float>(), eta_ptr, eta_rows, iterations,
+                           static_cast<float>(lm), static_cast<float>(ep), dx.data_ptr<float>(),
+                           motion_only ? nullptr : dz.data_ptr<float>(),
+                           ws_ptr, static_cast<size_t>(ws.numel() - 256), want_status ? &ok : nullptr,
+                           current_stream()),
+               "ba");
+  std::vector<torch::Tensor> out = {iterations > 0 ? dx : torch::Tensor(),
+                                    iterations > 0 ? dz : torch::Tensor(), torch::Tensor()};
+  if (want_status) out.push_back(torch::tensor({ok}, torch::kInt32));
+  return out;
+}
+
+std::vector<torch::Tensor> ba(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                              torch::Tensor targets, torch::Tensor weights, torch::Tensor eta,
+                              torch::Tensor ii, torch::Tensor jj, int t0, int t1, int iterations,
+                              double lm, double ep, bool motion_only) {
+  return ba_impl(poses, disps, intrinsics, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
+                 motion_only, HSL2_BA_SPLIT, false);
+}
+
+std::vector<torch::Tensor> ba_ex(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                 torch::Tensor targets, torch::Tensor weights, torch::Tensor eta,
+                                 torch::Tensor ii, torch::Tensor jj, int t0, int t1, int iterations,
+                                 double lm, double ep, bool motion_only, int precision) {
+  return ba_impl(poses, disps, intrinsics, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
+                 motion_only, precision, true);
+}
+
+// kx and kk_exp of a ba call, computed by the library's planner (host only)
+std::vector<torch::Tensor> ba_index(torch::Tensor ii, torch::Tensor jj, int t0, int t1, int64_t num_poses) {
+  torch::Tensor ii_h = ii.to(torch::kCPU).to(torch::kInt64).contiguous();
+  torch::Tensor jj_h = jj.to(torch::kCPU).to(torch::kInt64).contiguous();
+  require(ii_h.numel() == jj_h.numel(), "ba_index: ii and jj differ in length");
+  hsl2_ba_plan* plan = nullptr;
+  raise_status(hsl2_ba_plan_create(ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), ii_h.numel(),
+                                   num_poses, 1, 1, t0, t1, 0, HSL2_BA_SPLIT, &plan),
+               "ba_index");
+  std::unique_ptr<hsl2_ba_plan, void (*)(hsl2_ba_plan*)> guard_plan(plan, hsl2_ba_plan_destroy);
+  torch::Tensor kx = torch::empty({hsl2_ba_plan_num_frames(plan)}, torch::kInt64);
+  torch::Tensor kk = torch::empty({(t1 - t0) + ii_h.numel()}, torch::kInt64);
+  raise_status(hsl2_ba_plan_frames(plan, kx.data_ptr<int64_t>()), "ba_index");
+  raise_status(hsl2_ba_plan_inverse(plan, kk.data_ptr<int64_t>()), "ba_index");
+  return {kx, kk};
+}
+
+// ---- corr_index (src/droid.cpp:254-275) ----------------------------------------------
+std::vector<torch::Tensor> corr_index_forward(torch::Tensor volume, torch::Tensor coords, int radius) {
+  CHECK_CONTIGUOUS(volume);
+  CHECK_CONTIGUOUS(coords);
+  require_device(volume, "volume");
+  require_device(coords, "coords");
+  require(volume.dim() == 5, "corr_index_forward: volume must be [B, h1, w1, h2, w2]");
+  require(coords.scalar_type() == torch::kFloat32, "corr_index_forward: coords must be float32");
+  const int64_t B = volume.size(0), h1 = volume.size(1), w1 = volume.size(2);
+  require(coords.dim() == 4 && coords.size(0) == B && coords.size(1) == 2 && coords.size(2) == h1 &&
+              coords.size(3) == w1,
+          "corr_index_forward: coords must be [B, 2, h1, w1]");
+  require(radius >= 0, "corr_index_forward: negative radius");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(volume.device());
+  torch::Tensor corr = torch::empty({B, 2 * radius + 1, 2 * radius + 1, h1, w1}, volume.options());
+  raise_status(hsl2_corr_index_forward(volume.data_ptr(), coords.data_ptr<float>(), corr.data_ptr(),
+                                       dtype_code(volume), B, h1, w1, volume.size(3), volume.size(4),
+                                       radius, current_stream()),
+               "corr_index_forward");
+  return {corr};
+}
+
+std::vector<torch::Tensor> corr_index_backward(torch::Tensor volume, torch::Tensor coords,
+                                               torch::Tensor corr_grad, int radius) {
+  CHECK_CONTIGUOUS(volume);
+  CHECK_CONTIGUOUS(coords);
+  CHECK_CONTIGUOUS(corr_grad);
+  require_device(volume, "volume");
+  require(volume.dim() == 5, "corr_index_backward: volume must be [B, h1, w1, h2, w2]");
+  const int64_t B = volume.size(0), h1 = volume.size(1), w1 = volume.size(2);
+  require(coords.scalar_type() == torch::kFloat32 && coords.numel() == B * 2 * h1 * w1,
+          "corr_index_backward: coords must be float32 [B, 2, h1, w1]");
+  require(corr_grad.scalar_type() == volume.scalar_type() &&
+              corr_grad.numel() == B * (2 * radius + 1) * (2 * radius + 1) * h1 * w1,
+          "corr_index_backward: corr_grad must match the forward output");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(volume.device());
+  torch::Tensor vgrad = torch::empty_like(volume);
+  raise_status(hsl2_corr_index_backward(coords.data_ptr<float>(), corr_grad.data_ptr(), vgrad.data_ptr(),
+                                        dtype_code(volume), B, h1, w1, volume.size(3), volume.size(4),
+                                        radius, current_stream()),
+               "corr_index_backward");
+  return {vgrad};
+}
+
+// Fused CorrBlock.__call__ (hislam2/modules/corr.py:40-50): all levels, one launch.
+torch::Tensor corr_pyramid_lookup(std::vector<torch::Tensor> levels, torch::Tensor coords, int radius) {
+  CHECK_CONTIGUOUS(coords);
+  require(!levels.empty() && levels.size() <= 4, "corr_pyramid_lookup: 1..4 levels");
+  require(coords.dim() == 5 && coords.size(4) == 2 && coords.scalar_type() == torch::kFloat32,
+          "corr_pyramid_lookup: coords must be float32 [B, N, h, w, 2]");
+  const int64_t B = coords.size(0), N = coords.size(1), h = coords.size(2), w = coords.size(3);
+  std::vector<const void*> ptrs;
+  for (size_t i = 0; i < levels.size(); ++i) {
+    CHECK_CONTIGUOUS(levels[i]);
+    require_device(levels[i], "levels");
+    require(levels[i].scalar_type() == levels[0].scalar_type(), "corr_pyramid_lookup: mixed level dtypes");
+    require(levels[i].dim() == 5 && levels[i].size(0) == B * N && levels[i].size(1) == h &&
+                levels[i].size(2) == w && levels[i].size(3) == (h >> i) && levels[i].size(4) == (w >> i),
+            "corr_pyramid_lookup: level " + std::to_string(i) + " must be [B*N, h, w, h>>i, w>>i]");
+    ptrs.push_back(levels[i].data_ptr());
+  }
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(coords.device());
+  const int64_t R = 2 * radius + 1;
+  torch::Tensor out = torch::empty({B, N, static_cast<int64_t>(levels.size()) * R * R, h, w},
+                                   levels[0].options());
+  raise_status(hsl2_corr_pyramid_lookup(ptrs.data(), static_cast<int>(levels.size()), coords.data_ptr<float>(),
+                                        out.data_ptr(), dtype_code(levels[0]), B * N, h, w, radius,
+                                        current_stream()),
+               "corr_pyramid_lookup");
+  return out;
+}
+
+// ---- altcorr (src/droid.cpp:277-305) ---------------------------------------------------
+struct AltDims {
+  int64_t B, N1, N2, C, H, W, H2, W2, M;
+};
+
+AltDims alt_dims(const torch::Tensor& fmap1, const torch::Tensor& fmap2, const torch::Tensor& coords,
+                 const torch::Tensor& ii, const torch::Tensor& jj) {
+  require(fmap1.dim() == 5 && fmap2.dim() == 5, "altcorr: fmaps must be [B, N, C, H, W]");
+  require(coords.dim() == 5 && coords.size(2) == 2, "altcorr: coords must be [B, M, 2, H, W]");
+  require(fmap1.scalar_type() == fmap2.scalar_type(), "altcorr: fmap dtypes differ");
+  require(coords.scalar_type() == torch::kFloat32, "altcorr: coords must be float32");
+  AltDims d{fmap1.size(0), fmap1.size(1), fmap2.size(1), fmap1.size(2), fmap1.size(3), fmap1.size(4),
+            fmap2.size(3), fmap2.size(4), coords.size(1)};
+  require(fmap2.size(0) == d.B && fmap2.size(2) == d.C && coords.size(0) == d.B && coords.size(3) == d.H &&
+              coords.size(4) == d.W,
+          "altcorr: inconsistent shapes");
+  require(ii.numel() == d.M && jj.numel() == d.M && ii.scalar_type() == torch::kInt64 &&
+              jj.scalar_type() == torch::kInt64 && ii.is_cuda() && jj.is_cuda(),
+          "altcorr: ii, jj must be int64 GPU tensors with one entry per edge");
+  return d;
+}
+
+torch::Tensor to_channels_last(const torch::Tensor& f, const AltDims&) {
+  torch::Tensor out = torch::empty_like(f);
+  raise_status(hsl2_channels_last(f.data_ptr(), out.data_ptr(), dtype_code(f), f.size(0) * f.size(1), f.size(2),
+                                  f.size(3) * f.size(4), current_stream()),
+               "altcorr");
+  return out;
+}
+
+std::vector<torch::Tensor> altcorr_forward(torch::Tensor fmap1, torch::Tensor fmap2, torch::Tensor coords,
+                                           torch::Tensor ii, torch::Tensor jj, int radius) {
+  CHECK_CONTIGUOUS(fmap1);
+  CHECK_CONTIGUOUS(fmap2);
+  CHECK_CONTIGUOUS(coords);
+  require_device(fmap1, "fmap1");
+  const AltDims d = alt_dims(fmap1, fmap2, coords, ii, jj);
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(fmap1.device());
+  torch::Tensor ii_c = ii.contiguous(), jj_c = jj.contiguous();
+  torch::Tensor f1 = to_channels_last(fmap1, d), f2 = to_channels_last(fmap2, d);
+  torch::Tensor corr = torch::empty({d.B, d.M, 2 * radius + 1, 2 * radius + 1, d.H, d.W}, fmap1.options());
+  raise_status(hsl2_altcorr_forward(f1.data_ptr(), f2.data_ptr(), coords.data_ptr<float>(),
+                                    ii_c.data_ptr<int64_t>(), jj_c.data_ptr<int64_t>(), corr.data_ptr(),
+                                    dtype_code(fmap1), d.B, d.N1, d.N2, d.C, d.H, d.W, d.H2, d.W2, d.M,
+                                    radius, current_stream()),
+               "altcorr_forward");
+  return {corr};
+}
+
+std::vector<torch::Tensor> altcorr_backward(torch::Tensor fmap1, torch::Tensor fmap2, torch::Tensor coords,
+                                            torch::Tensor corr_grad, torch::Tensor ii, torch::Tensor jj,
+                                            int radius) {
+  CHECK_CONTIGUOUS(fmap1);
+  CHECK_CONTIGUOUS(fmap2);
+  CHECK_CONTIGUOUS(coords);
+  CHECK_CONTIGUOUS(corr_grad);
+  require_device(fmap1, "fmap1");
+  const AltDims d = alt_dims(fmap1, fmap2, coords, ii, jj);
+  require(corr_grad.numel() == d.B * d.M * (2 * radius + 1) * (2 * radius + 1) * d.H * d.W,
+          "altcorr_backward: corr_grad must match the forward output");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(fmap1.device());
+  torch::Tensor g = corr_grad.to(torch::kFloat32).contiguous();
+  torch::Tensor ii_c = ii.contiguous(), jj_c = jj.contiguous();
+  torch::Tensor f1 = to_channels_last(fmap1, d), f2 = to_channels_last(fmap2, d);
+  auto fopts = fmap1.options().dtype(torch::kFloat32);
+  torch::Tensor g1 = torch::empty({d.B, d.N1, d.H, d.W, d.C}, fopts);
+  torch::Tensor g2 = torch::empty({d.B, d.N2, d.H2, d.W2, d.C}, fopts);
+  raise_status(hsl2_altcorr_backward(f1.data_ptr(), f2.data_ptr(), coords.data_ptr<float>(),
+                                     g.data_ptr<float>(), ii_c.data_ptr<int64_t>(), jj_c.data_ptr<int64_t>(),
+                                     g1.data_ptr<float>(), g2.data_ptr<float>(), dtype_code(fmap1), d.B, d.N1,
+                                     d.N2, d.C, d.H, d.W, d.H2, d.W2, d.M, radius, current_stream()),
+               "altcorr_backward");
+  torch::Tensor o1 = torch::empty({d.B, d.N1, d.C, d.H, d.W}, fopts);
+  torch::Tensor o2 = torch::empty({d.B, d.N2, d.C, d.H2, d.W2}, fopts);
+  raise_status(hsl2_channels_first_f32(g1.data_ptr<float>(), o1.data_ptr<float>(), d.B * d.N1, d.C, d.H * d.W,
+                                       current_stream()),
+               "altcorr_backward");
+  raise_status(hsl2_channels_first_f32(g2.data_ptr<float>(), o2.data_ptr<float>(), d.B * d.N2, d.C,
+                                       d.H2 * d.W2, current_stream()),
+               "altcorr_backward");
+  return {o1.to(fmap1.scalar_type()), o2.to(fmap2.scalar_type())};
+}
+
+// ---- correlation build (hislam2/modules/corr.py:24-38, 63-71) ---------------------------
+std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, int num_levels,
+                                      py::object out_dtype) {
+  CHECK_CONTIGUOUS(fmap1);
+  CHECK_CONTIGUOUS(fmap2);
+  require_device(fmap1, "fmap1");
+  require(fmap1.dim() == 5 && fmap1.sizes() == fmap2.sizes() && fmap1.scalar_type() == fmap2.scalar_type(),
+          "corr_build: fmaps must both be [B, N, C, h, w] of one dtype");
+  torch::ScalarType od = out_dtype.is_none() ? fmap1.scalar_type()
+                                             : torch::python::detail::py_object_to_dtype(out_dtype);
+  const int64_t B = fmap1.size(0), N = fmap1.size(1), C = fmap1.size(2), h = fmap1.size(3), w = fmap1.size(4);
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(fmap1.device());
+  std::vector<torch::Tensor> levels;
+  std::vector<void*> ptrs;
+  for (int i = 0; i < num_levels; ++i) {
+    levels.push_back(torch::empty({B * N, h, w, h >> i, w >> i}, fmap1.options().dtype(od)));
+    ptrs.push_back(levels.back().data_ptr());
+  }
+  int odc = od == torch::kBFloat16 ? HSL2_BF16 : od == torch::kFloat32 ? HSL2_F32 : od == torch::kFloat16 ? HSL2_F16 : -1;
+  raise_status(hsl2_corr_build(fmap1.data_ptr(), fmap2.data_ptr(), dtype_code(fmap1), ptrs.data(), odc, B * N, C,
+                               h, w, num_levels, current_stream()),
+               "corr_build");
+  return levels;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+  m.doc() = "MI355X (gfx950) dense-tracking kernels behind the droid_backends interface";
+  // bundle adjustment (src/droid.cpp:327)
+  m.def("ba", &ba, "dense bundle adjustment (split precision: FP32 linearization, FP64 reduced solve)");
+  // correlation volume kernels (src/droid.cpp:337-340)
+  m.def("altcorr_forward", &altcorr_forward, "on-the-fly correlation lookup");
+  m.def("altcorr_backward", &altcorr_backward, "on-the-fly correlation lookup adjoint");
+  m.def("corr_index_forward", &corr_index_forward, "radius-r lookup into a correlation volume");
+  m.def("corr_index_backward", &corr_index_backward, "adjoint of corr_index_forward");
+  // additions of this build
+  m.def("ba_ex", &ba_ex, "ba with an explicit precision mode (0 split, 1 all-FP32); returns [dx, dz, None, ok]");
+  m.def("ba_index", &ba_index, "kx and kk_exp of a ba call (host-side planner)");
+  m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch");
+  m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
+        py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none());
+  m.attr("PRECISION_SPLIT") = py::int_(static_cast<int>(HSL2_BA_SPLIT));
+  m.attr("PRECISION_FP32") = py::int_(static_cast<int>(HSL2_BA_FP32));
+}

@@ -1,0 +1,166 @@
+/*
+ * hsl2.h -- C ABI of the MI355X (gfx950) dense-tracking library.
+ *
+ * Plain pointers and sizes only: no torch types cross this boundary.  Every
+ * entry point returns an int status (HSL2_OK == 0); a human-readable detail of
+ * the last failure on the calling thread is available from hsl2_last_error().
+ * Device pointers must be allocated on the device that is current when the
+ * call is made; `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream).  No entry point allocates device memory except through rocSOLVER's
+ * own handle (see hsl2_ba_run); scratch is always supplied by the caller.
+ *
+ * Each function names the reference interface it replaces (paths relative to
+ * the HI-SLAM2 source tree).
+ */
+#ifndef HSL2_H_
+#define HSL2_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  HSL2_OK = 0,
+  HSL2_ERR_ARGUMENT = 1,   /* null pointer, unknown dtype, bad scalar argument */
+  HSL2_ERR_SHAPE = 2,      /* sizes inconsistent with each other */
+  HSL2_ERR_INDEX = 3,      /* an edge / frame index outside its valid range */
+  HSL2_ERR_HIP = 4,        /* a HIP runtime or rocSOLVER call failed */
+  HSL2_ERR_WORKSPACE = 5,  /* caller-supplied workspace too small */
+  HSL2_ERR_UNSUPPORTED = 6 /* valid request this build does not implement */
+};
+
+/* ---- element types -------------------------------------------------------- */
+enum {
+  HSL2_F32 = 0,
+  HSL2_F16 = 1,
+  HSL2_BF16 = 2,
+  HSL2_F64 = 3
+};
+
+/* ---- BA precision modes ---------------------------------------------------- */
+enum {
+  HSL2_BA_SPLIT = 0,  /* FP32 linearization + Schur Gram, FP64 reduced Cholesky and back-substitution */
+  HSL2_BA_FP32 = 1    /* the same kernels with an FP32 reduced Cholesky and back-substitution */
+};
+
+int hsl2_version(void);
+const char* hsl2_status_string(int status);
+const char* hsl2_last_error(void);
+
+/* ---- correlation lookup ---------------------------------------------------- */
+
+/* Replaces corr_index_forward (src/droid.cpp:254-262 -> src/correlation_kernels.cu:126-155).
+ * volume [batch, h1, w1, h2, w2] of `dtype` (F32/F16/BF16/F64), coords [batch, 2, h1, w1] f32
+ * (x then y), corr [batch, 2r+1, 2r+1, h1, w1] of `dtype`, fully written (no pre-zeroing). */
+int hsl2_corr_index_forward(const void* volume, const float* coords, void* corr, int dtype,
+                            int64_t batch, int64_t h1, int64_t w1, int64_t h2, int64_t w2,
+                            int radius, void* stream);
+
+/* Replaces corr_index_backward (src/droid.cpp:264-275 -> src/correlation_kernels.cu:157-185).
+ * corr_grad [batch, 2r+1, 2r+1, h1, w1] of `dtype`; volume_grad [batch, h1, w1, h2, w2] of
+ * `dtype` is fully written (zeros outside the taps). */
+int hsl2_corr_index_backward(const float* coords, const void* corr_grad, void* volume_grad,
+                             int dtype, int64_t batch, int64_t h1, int64_t w1, int64_t h2,
+                             int64_t w2, int radius, void* stream);
+
+/* Replaces CorrBlock.__call__ (hislam2/modules/corr.py:40-50: one corr_index_forward per
+ * level on coords / 2^level, then torch.cat) with one launch.  levels[i] is the level-i volume
+ * [batch, h1, w1, h1 >> i, w1 >> i] of `dtype`; coords [batch, h1, w1, 2] f32 (x, y
+ * interleaved, the CorrBlock call layout); corr [batch, L*(2r+1)^2, h1, w1] of `dtype`, channel
+ * order [level][x-offset][y-offset], fully written. */
+int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const float* coords,
+                             void* corr, int dtype, int64_t batch, int64_t h1, int64_t w1,
+                             int radius, void* stream);
+
+/* ---- correlation build ------------------------------------------------------ */
+
+/* Replaces CorrBlock.__init__ + CorrBlock.corr (hislam2/modules/corr.py:24-38, 63-71):
+ * all-pairs correlation (f1/4)^T (f2/4) of fmap1/fmap2 [batch, C, h, w] (`in_dtype`
+ * F32/BF16/F16) followed by `num_levels` 2x2 average-pool levels.  levels[i] points at
+ * [batch, h, w, h >> i, w >> i] of `out_dtype` (F32/BF16).  Requires h, w divisible by
+ * 2^(num_levels-1) and C a multiple of 32. */
+int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype, void* const* levels,
+                    int out_dtype, int64_t batch, int64_t channels, int64_t h, int64_t w,
+                    int num_levels, void* stream);
+
+/* ---- on-the-fly correlation ------------------------------------------------- */
+
+/* Layout helper: [rows, C, P] -> [rows, P, C] for `dtype` (F32/F16/BF16).  The altcorr
+ * kernels read channel-last feature maps so that a tap's channel vector is contiguous. */
+int hsl2_channels_last(const void* in, void* out, int dtype, int64_t rows, int64_t channels,
+                       int64_t pixels, void* stream);
+/* Layout helper: [rows, P, C] -> [rows, C, P], f32. */
+int hsl2_channels_first_f32(const float* in, float* out, int64_t rows, int64_t channels,
+                            int64_t pixels, void* stream);
+
+/* Replaces altcorr_forward (src/droid.cpp:277-289 -> src/altcorr_kernel.cu:132-172).
+ * fmap1 [B, N1, H, W, C], fmap2 [B, N2, H2, W2, C] CHANNEL-LAST of `dtype` (F32/F16/BF16;
+ * hsl2_channels_last converts the reference's [B, N, C, H, W]), coords [B, M, 2, H, W] f32,
+ * ii/jj [M] int64 (device) index fmap1/fmap2 (an out-of-range edge yields zeros),
+ * corr [B, M, 2r+1 (x-offset), 2r+1 (y-offset), H, W] of `dtype`, fully written.
+ * C must be a multiple of 8. */
+int hsl2_altcorr_forward(const void* fmap1, const void* fmap2, const float* coords,
+                         const int64_t* ii, const int64_t* jj, void* corr, int dtype,
+                         int64_t B, int64_t N1, int64_t N2, int64_t C, int64_t H, int64_t W,
+                         int64_t H2, int64_t W2, int64_t M, int radius, void* stream);
+
+/* Replaces altcorr_backward (src/droid.cpp:291-305 -> src/altcorr_kernel.cu:175-225) with
+ * the argument order the binding declares and the exact adjoint of the forward (including
+ * its 1/16 scaling).  Same layouts as the forward; corr_grad is f32; fmap1_grad
+ * [B, N1, H, W, C] and fmap2_grad [B, N2, H2, W2, C] are f32 channel-last, zeroed and then
+ * accumulated by this call. */
+int hsl2_altcorr_backward(const void* fmap1, const void* fmap2, const float* coords,
+                          const float* corr_grad, const int64_t* ii, const int64_t* jj,
+                          float* fmap1_grad, float* fmap2_grad, int dtype,
+                          int64_t B, int64_t N1, int64_t N2, int64_t C, int64_t H, int64_t W,
+                          int64_t H2, int64_t W2, int64_t M, int radius, void* stream);
+
+/* ---- dense bundle adjustment ------------------------------------------------- */
+
+typedef struct hsl2_ba_plan hsl2_ba_plan;
+
+/* Host-side preparation of one ba call (the index work of src/droid_kernels.cu:1837-1849,
+ * 1413-1463 and 1689-1746, done once per call instead of per iteration).
+ * ii, jj: HOST arrays of num_edges frame indices.  Requires 0 <= t0 < t1 <= num_poses and
+ * 0 <= ii, jj < t1.  num_poses = rows of the poses buffer, ht x wd = disparity map size. */
+int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t num_edges,
+                        int64_t num_poses, int64_t ht, int64_t wd, int t0, int t1,
+                        int motion_only, int precision, hsl2_ba_plan** plan);
+void hsl2_ba_plan_destroy(hsl2_ba_plan* plan);
+/* number of unique depth frames |kx| (rows of eta and of dz) */
+int64_t hsl2_ba_plan_num_frames(const hsl2_ba_plan* plan);
+/* copies kx (sorted unique of [t0,t1) ++ ii) into a host array of num_frames entries */
+int hsl2_ba_plan_frames(const hsl2_ba_plan* plan, int64_t* kx);
+/* copies the inverse index kk_exp (one entry per expanded edge, t1-t0+num_edges) */
+int hsl2_ba_plan_inverse(const hsl2_ba_plan* plan, int64_t* kk_exp);
+/* bytes of device workspace hsl2_ba_run needs (256-byte aligned base required) */
+size_t hsl2_ba_plan_workspace_bytes(const hsl2_ba_plan* plan);
+
+/* Replaces ba (src/droid.cpp:141-168 -> src/droid_kernels.cu:1820-1937).
+ * Device arrays: poses [num_poses, 7] f32 and disps [num_poses, ht, wd] f32 (updated in
+ * place), intrinsics [4] f32, targets/weights [num_edges, 2, ht, wd] f32, eta
+ * [eta_rows, ht, wd] f32 with eta_rows == num_frames or 1 (broadcast).
+ * Outputs: dx [t1-t0, 6] f32 (last iteration's pose update), dz [num_frames, ht*wd] f32
+ * (last iteration's depth update; may be NULL; ignored when motion_only).
+ * *solve_ok (host, may be NULL) receives 1 if every reduced Cholesky succeeded; reading it
+ * synchronizes the stream.  A failed Cholesky gives dx = 0 as in the reference. */
+int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps, const float* intrinsics,
+                const float* targets, const float* weights, const float* eta, int64_t eta_rows,
+                int iterations, float lm, float ep, float* dx, float* dz,
+                void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
+
+/* Debug/test view of the last iteration's intermediates inside a workspace after
+ * hsl2_ba_run: per-edge Hessian blocks Hs [4, E, 6, 6] and gradients vs [2, E, 6] (f32),
+ * exactly the buffers of src/droid_kernels.cu:1856-1857.  Device pointers into workspace. */
+int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, float** Hs, float** vs,
+                        void** reduced_matrix, void** reduced_rhs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSL2_H_ */

@@ -1,0 +1,85 @@
+"""CPU-side checks of the boundary: the C ABI library loads and exports every
+symbol include/hsl2.h declares, the host planner reproduces the reference's
+index structures bit-exactly, and bad arguments are rejected before any GPU
+work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from hislam2_amd import synth
+from oracle import ba as oba
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, 'hi-slam2_amd', 'libhsl2.so')
+HEADER = os.path.join(ROOT, 'include', 'hsl2.h')
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(hsl2_[a-z0-9_]+)\s*\(', text)))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    if not os.path.exists(LIB):
+        pytest.skip('libhsl2.so not built')
+    return ctypes.CDLL(LIB)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_status_strings(lib):
+    lib.hsl2_status_string.restype = ctypes.c_char_p
+    assert lib.hsl2_status_string(0) == b'ok'
+    assert lib.hsl2_status_string(3) == b'index out of range'
+    assert lib.hsl2_version() >= 1
+
+
+@pytest.fixture(scope='module')
+def db():
+    from hislam2_amd import backend
+    return backend()
+
+
+@pytest.mark.parametrize('name', ['small', 'window', 'global300'])
+def test_planner_index_structures_bit_exact(db, name):
+    p = synth.PROBLEMS[name]()
+    kx, kk = db.ba_index(torch.from_numpy(p.ii), torch.from_numpy(p.jj), p.t0, p.t1, p.poses.shape[0])
+    _, _, rkx, rkk = oba.index_structures(p.ii, p.jj, p.t0, p.t1)
+    assert np.array_equal(kx.numpy(), rkx)
+    assert np.array_equal(kk.numpy(), rkk)
+    assert len(kx) == p.eta.shape[0]
+
+
+def test_planner_rejects_out_of_window_edges(db):
+    ii = torch.tensor([0, 1, 2])
+    jj = torch.tensor([1, 2, 3])
+    with pytest.raises(ValueError):
+        db.ba_index(ii, jj, 1, 3, 5)          # jj == 3 >= t1
+    with pytest.raises(ValueError):
+        db.ba_index(ii, jj, 2, 1, 5)          # t1 <= t0
+    with pytest.raises(ValueError):
+        db.ba_index(torch.tensor([-1]), torch.tensor([0]), 0, 2, 5)
+
+
+def test_planner_empty_edge_list(db):
+    kx, kk = db.ba_index(torch.zeros(0, dtype=torch.int64), torch.zeros(0, dtype=torch.int64), 2, 5, 8)
+    assert kx.tolist() == [2, 3, 4] and kk.tolist() == [0, 1, 2]
+
+
+def test_extension_rejects_cpu_tensors(db):
+    p = synth.ba_small()
+    t = lambda a: torch.from_numpy(a)
+    with pytest.raises(ValueError):
+        db.ba(t(p.poses), t(p.disps), t(p.intrinsics), t(p.targets), t(p.weights), t(p.eta),
+              t(p.ii), t(p.jj), p.t0, p.t1, 1, 1e-4, 0.1, False)

@@ -1,0 +1,148 @@
+"""GPU parity of droid_backends.ba against the float64 oracle (ba_cuda semantics).
+
+Tolerance (north star): pose / depth updates within 1e-4 relative to the
+largest reference magnitude; the same bound is applied to the updated poses and
+disparities.  The error against the oracle is printed for every case.
+"""
+import numpy as np
+import pytest
+import torch
+
+from hislam2_amd import synth
+from oracle import ba as oba
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def db():
+    from hislam2_amd import backend
+    return backend()
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+def to_dev(p):
+    c = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return (c(p.poses), c(p.disps), c(p.intrinsics), c(p.targets), c(p.weights), c(p.eta),
+            c(p.ii), c(p.jj))
+
+
+def run_gpu(db, p, iterations=None, precision=0, motion_only=False, weights=None, lm=None, ep=None):
+    poses, disps, intr, tg, wt, eta, ii, jj = to_dev(p)
+    if weights is not None:
+        wt = torch.from_numpy(weights).cuda()
+    it = p.iterations if iterations is None else iterations
+    dx, dz, _, ok = db.ba_ex(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, it,
+                             p.lm if lm is None else lm, p.ep if ep is None else ep, motion_only,
+                             precision)
+    torch.cuda.synchronize()
+    return (poses.cpu().numpy(), disps.cpu().numpy(), dx.cpu().numpy(),
+            None if dz is None else dz.cpu().numpy(), int(ok.item()))
+
+
+def run_oracle(p, iterations=None, motion_only=False, weights=None, lm=None, ep=None):
+    return oba.ba(p.poses, p.disps, p.intrinsics, p.targets,
+                  p.weights if weights is None else weights, p.eta, p.ii, p.jj, p.t0, p.t1,
+                  p.iterations if iterations is None else iterations,
+                  p.lm if lm is None else lm, p.ep if ep is None else ep, motion_only=motion_only)
+
+
+@pytest.mark.parametrize('name', ['small', 'window'])
+def test_ba_matches_oracle(db, name):
+    p = synth.PROBLEMS[name]()
+    poses, disps, dx, dz, ok = run_gpu(db, p)
+    rposes, rdisps, rdx, rdz = run_oracle(p)
+    errs = dict(dx=rel_err(dx, rdx), dz=rel_err(dz, rdz), poses=rel_err(poses, rposes),
+                disps=rel_err(disps, rdisps))
+    print(name, errs)
+    assert ok == 1
+    assert errs['dx'] <= 1e-4 and errs['dz'] <= 1e-4
+    assert errs['poses'] <= 1e-4 and errs['disps'] <= 1e-4
+    # frames outside the window keep their poses bit-exactly
+    assert np.array_equal(poses[:p.t0], p.poses[:p.t0])
+
+
+def test_ba_single_iteration_updates(db):
+    p = synth.ba_window()
+    _, _, dx, dz, _ = run_gpu(db, p, iterations=1)
+    _, _, rdx, rdz = run_oracle(p, iterations=1)
+    print('dx', rel_err(dx, rdx), 'dz', rel_err(dz, rdz))
+    assert rel_err(dx, rdx) <= 1e-4 and rel_err(dz, rdz) <= 1e-4
+
+
+def test_ba_plain_fp32_mode(db):
+    p = synth.ba_window()
+    poses, disps, dx, dz, ok = run_gpu(db, p, precision=1)
+    rposes, rdisps, rdx, rdz = run_oracle(p)
+    errs = dict(dx=rel_err(dx, rdx), dz=rel_err(dz, rdz), poses=rel_err(poses, rposes))
+    print('fp32', errs)
+    assert ok == 1
+    assert errs['dx'] <= 1e-3 and errs['dz'] <= 1e-3
+
+
+def test_ba_motion_only(db):
+    p = synth.ba_window()
+    poses, disps, dx, dz, ok = run_gpu(db, p, iterations=2, motion_only=True)
+    rposes, rdisps, rdx, _ = run_oracle(p, iterations=2, motion_only=True)
+    print('motion_only', rel_err(dx, rdx))
+    assert dz is None
+    assert np.array_equal(disps, p.disps)
+    assert rel_err(dx, rdx) <= 1e-4 and rel_err(poses, rposes) <= 1e-4
+
+
+def test_ba_failed_cholesky_gives_zero_pose_update(db):
+    """Zero weights and no damping make the reduced system singular: dx = 0, poses
+    unchanged, the status flag reports the failure (src/droid_kernels.cu:1669-1677)."""
+    p = synth.ba_small()
+    wt = np.zeros_like(p.weights)
+    poses, disps, dx, dz, ok = run_gpu(db, p, iterations=1, weights=wt, lm=0.0, ep=0.0)
+    assert ok == 0
+    assert np.all(dx == 0)
+    assert np.array_equal(poses, p.poses)
+    assert np.all(dz == 0)
+
+
+def test_ba_reference_signature_and_inplace(db):
+    """The 14-argument reference call mutates poses/disps in place and returns [dx, dz, None]."""
+    p = synth.ba_small()
+    poses, disps, intr, tg, wt, eta, ii, jj = to_dev(p)
+    out = db.ba(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, 2, 1e-4, 0.1, False)
+    assert len(out) == 3 and out[2] is None
+    assert out[0].shape == (p.t1 - p.t0, 6) and out[1].shape == (len(np.unique(np.r_[np.arange(p.t0, p.t1), p.ii])), 48 * 64)
+    assert not torch.equal(poses.cpu(), torch.from_numpy(p.poses))
+
+
+def test_ba_eta_single_row_broadcasts(db):
+    p = synth.ba_small()
+    poses, disps, intr, tg, wt, eta, ii, jj = to_dev(p)
+    eta1 = eta[:1].contiguous()
+    db.ba(poses, disps, intr, tg, wt, eta1, ii, jj, p.t0, p.t1, 1, 1e-4, 0.1, False)
+    rposes, rdisps, _, _ = oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights,
+                                  np.broadcast_to(p.eta[:1], p.eta.shape), p.ii, p.jj, p.t0, p.t1, 1)
+    assert rel_err(disps.cpu().numpy(), rdisps) <= 1e-4
+
+
+def test_ba_rejects_bad_arguments(db):
+    p = synth.ba_small()
+    poses, disps, intr, tg, wt, eta, ii, jj = to_dev(p)
+    with pytest.raises(ValueError):
+        db.ba(poses, disps, intr, tg, wt, eta, ii, jj, 1, 3, 1, 1e-4, 0.1, False)   # jj == 3 >= t1
+    with pytest.raises(ValueError):
+        db.ba(poses, disps, intr, tg, wt, eta[:2].contiguous(), ii, jj, 1, 4, 1, 1e-4, 0.1, False)
+    with pytest.raises(RuntimeError):
+        db.ba(poses, disps, intr, tg.transpose(2, 3), wt, eta, ii, jj, 1, 4, 1, 1e-4, 0.1, False)
+
+
+def test_ba_global300_first_iteration(db):
+    """Full-size 300-keyframe / 3000-edge problem, one iteration against the oracle."""
+    p = synth.ba_global300()
+    _, _, dx, dz, ok = run_gpu(db, p, iterations=1)
+    _, _, rdx, rdz = run_oracle(p, iterations=1)
+    print('global300 it1', rel_err(dx, rdx), rel_err(dz, rdz))
+    assert ok == 1
+    assert rel_err(dx, rdx) <= 1e-4 and rel_err(dz, rdz) <= 1e-4

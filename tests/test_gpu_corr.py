@@ -1,0 +1,152 @@
+"""GPU parity of the correlation kernels against the float64 oracle.
+
+Inputs are seeded; reduced-precision inputs are rounded to their dtype before the
+oracle sees them, so the tolerance covers only output rounding and summation
+order.  Stated tolerances (error vs the float64 oracle, relative to the largest
+reference magnitude): f64 1e-12, f32 1e-5, f16 2e-3, bf16 1e-2; the BF16
+absolute bound of 2e-2 is also asserted.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import corr as oc
+
+pytestmark = pytest.mark.gpu
+
+RTOL = {torch.float64: 1e-12, torch.float32: 1e-5, torch.float16: 2e-3, torch.bfloat16: 1e-2}
+
+
+@pytest.fixture(scope='module')
+def db():
+    from hislam2_amd import backend
+    return backend()
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+def rounded(x, dtype):
+    return torch.as_tensor(x).to(dtype)
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64, torch.float16, torch.bfloat16])
+@pytest.mark.parametrize('r', [3, 4])
+def test_lookup_forward(db, dtype, r):
+    rng = np.random.default_rng(10 + r)
+    BN, h1, w1, h2, w2 = 2, 12, 16, 12, 16
+    vol = rounded(rng.standard_normal((BN, h1, w1, h2, w2)), dtype)
+    coords = np.stack([rng.uniform(-4, w2 + 4, (BN, h1, w1)), rng.uniform(-4, h2 + 4, (BN, h1, w1))], 1)
+    coords = torch.as_tensor(coords, dtype=torch.float32)
+    coords[0, :, 0, 0] = torch.tensor([3.0, 5.0])     # integer coordinate
+    coords[1, :, 2, 3] = torch.tensor([-50.0, 80.0])  # far outside: all zero
+    out, = db.corr_index_forward(vol.cuda(), coords.cuda(), r)
+    ref = oc.corr_index_forward(vol.double().numpy(), coords.double().numpy(), r)
+    assert out.dtype == dtype and out.shape == (BN, 2 * r + 1, 2 * r + 1, h1, w1)
+    got = out.double().cpu().numpy()
+    assert rel_err(got, ref) <= RTOL[dtype]
+    assert np.all(got[1, :, :, 2, 3] == 0)
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64, torch.bfloat16])
+def test_lookup_backward(db, dtype):
+    rng = np.random.default_rng(20)
+    BN, h1, w1, h2, w2, r = 2, 8, 12, 9, 11, 3
+    vol = rounded(rng.standard_normal((BN, h1, w1, h2, w2)), dtype)
+    coords = torch.as_tensor(np.stack([rng.uniform(-3, w2 + 3, (BN, h1, w1)),
+                                       rng.uniform(-3, h2 + 3, (BN, h1, w1))], 1), dtype=torch.float32)
+    g = rounded(rng.standard_normal((BN, 2 * r + 1, 2 * r + 1, h1, w1)), dtype)
+    vg, = db.corr_index_backward(vol.cuda(), coords.cuda(), g.cuda(), r)
+    ref = oc.corr_index_backward(vol.shape, coords.double().numpy(), g.double().numpy(), r)
+    assert vg.shape == vol.shape and vg.dtype == dtype
+    assert rel_err(vg.double().cpu().numpy(), ref) <= RTOL[dtype]
+
+
+@pytest.mark.parametrize('hw', [(16, 32), (48, 64)])
+@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
+def test_corr_build_pyramid(db, hw, dtype):
+    h, w = hw
+    rng = np.random.default_rng(30)
+    f1 = rounded(rng.standard_normal((1, 2, 128, h, w)) / np.sqrt(128), dtype)
+    f2 = rounded(rng.standard_normal((1, 2, 128, h, w)) / np.sqrt(128), dtype)
+    levels = db.corr_build(f1.cuda(), f2.cuda(), 4, dtype)
+    ref = oc.pyramid(oc.corr_allpairs(f1.double().numpy(), f2.double().numpy()), 4)
+    tol = 1e-5 if dtype == torch.float32 else 1e-2
+    for i in range(4):
+        assert levels[i].shape == (2, h, w, h >> i, w >> i) and levels[i].dtype == dtype
+        got = levels[i].double().cpu().numpy()
+        assert rel_err(got, ref[i]) <= tol, i
+        if dtype == torch.bfloat16:
+            assert np.max(np.abs(got - ref[i])) <= 2e-2
+
+
+def test_corrblock_small_config(db):
+    """4-level radius-3 CorrBlock lookup at 48x64 (196 channels), fp32 and bf16 volumes."""
+    from hislam2_amd import corr as hc
+    from hislam2_amd import synth
+    fm = synth.fmaps(0, 2)
+    coords = synth.lookup_coords(1, 1)
+    f1, f2 = torch.from_numpy(fm[:, :1]), torch.from_numpy(fm[:, 1:])
+    ref_pyr = oc.pyramid(oc.corr_allpairs(f1.double().numpy(), f2.double().numpy()), 4)
+    ref = oc.corrblock_lookup(ref_pyr, coords.astype(np.float64), 3)
+    for dtype, tol in ((torch.float32, 1e-5), (torch.bfloat16, 1e-2)):
+        blk = hc.CorrBlock(f1.cuda().to(dtype), f2.cuda().to(dtype), dtype=dtype)
+        out = blk(torch.from_numpy(coords).cuda())
+        assert out.shape == (1, 1, 196, 48, 64)
+        if dtype == torch.bfloat16:
+            # bf16 operands: compare with the oracle on bf16-rounded features too
+            r1, r2 = f1.to(dtype).double().numpy(), f2.to(dtype).double().numpy()
+            ref_b = oc.corrblock_lookup(oc.pyramid(oc.corr_allpairs(r1, r2), 4), coords.astype(np.float64), 3)
+            assert rel_err(out.double().cpu().numpy(), ref_b) <= tol
+            assert np.max(np.abs(out.double().cpu().numpy() - ref)) <= 2e-2
+        else:
+            assert rel_err(out.double().cpu().numpy(), ref) <= tol
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize('r', [3, 4])
+def test_altcorr_forward(db, dtype, r):
+    rng = np.random.default_rng(40 + r)
+    B, N, C, H, W = 1, 3, 32, 8, 12
+    f1 = rounded(rng.standard_normal((B, N, C, H, W)), dtype)
+    f2 = rounded(rng.standard_normal((B, N, C, H // 2, W // 2)), dtype)
+    ii = torch.tensor([0, 1, 2, 0, 2])
+    jj = torch.tensor([1, 2, 0, 0, 2])
+    coords = torch.as_tensor(rng.uniform(-3, 9, (B, 5, 2, H, W)), dtype=torch.float32)
+    out, = db.altcorr_forward(f1.cuda(), f2.cuda(), coords.cuda(), ii.cuda(), jj.cuda(), r)
+    ref = oc.altcorr_forward(f1.double().numpy(), f2.double().numpy(), coords.double().numpy(),
+                             ii.numpy(), jj.numpy(), r)
+    assert out.shape == (B, 5, 2 * r + 1, 2 * r + 1, H, W) and out.dtype == dtype
+    assert rel_err(out.double().cpu().numpy(), ref) <= (1e-5 if dtype == torch.float32 else 1e-2)
+
+
+def test_altcorr_backward(db):
+    rng = np.random.default_rng(50)
+    B, N, C, H, W, r = 1, 3, 16, 6, 8, 2
+    f1 = torch.as_tensor(rng.standard_normal((B, N, C, H, W)), dtype=torch.float32)
+    f2 = torch.as_tensor(rng.standard_normal((B, N, C, H, W)), dtype=torch.float32)
+    ii, jj = torch.tensor([0, 1, 2, 1]), torch.tensor([1, 2, 0, 1])
+    coords = torch.as_tensor(rng.uniform(-2, 9, (B, 4, 2, H, W)), dtype=torch.float32)
+    g = torch.as_tensor(rng.standard_normal((B, 4, 2 * r + 1, 2 * r + 1, H, W)), dtype=torch.float32)
+    g1, g2 = db.altcorr_backward(f1.cuda(), f2.cuda(), coords.cuda(), g.cuda(), ii.cuda(), jj.cuda(), r)
+    r1, r2 = oc.altcorr_backward(f1.double().numpy(), f2.double().numpy(), coords.double().numpy(),
+                                 g.double().numpy(), ii.numpy(), jj.numpy(), r)
+    assert rel_err(g1.double().cpu().numpy(), r1) <= 1e-5
+    assert rel_err(g2.double().cpu().numpy(), r2) <= 1e-5
+
+
+def test_altcorr_block_equals_corrblock(db):
+    """AltCorrBlock and CorrBlock agree (avg-pooling commutes with the correlation)."""
+    from hislam2_amd import corr as hc
+    from hislam2_amd import synth
+    fm = torch.from_numpy(synth.fmaps(3, 3, h=16, w=32)).cuda()
+    ii, jj = torch.tensor([0, 1, 2]).cuda(), torch.tensor([1, 2, 0]).cuda()
+    coords = torch.from_numpy(synth.lookup_coords(4, 3, h=16, w=32)).cuda()
+    alt = hc.AltCorrBlock(fm)(coords, ii, jj)
+    blk = hc.CorrBlock(fm[:, ii], fm[:, jj], dtype=torch.float32)
+    ref = blk(coords)
+    assert alt.shape == ref.shape == (1, 3, 196, 16, 32)
+    assert rel_err(alt.double().cpu().numpy(), ref.double().cpu().numpy()) <= 1e-5
