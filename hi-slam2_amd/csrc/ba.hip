@@ -148,19 +148,21 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
              const float* __restrict__ intr, const float* __restrict__ targets,
              const float* __restrict__ weights, const int* __restrict__ edge_ii,
              const int* __restrict__ edge_jj, int HW, int wd, int E,
-             float* __restrict__ Hs, float* __restrict__ vs, float* __restrict__ Eij,
+             double* __restrict__ Hs, double* __restrict__ vs, float* __restrict__ Eij,
              float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz) {
-  __shared__ float lds[16 * kLinThreads];
-  __shared__ float red[90];
+  __shared__ double lds[16 * kLinThreads];
+  __shared__ double red[90];
   const int e = blockIdx.x;
   const int ix = edge_ii[e], jx = edge_jj[e];
   const bool self = ix == jx;
   EdgeGeom g;
   load_edge_geom(poses, intr, ix, jx, g);
 
-  float acc[90];
+  // Per-pixel products are formed in FP32; their sums are kept in FP64 (the FP64 VALU
+  // rate equals the FP32 rate on this part, so this costs registers, not throughput).
+  double acc[90];
 #pragma unroll
-  for (int k = 0; k < 90; ++k) acc[k] = 0.f;
+  for (int k = 0; k < 90; ++k) acc[k] = 0.0;
 
   const float* dsp = disps + (int64_t)ix * HW;
   const float* tg = targets + (int64_t)e * 2 * HW;
@@ -193,29 +195,27 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
       }
     }
     if (ok && !self) {
+      float Ju[12], Jv[12];
 #pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const RowJac& J = rr == 0 ? ru : rv;
-        float Jx[12];
-#pragma unroll
-        for (int n = 0; n < 6; ++n) {
-          Jx[n] = J.Ji[n];
-          Jx[6 + n] = J.Jj[n];
-        }
-#pragma unroll
-        for (int n = 0; n < 12; ++n) {
-          const float wn = J.w * Jx[n];
-#pragma unroll
-          for (int m = 0; m <= n; ++m) acc[tri(n, m)] += wn * Jx[m];
-        }
-        const float wr = J.w * J.r;
-#pragma unroll
-        for (int n = 0; n < 12; ++n) acc[78 + n] += wr * Jx[n];
+      for (int n = 0; n < 6; ++n) {
+        Ju[n] = ru.Ji[n];
+        Ju[6 + n] = ru.Jj[n];
+        Jv[n] = rv.Ji[n];
+        Jv[6 + n] = rv.Jj[n];
       }
+#pragma unroll
+      for (int n = 0; n < 12; ++n) {
+        const float wun = ru.w * Ju[n], wvn = rv.w * Jv[n];
+#pragma unroll
+        for (int m = 0; m <= n; ++m) acc[tri(n, m)] += static_cast<double>(wun * Ju[m] + wvn * Jv[m]);
+      }
+      const float wru = ru.w * ru.r, wrv = rv.w * rv.r;
+#pragma unroll
+      for (int n = 0; n < 12; ++n) acc[78 + n] += static_cast<double>(wru * Ju[n] + wrv * Jv[n]);
     }
   }
 
-  block_reduce_store<90, kLinThreads>(acc, lds, red);
+  block_reduce_store<90, kLinThreads, double>(acc, lds, red);
   __syncthreads();
   const int t = threadIdx.x;
   if (t < 144) {
@@ -243,25 +243,25 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= HW) return;
   const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
-  float c = 0.f, b = 0.f;
+  double c = 0.0, b = 0.0;
   for (int q = e0; q < e1; ++q) {
     const int e = frame_edges[q];
     c += Cii[(int64_t)e * HW + k];
     b += bz[(int64_t)e * HW + k];
   }
   const float et = eta[(int64_t)(eta_rows == 1 ? 0 : f) * HW + k];
-  Q[(int64_t)f * HW + k] = 1.f / (c + et);
-  w[(int64_t)f * HW + k] = b;
+  Q[(int64_t)f * HW + k] = static_cast<float>(1.0 / (c + et));
+  w[(int64_t)f * HW + k] = static_cast<float>(b);
   const int p = frame_pose[f];
   if (p >= 0) {
-    float ei[6] = {0, 0, 0, 0, 0, 0};
+    double ei[6] = {0, 0, 0, 0, 0, 0};
     for (int q = e0; q < e1; ++q) {
       const int e = frame_edges[q];
 #pragma unroll
       for (int n = 0; n < 6; ++n) ei[n] += Eii[((int64_t)e * 6 + n) * HW + k];
     }
 #pragma unroll
-    for (int n = 0; n < 6; ++n) Eall[((int64_t)p * 6 + n) * HW + k] = ei[n];
+    for (int n = 0; n < 6; ++n) Eall[((int64_t)p * 6 + n) * HW + k] = static_cast<float>(ei[n]);
   }
 }
 
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(256)
 ba_schur_gram(const float* __restrict__ Eall, const float* __restrict__ Q,
               const float* __restrict__ w, const int* __restrict__ tasks,
               const int* __restrict__ slot_row, int NT, int NPAIR, int HW, int task_blocks8,
-              float* __restrict__ S, float* __restrict__ Ev) {
+              double* __restrict__ S, double* __restrict__ Ev) {
   // XCD-aware order: the 8 XCDs receive blocks round-robin, so give each XCD a
   // contiguous run of tasks (tasks of one depth frame share their E rows in L2).
   const int b = blockIdx.x;
@@ -286,12 +286,12 @@ ba_schur_gram(const float* __restrict__ Eall, const float* __restrict__ Q,
   const float* Ea = Eall + (int64_t)slot_row[sa] * 6 * HW;
   if (kind == kTaskPair) {
     const float* Eb = Eall + (int64_t)slot_row[sb] * 6 * HW;
-    float acc[36];
+    double acc[36];
 #pragma unroll
-    for (int k = 0; k < 36; ++k) acc[k] = 0.f;
+    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
     for (int k = lane; k < HW; k += kWave) {
-      const float q = Qf[k];
-      float ea[6], eb[6];
+      const double q = Qf[k];
+      double ea[6], eb[6];
 #pragma unroll
       for (int n = 0; n < 6; ++n) {
         ea[n] = Ea[n * HW + k] * q;
@@ -304,20 +304,20 @@ ba_schur_gram(const float* __restrict__ Eall, const float* __restrict__ Q,
     }
 #pragma unroll
     for (int k = 0; k < 36; ++k) {
-      const float v = wave_sum(acc[k]);
+      const double v = wave_sum(acc[k]);
       if (lane == k) S[(int64_t)(task) * 36 + k] = v;
     }
   } else {
     const float* wf = w + (int64_t)f * HW;
-    float acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int k = lane; k < HW; k += kWave) {
-      const float qw = Qf[k] * wf[k];
+      const double qw = static_cast<double>(Qf[k]) * wf[k];
 #pragma unroll
       for (int n = 0; n < 6; ++n) acc[n] += qw * Ea[n * HW + k];
     }
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
-      const float v = wave_sum(acc[n]);
+      const double v = wave_sum(acc[n]);
       if (lane == n) Ev[(int64_t)sa * 6 + n] = v;
     }
   }
@@ -326,7 +326,7 @@ ba_schur_gram(const float* __restrict__ Eall, const float* __restrict__ Q,
 // ---- 4. reduced camera system, dense column-major ----------------------------
 template <typename R>
 __global__ void __launch_bounds__(256)
-ba_assemble(const float* __restrict__ Hs, const float* __restrict__ S,
+ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
             const int* __restrict__ blk_ptr, const int* __restrict__ blk_src, int E, int P,
             float lm, float ep, R* __restrict__ A) {
   const int64_t n = 6 * (int64_t)P;
@@ -352,7 +352,7 @@ ba_assemble(const float* __restrict__ Hs, const float* __restrict__ S,
 
 template <typename R>
 __global__ void __launch_bounds__(256)
-ba_assemble_rhs(const float* __restrict__ vs, const float* __restrict__ Ev,
+ba_assemble_rhs(const double* __restrict__ vs, const double* __restrict__ Ev,
                 const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
                 R* __restrict__ bvec) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,10 +510,10 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
   float* bz = reinterpret_cast<float*>(ws + p->w_bz);
   float* Q = reinterpret_cast<float*>(ws + p->w_Q);
   float* wv = reinterpret_cast<float*>(ws + p->w_w);
-  float* Hs = reinterpret_cast<float*>(ws + p->w_Hs);
-  float* vs = reinterpret_cast<float*>(ws + p->w_vs);
-  float* S = reinterpret_cast<float*>(ws + p->w_S);
-  float* Ev = reinterpret_cast<float*>(ws + p->w_Ev);
+  double* Hs = reinterpret_cast<double*>(ws + p->w_Hs);
+  double* vs = reinterpret_cast<double*>(ws + p->w_vs);
+  double* S = reinterpret_cast<double*>(ws + p->w_S);
+  double* Ev = reinterpret_cast<double*>(ws + p->w_Ev);
   R* A = reinterpret_cast<R*>(ws + p->w_A);
   R* bvec = reinterpret_cast<R*>(ws + p->w_b);
   int* info = reinterpret_cast<int*>(ws + p->w_info);
@@ -629,12 +629,12 @@ extern "C" int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps,
   return HSL2_OK;
 }
 
-extern "C" int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, float** Hs,
-                                   float** vs, void** reduced_matrix, void** reduced_rhs) {
+extern "C" int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, double** Hs,
+                                   double** vs, void** reduced_matrix, void** reduced_rhs) {
   HSL2_REQUIRE(plan && workspace, HSL2_ERR_ARGUMENT, "ba: null pointer");
   char* ws = static_cast<char*>(workspace);
-  if (Hs) *Hs = reinterpret_cast<float*>(ws + plan->w_Hs);
-  if (vs) *vs = reinterpret_cast<float*>(ws + plan->w_vs);
+  if (Hs) *Hs = reinterpret_cast<double*>(ws + plan->w_Hs);
+  if (vs) *vs = reinterpret_cast<double*>(ws + plan->w_vs);
   if (reduced_matrix) *reduced_matrix = ws + plan->w_A;
   if (reduced_rhs) *reduced_rhs = ws + plan->w_b;
   return HSL2_OK;

@@ -210,10 +210,11 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
   p->w_bz = take(schur ? E * HW * f4 : 0);
   p->w_Q = take(schur ? F * HW * f4 : 0);
   p->w_w = take(schur ? F * HW * f4 : 0);
-  p->w_Hs = take(4 * E * 36 * f4);
-  p->w_vs = take(2 * E * 6 * f4);
-  p->w_S = take(p->NPAIR * 36 * f4);
-  p->w_Ev = take(p->NS * 6 * f4);
+  const size_t f8 = sizeof(double);
+  p->w_Hs = take(4 * E * 36 * f8);
+  p->w_vs = take(2 * E * 6 * f8);
+  p->w_S = take(p->NPAIR * 36 * f8);
+  p->w_Ev = take(p->NS * 6 * f8);
   p->w_A = take(36 * P * P * real);
   p->w_b = take(6 * P * real);
   p->w_info = take(4 * sizeof(int32_t));

@@ -44,10 +44,10 @@ struct hsl2_ba_plan {
   size_t w_bz = 0;       // f32 [E][HW]
   size_t w_Q = 0;        // f32 [F][HW]
   size_t w_w = 0;        // f32 [F][HW]
-  size_t w_Hs = 0;       // f32 [4][E][6][6]
-  size_t w_vs = 0;       // f32 [2][E][6]
-  size_t w_S = 0;        // f32 [NPAIR][6][6]
-  size_t w_Ev = 0;       // f32 [NS][6]
+  size_t w_Hs = 0;       // f64 [4][E][6][6]
+  size_t w_vs = 0;       // f64 [2][E][6]
+  size_t w_S = 0;        // f64 [NPAIR][6][6]
+  size_t w_Ev = 0;       // f64 [NS][6]
   size_t w_A = 0;        // f64 or f32 [6P][6P], column-major
   size_t w_b = 0;        // f64 or f32 [6P]
   size_t w_info = 0;     // int32 [2]: solver info, all-ok flag

@@ -69,11 +69,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Reduce V per-thread values over a block of NT threads (NT a multiple of 64, <= 1024)
-// through LDS in chunks of 16 values; thread t < V ends up with no value -- instead the
-// results are written by the block to out[0..V) (plain stores).  `lds` must hold
-// 16 * NT floats.  Every thread of the block must call this.
-template <int V, int NT>
-__device__ void block_reduce_store(const float (&vals)[V], float* lds, float* out) {
+// through LDS in chunks of 16 values; the block writes the V sums to out[0..V) (plain
+// stores).  `lds` must hold 16 * NT elements.  Every thread of the block must call this.
+template <int V, int NT, typename T>
+__device__ void block_reduce_store(const T (&vals)[V], T* lds, T* out) {
   constexpr int C = 16;
   const int t = threadIdx.x;
 #pragma unroll
@@ -86,7 +85,7 @@ __device__ void block_reduce_store(const float (&vals)[V], float* lds, float* ou
     constexpr int SEG = NT / C;                 // threads per value
     const int c = t / SEG;
     const int s = t % SEG;
-    float acc = 0.f;
+    T acc = 0;
     if (c0 + c < V) {
       for (int k = s; k < NT; k += SEG) acc += lds[c * NT + k];
     }

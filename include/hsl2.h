@@ -154,9 +154,11 @@ int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps, const floa
                 void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
 
 /* Debug/test view of the last iteration's intermediates inside a workspace after
- * hsl2_ba_run: per-edge Hessian blocks Hs [4, E, 6, 6] and gradients vs [2, E, 6] (f32),
- * exactly the buffers of src/droid_kernels.cu:1856-1857.  Device pointers into workspace. */
-int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, float** Hs, float** vs,
+ * hsl2_ba_run: per-edge Hessian blocks Hs [4, E, 6, 6] and gradients vs [2, E, 6], the
+ * buffers of src/droid_kernels.cu:1856-1857 but accumulated in f64, and the damped reduced
+ * system (f64 column-major [6P, 6P] and rhs [6P] in the split mode, f32 in the plain mode;
+ * the rhs is overwritten by the solution).  Device pointers into workspace. */
+int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, double** Hs, double** vs,
                         void** reduced_matrix, void** reduced_rhs);
 
 #ifdef __cplusplus

@@ -1,8 +1,13 @@
 """GPU parity of droid_backends.ba against the float64 oracle (ba_cuda semantics).
 
-Tolerance (north star): pose / depth updates within 1e-4 relative to the
-largest reference magnitude; the same bound is applied to the updated poses and
-disparities.  The error against the oracle is printed for every case.
+Tolerances (north star: pose / depth updates within 1e-4 relative):
+* the updates dx, dz of a single iteration from identical inputs: max |error| <= 1e-4 x
+  max |oracle update|;
+* after several iterations, the updated poses and disparities: 1e-4 relative, and the last
+  iteration's updates (which shrink as the problem converges, so their own magnitude is no
+  yardstick) within 1e-4 x the first iteration's update magnitude;
+* the plain all-FP32 mode: 1e-3 on the same scales.
+The error against the oracle is printed for every case.
 """
 import numpy as np
 import pytest
@@ -52,26 +57,33 @@ def run_oracle(p, iterations=None, motion_only=False, weights=None, lm=None, ep=
                   p.lm if lm is None else lm, p.ep if ep is None else ep, motion_only=motion_only)
 
 
+def first_update_scale(p, motion_only=False):
+    _, _, dx1, dz1 = run_oracle(p, iterations=1, motion_only=motion_only)
+    return float(np.abs(dx1).max()), (None if dz1 is None else float(np.abs(dz1).max()))
+
+
 @pytest.mark.parametrize('name', ['small', 'window'])
 def test_ba_matches_oracle(db, name):
     p = synth.PROBLEMS[name]()
     poses, disps, dx, dz, ok = run_gpu(db, p)
     rposes, rdisps, rdx, rdz = run_oracle(p)
-    errs = dict(dx=rel_err(dx, rdx), dz=rel_err(dz, rdz), poses=rel_err(poses, rposes),
-                disps=rel_err(disps, rdisps))
+    sx, sz = first_update_scale(p)
+    errs = dict(poses=rel_err(poses, rposes), disps=rel_err(disps, rdisps),
+                dx_last=float(np.abs(dx - rdx).max()) / sx, dz_last=float(np.abs(dz - rdz).max()) / sz)
     print(name, errs)
     assert ok == 1
-    assert errs['dx'] <= 1e-4 and errs['dz'] <= 1e-4
     assert errs['poses'] <= 1e-4 and errs['disps'] <= 1e-4
+    assert errs['dx_last'] <= 1e-4 and errs['dz_last'] <= 1e-4
     # frames outside the window keep their poses bit-exactly
     assert np.array_equal(poses[:p.t0], p.poses[:p.t0])
 
 
-def test_ba_single_iteration_updates(db):
-    p = synth.ba_window()
+@pytest.mark.parametrize('name', ['small', 'window'])
+def test_ba_single_iteration_updates(db, name):
+    p = synth.PROBLEMS[name]()
     _, _, dx, dz, _ = run_gpu(db, p, iterations=1)
     _, _, rdx, rdz = run_oracle(p, iterations=1)
-    print('dx', rel_err(dx, rdx), 'dz', rel_err(dz, rdz))
+    print(name, 'dx', rel_err(dx, rdx), 'dz', rel_err(dz, rdz))
     assert rel_err(dx, rdx) <= 1e-4 and rel_err(dz, rdz) <= 1e-4
 
 
@@ -79,20 +91,30 @@ def test_ba_plain_fp32_mode(db):
     p = synth.ba_window()
     poses, disps, dx, dz, ok = run_gpu(db, p, precision=1)
     rposes, rdisps, rdx, rdz = run_oracle(p)
-    errs = dict(dx=rel_err(dx, rdx), dz=rel_err(dz, rdz), poses=rel_err(poses, rposes))
+    sx, sz = first_update_scale(p)
+    errs = dict(poses=rel_err(poses, rposes), disps=rel_err(disps, rdisps),
+                dx_last=float(np.abs(dx - rdx).max()) / sx, dz_last=float(np.abs(dz - rdz).max()) / sz)
     print('fp32', errs)
     assert ok == 1
-    assert errs['dx'] <= 1e-3 and errs['dz'] <= 1e-3
+    assert max(errs.values()) <= 1e-3
+    _, _, dx1, dz1, _ = run_gpu(db, p, iterations=1, precision=1)
+    _, _, rdx1, rdz1 = run_oracle(p, iterations=1)
+    print('fp32 first iteration', rel_err(dx1, rdx1), rel_err(dz1, rdz1))
+    assert rel_err(dx1, rdx1) <= 1e-3 and rel_err(dz1, rdz1) <= 1e-3
 
 
 def test_ba_motion_only(db):
     p = synth.ba_window()
     poses, disps, dx, dz, ok = run_gpu(db, p, iterations=2, motion_only=True)
     rposes, rdisps, rdx, _ = run_oracle(p, iterations=2, motion_only=True)
-    print('motion_only', rel_err(dx, rdx))
+    sx, _ = first_update_scale(p, motion_only=True)
+    print('motion_only', rel_err(poses, rposes), float(np.abs(dx - rdx).max()) / sx)
     assert dz is None
     assert np.array_equal(disps, p.disps)
-    assert rel_err(dx, rdx) <= 1e-4 and rel_err(poses, rposes) <= 1e-4
+    assert rel_err(poses, rposes) <= 1e-4 and float(np.abs(dx - rdx).max()) / sx <= 1e-4
+    _, _, dx1, _, _ = run_gpu(db, p, iterations=1, motion_only=True)
+    _, _, rdx1, _ = run_oracle(p, iterations=1, motion_only=True)
+    assert rel_err(dx1, rdx1) <= 1e-4
 
 
 def test_ba_failed_cholesky_gives_zero_pose_update(db):

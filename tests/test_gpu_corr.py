@@ -47,7 +47,8 @@ def test_lookup_forward(db, dtype, r):
     ref = oc.corr_index_forward(vol.double().numpy(), coords.double().numpy(), r)
     assert out.dtype == dtype and out.shape == (BN, 2 * r + 1, 2 * r + 1, h1, w1)
     got = out.double().cpu().numpy()
-    assert rel_err(got, ref) <= RTOL[dtype]
+    e = rel_err(got, ref)
+    assert e <= RTOL[dtype], e
     assert np.all(got[1, :, :, 2, 3] == 0)
 
 
@@ -62,7 +63,8 @@ def test_lookup_backward(db, dtype):
     vg, = db.corr_index_backward(vol.cuda(), coords.cuda(), g.cuda(), r)
     ref = oc.corr_index_backward(vol.shape, coords.double().numpy(), g.double().numpy(), r)
     assert vg.shape == vol.shape and vg.dtype == dtype
-    assert rel_err(vg.double().cpu().numpy(), ref) <= RTOL[dtype]
+    e = rel_err(vg.double().cpu().numpy(), ref)
+    assert e <= RTOL[dtype], e
 
 
 @pytest.mark.parametrize('hw', [(16, 32), (48, 64)])
@@ -78,7 +80,8 @@ def test_corr_build_pyramid(db, hw, dtype):
     for i in range(4):
         assert levels[i].shape == (2, h, w, h >> i, w >> i) and levels[i].dtype == dtype
         got = levels[i].double().cpu().numpy()
-        assert rel_err(got, ref[i]) <= tol, i
+        e = rel_err(got, ref[i])
+        assert e <= tol, (i, e)
         if dtype == torch.bfloat16:
             assert np.max(np.abs(got - ref[i])) <= 2e-2
 
@@ -100,10 +103,12 @@ def test_corrblock_small_config(db):
             # bf16 operands: compare with the oracle on bf16-rounded features too
             r1, r2 = f1.to(dtype).double().numpy(), f2.to(dtype).double().numpy()
             ref_b = oc.corrblock_lookup(oc.pyramid(oc.corr_allpairs(r1, r2), 4), coords.astype(np.float64), 3)
-            assert rel_err(out.double().cpu().numpy(), ref_b) <= tol
+            e = rel_err(out.double().cpu().numpy(), ref_b)
+            assert e <= tol, e
             assert np.max(np.abs(out.double().cpu().numpy() - ref)) <= 2e-2
         else:
-            assert rel_err(out.double().cpu().numpy(), ref) <= tol
+            e = rel_err(out.double().cpu().numpy(), ref)
+            assert e <= tol, e
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
@@ -120,7 +125,8 @@ def test_altcorr_forward(db, dtype, r):
     ref = oc.altcorr_forward(f1.double().numpy(), f2.double().numpy(), coords.double().numpy(),
                              ii.numpy(), jj.numpy(), r)
     assert out.shape == (B, 5, 2 * r + 1, 2 * r + 1, H, W) and out.dtype == dtype
-    assert rel_err(out.double().cpu().numpy(), ref) <= (1e-5 if dtype == torch.float32 else 1e-2)
+    e = rel_err(out.double().cpu().numpy(), ref)
+    assert e <= (1e-5 if dtype == torch.float32 else 1e-2), e
 
 
 def test_altcorr_backward(db):
@@ -134,8 +140,10 @@ def test_altcorr_backward(db):
     g1, g2 = db.altcorr_backward(f1.cuda(), f2.cuda(), coords.cuda(), g.cuda(), ii.cuda(), jj.cuda(), r)
     r1, r2 = oc.altcorr_backward(f1.double().numpy(), f2.double().numpy(), coords.double().numpy(),
                                  g.double().numpy(), ii.numpy(), jj.numpy(), r)
-    assert rel_err(g1.double().cpu().numpy(), r1) <= 1e-5
-    assert rel_err(g2.double().cpu().numpy(), r2) <= 1e-5
+    e = rel_err(g1.double().cpu().numpy(), r1)
+    assert e <= 1e-5, e
+    e = rel_err(g2.double().cpu().numpy(), r2)
+    assert e <= 1e-5, e
 
 
 def test_altcorr_block_equals_corrblock(db):
@@ -149,4 +157,5 @@ def test_altcorr_block_equals_corrblock(db):
     blk = hc.CorrBlock(fm[:, ii], fm[:, jj], dtype=torch.float32)
     ref = blk(coords)
     assert alt.shape == ref.shape == (1, 3, 196, 16, 32)
-    assert rel_err(alt.double().cpu().numpy(), ref.double().cpu().numpy()) <= 1e-5
+    e = rel_err(alt.double().cpu().numpy(), ref.double().cpu().numpy())
+    assert e <= 1e-5, e

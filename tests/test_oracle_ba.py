@@ -231,3 +231,14 @@ def test_motion_only_keeps_depths():
     assert dz is None
     assert np.array_equal(disps, p.disps.astype(np.float64))
     assert np.abs(dx).max() > 0
+
+
+def test_oracle_reproduces_golden_fixture():
+    """Regression pin: the committed fixture equals what the oracle computes now."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'ba_small_24x32.npz'))
+    t0, t1, it, lm, ep = f['scalars']
+    poses, disps, dx, dz = oba.ba(f['poses'], f['disps'], f['intrinsics'], f['targets'], f['weights'],
+                                  f['eta'], f['ii'], f['jj'], int(t0), int(t1), int(it), lm, ep)
+    for a, b in ((poses, f['out_poses']), (disps, f['out_disps']), (dx, f['out_dx']), (dz, f['out_dz'])):
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-15)

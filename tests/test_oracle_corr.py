@@ -119,3 +119,11 @@ def test_altcorr_backward_is_adjoint():
     fp = np.sum(oc.altcorr_forward(f1 + eps * d1, f2 + eps * d2, coords, ii, jj, r) * g)
     fm = np.sum(oc.altcorr_forward(f1 - eps * d1, f2 - eps * d2, coords, ii, jj, r) * g)
     assert np.isclose((fp - fm) / (2 * eps), np.sum(g1 * d1) + np.sum(g2 * d2), rtol=1e-7)
+
+
+def test_oracle_reproduces_golden_fixture():
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'corrblock_8x16.npz'))
+    pyr = oc.pyramid(oc.corr_allpairs(f['fmap1'], f['fmap2']), 4)
+    out = oc.corrblock_lookup(pyr, f['coords'].astype(np.float64), 3)
+    assert np.allclose(out, f['out'], rtol=1e-12, atol=1e-15)
