@@ -13,6 +13,8 @@
 // pixels on the fast lane axis, so the [a][b][pixel] output rows are written fully
 // coalesced.  Taps and sums are kept in float (double for f64 volumes) and rounded
 // once on store.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace hsl2 {
@@ -25,17 +27,13 @@ template <typename T> struct Acc { using type = float; };
 template <> struct Acc<double> { using type = double; };
 
 template <typename T> __device__ __forceinline__ typename Acc<T>::type load_acc(const T* p) {
-  return static_cast<typename Acc<T>::type>(to_f<T>(*p));
+  if constexpr (std::is_same<T, double>::value) return *p;
+  else return to_f<T>(*p);
 }
-template <> __device__ __forceinline__ double load_acc<double>(const double* p) { return *p; }
 
-template <typename T> __device__ __forceinline__ T store_acc(float v) { return from_f<T>(v); }
-template <typename T> __device__ __forceinline__ T store_acc(double v) { return static_cast<T>(v); }
-template <> __device__ __forceinline__ __half store_acc<__half>(double v) {
-  return __float2half(static_cast<float>(v));
-}
-template <> __device__ __forceinline__ __hip_bfloat16 store_acc<__hip_bfloat16>(double v) {
-  return __float2bfloat16(static_cast<float>(v));
+template <typename T, typename A> __device__ __forceinline__ T store_acc(A v) {
+  if constexpr (std::is_same<T, double>::value) return static_cast<double>(v);
+  else return from_f<T>(static_cast<float>(v));
 }
 
 // Up to four pyramid levels looked up by one launch (blockIdx.y = level).
@@ -55,7 +53,7 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
                        int64_t out_batch_stride, int tiles_per_item) {
   using A = typename Acc<T>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  A* taps = reinterpret_cast<A*>(smem_raw);                 // [kPix][D*D]
+  A* taps = reinterpret_cast<A*>(smem_raw);                 // [kPix][D*D + 1]
   __shared__ int s_x0[kPix], s_y0[kPix];
   __shared__ float s_dx[kPix], s_dy[kPix];
 
@@ -67,6 +65,7 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   const int pix0 = (blockIdx.x % tiles_per_item) * kPix;
   const int D = 2 * r + 2;
   const int D2 = D * D;
+  const int TS = D2 + 1;        // padded per-pixel stride: pixels on the fast lane axis hit distinct banks
   const int npix = min(kPix, hw1 - pix0);
   const int t = threadIdx.x;
 
@@ -101,7 +100,7 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     const int x1 = s_x0[p] + tx;
     A v = 0;
     if (y1 >= 0 && y1 < h2 && x1 >= 0 && x1 < w2) v = load_acc<T>(vbase + p * plane + y1 * w2 + x1);
-    taps[p * D2 + tap] = v;
+    taps[p * TS + tap] = v;
   }
   __syncthreads();
 
@@ -112,10 +111,13 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     const int p = idx - o * npix;
     const int a = o / R;        // x offset
     const int b = o - a * R;    // y offset
-    const A fx = s_dx[p], fy = s_dy[p];
-    const A* tp = taps + p * D2;
-    const A v = (1 - fx) * (1 - fy) * tp[b * D + a] + fx * (1 - fy) * tp[b * D + a + 1] +
-                (1 - fx) * fy * tp[(b + 1) * D + a] + fx * fy * tp[(b + 1) * D + a + 1];
+    // bilinear weights in float, as the reference forms them (src/correlation_kernels.cu:41-63)
+    const float fx = s_dx[p], fy = s_dy[p];
+    const A w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy);
+    const A w01 = (1.f - fx) * fy, w11 = fx * fy;
+    const A* tp = taps + p * TS;
+    const A v = w00 * tp[b * D + a] + w10 * tp[b * D + a + 1] + w01 * tp[(b + 1) * D + a] +
+                w11 * tp[(b + 1) * D + a + 1];
     obase[(int64_t)o * hw1 + p] = store_acc<T>(v);
   }
 }
@@ -166,13 +168,15 @@ corr_lookup_bwd_kernel(const float* __restrict__ coords, const T* __restrict__ g
     const int tap = idx / npix;
     const int ty = tap / D;
     const int tx = tap - ty * D;
-    const A fx = s_dx[p], fy = s_dy[p];
+    const float fx = s_dx[p], fy = s_dy[p];
+    const A w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy);
+    const A w01 = (1.f - fx) * fy, w11 = fx * fy;
     A g = 0;
     // output (a, b) reads tap (a + i, b + j) with weight w_ij
-    if (tx < R && ty < R) g += (1 - fx) * (1 - fy) * load_acc<T>(gbase + (int64_t)(tx * R + ty) * hw1 + p);
-    if (tx > 0 && ty < R) g += fx * (1 - fy) * load_acc<T>(gbase + (int64_t)((tx - 1) * R + ty) * hw1 + p);
-    if (tx < R && ty > 0) g += (1 - fx) * fy * load_acc<T>(gbase + (int64_t)(tx * R + ty - 1) * hw1 + p);
-    if (tx > 0 && ty > 0) g += fx * fy * load_acc<T>(gbase + (int64_t)((tx - 1) * R + ty - 1) * hw1 + p);
+    if (tx < R && ty < R) g += w00 * load_acc<T>(gbase + (int64_t)(tx * R + ty) * hw1 + p);
+    if (tx > 0 && ty < R) g += w10 * load_acc<T>(gbase + (int64_t)((tx - 1) * R + ty) * hw1 + p);
+    if (tx < R && ty > 0) g += w01 * load_acc<T>(gbase + (int64_t)(tx * R + ty - 1) * hw1 + p);
+    if (tx > 0 && ty > 0) g += w11 * load_acc<T>(gbase + (int64_t)((tx - 1) * R + ty - 1) * hw1 + p);
     gt[p * D2 + tap] = g;
   }
   __syncthreads();
@@ -201,7 +205,7 @@ int launch_fwd(const LevelTable& lv, int levels, const float* coords, int64_t cb
   const int64_t blocks = batch * tiles;
   HSL2_REQUIRE(blocks < (1ll << 31), HSL2_ERR_SHAPE, "corr lookup: grid too large");
   const int D = 2 * r + 2;
-  const size_t lds = sizeof(typename Acc<T>::type) * kPix * D * D;
+  const size_t lds = sizeof(typename Acc<T>::type) * kPix * (D * D + 1);
   HSL2_REQUIRE(lds <= 64 * 1024, HSL2_ERR_UNSUPPORTED, "corr lookup: radius too large");
   if (blocks == 0) return HSL2_OK;
   hipLaunchKernelGGL(corr_lookup_fwd_kernel<T>, dim3(static_cast<unsigned>(blocks), levels),

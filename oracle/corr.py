@@ -52,12 +52,22 @@ def pyramid(corr, num_levels=4):
 
 
 def _taps(coords, r):
-    """Integer tap origins and fractional parts, as the kernels compute them."""
-    x0 = coords[:, 0].astype(np.float64)
-    y0 = coords[:, 1].astype(np.float64)
+    """Integer tap origins and fractional parts, as the kernels compute them: the
+    coordinates are float32 and ``dx = x0 - floor(x0)`` is a float32 subtraction
+    (src/correlation_kernels.cu:38-39), which rounds for x0 in (-1, 0)."""
+    x0 = np.asarray(coords[:, 0], np.float32)
+    y0 = np.asarray(coords[:, 1], np.float32)
     fx = np.floor(x0)
     fy = np.floor(y0)
     return fx.astype(np.int64) - r, fy.astype(np.int64) - r, x0 - fx, y0 - fy
+
+
+def _weights(dx, dy):
+    """Bilinear corner weights formed in float32 like the reference's
+    ``scalar_t(dx * dy)`` etc. (src/correlation_kernels.cu:50-63), then widened."""
+    one = np.float32(1.0)
+    w = [(one - dx) * (one - dy), dx * (one - dy), (one - dx) * dy, dx * dy]
+    return [np.asarray(v, np.float32).astype(np.float64) for v in w]
 
 
 def corr_index_forward(volume, coords, r):
@@ -78,10 +88,9 @@ def corr_index_forward(volume, coords, r):
             ok = (x1 >= 0) & (x1 < w2) & (y1 >= 0) & (y1 < h2)
             vals = vol[n_idx, y_idx, x_idx, np.clip(y1, 0, h2 - 1), np.clip(x1, 0, w2 - 1)]
             tap[:, i, j] = np.where(ok, vals, 0.0)
-    return (tap[:, :-1, :-1] * ((1 - dx) * (1 - dy))[:, None, None]
-            + tap[:, 1:, :-1] * (dx * (1 - dy))[:, None, None]
-            + tap[:, :-1, 1:] * ((1 - dx) * dy)[:, None, None]
-            + tap[:, 1:, 1:] * (dx * dy)[:, None, None])
+    w00, w10, w01, w11 = _weights(dx, dy)
+    return (tap[:, :-1, :-1] * w00[:, None, None] + tap[:, 1:, :-1] * w10[:, None, None]
+            + tap[:, :-1, 1:] * w01[:, None, None] + tap[:, 1:, 1:] * w11[:, None, None])
 
 
 def corr_index_backward(volume_shape, coords, grad, r):
@@ -90,11 +99,12 @@ def corr_index_backward(volume_shape, coords, grad, r):
     g = np.asarray(grad, np.float64)
     xb, yb, dx, dy = _taps(np.asarray(coords), r)
     D = 2 * r + 2
+    w00, w10, w01, w11 = _weights(dx, dy)
     gt = np.zeros((BN, D, D, h1, w1))
-    gt[:, :-1, :-1] += g * ((1 - dx) * (1 - dy))[:, None, None]
-    gt[:, 1:, :-1] += g * (dx * (1 - dy))[:, None, None]
-    gt[:, :-1, 1:] += g * ((1 - dx) * dy)[:, None, None]
-    gt[:, 1:, 1:] += g * (dx * dy)[:, None, None]
+    gt[:, :-1, :-1] += g * w00[:, None, None]
+    gt[:, 1:, :-1] += g * w10[:, None, None]
+    gt[:, :-1, 1:] += g * w01[:, None, None]
+    gt[:, 1:, 1:] += g * w11[:, None, None]
     out = np.zeros((BN, h1, w1, h2, w2))
     n_idx, y_idx, x_idx = np.meshgrid(np.arange(BN), np.arange(h1), np.arange(w1), indexing='ij')
     for i in range(D):
@@ -135,10 +145,10 @@ def altcorr_forward(fmap1, fmap2, coords, ii, jj, r):
         for m in range(M):
             a = f1[b, ii[m]]                          # [C,H,W]
             g = f2[b, jj[m]]                          # [C,H2,W2]
-            cx = np.asarray(coords[b, m, 0], np.float64)
-            cy = np.asarray(coords[b, m, 1], np.float64)
+            cx = np.asarray(coords[b, m, 0], np.float32)
+            cy = np.asarray(coords[b, m, 1], np.float32)
             fx0, fy0 = np.floor(cx), np.floor(cy)
-            dx, dy = cx - fx0, cy - fy0
+            w00, w10, w01, w11 = _weights(cx - fx0, cy - fy0)   # float32 dx, dy (altcorr_kernel.cu:160-163)
             tap = np.zeros((D, D, H, W))               # [y-offset][x-offset]
             for ty in range(D):
                 for tx in range(D):
@@ -147,8 +157,7 @@ def altcorr_forward(fmap1, fmap2, coords, ii, jj, r):
                     ok = (y1 >= 0) & (y1 < H2) & (x1 >= 0) & (x1 < W2)
                     gv = g[:, np.clip(y1, 0, H2 - 1), np.clip(x1, 0, W2 - 1)]
                     tap[ty, tx] = np.where(ok, np.sum(a * gv, axis=0), 0.0)
-            o = ((1 - dx) * (1 - dy) * tap[:-1, :-1] + dx * (1 - dy) * tap[:-1, 1:]
-                 + (1 - dx) * dy * tap[1:, :-1] + dx * dy * tap[1:, 1:])
+            o = w00 * tap[:-1, :-1] + w10 * tap[:-1, 1:] + w01 * tap[1:, :-1] + w11 * tap[1:, 1:]
             out[b, m] = np.swapaxes(o, 0, 1)          # permute(0,1,3,2,4,5)
     return out
 
@@ -170,15 +179,15 @@ def altcorr_backward(fmap1, fmap2, coords, grad, ii, jj, r):
     for b in range(B):
         for m in range(M):
             go = np.swapaxes(np.asarray(grad[b, m], np.float64), 0, 1)   # [y-off][x-off]
-            cx = np.asarray(coords[b, m, 0], np.float64)
-            cy = np.asarray(coords[b, m, 1], np.float64)
+            cx = np.asarray(coords[b, m, 0], np.float32)
+            cy = np.asarray(coords[b, m, 1], np.float32)
             fx0, fy0 = np.floor(cx), np.floor(cy)
-            dx, dy = cx - fx0, cy - fy0
+            w00, w10, w01, w11 = _weights(cx - fx0, cy - fy0)
             gt = np.zeros((D, D, H, W))
-            gt[:-1, :-1] += (1 - dx) * (1 - dy) * go
-            gt[:-1, 1:] += dx * (1 - dy) * go
-            gt[1:, :-1] += (1 - dx) * dy * go
-            gt[1:, 1:] += dx * dy * go
+            gt[:-1, :-1] += w00 * go
+            gt[:-1, 1:] += w10 * go
+            gt[1:, :-1] += w01 * go
+            gt[1:, 1:] += w11 * go
             yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing='ij')
             for ty in range(D):
                 for tx in range(D):

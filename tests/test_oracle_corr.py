@@ -61,10 +61,12 @@ def test_lookup_matches_grid_sample(r):
                        rng.uniform(-3, h2 + 2, size=(BN, h1, w1))], 1)
     coords[0, :, 0, 0] = [2.0, 3.0]                 # integer coordinates
     coords[1, :, 1, 1] = [-40.0, 100.0]             # far outside -> zeros
+    coords = coords.astype(np.float32).astype(np.float64)
     out = oc.corr_index_forward(vol, coords, r)
     ref = grid_sample_lookup(vol, coords, r)
     assert out.shape == (BN, 2 * r + 1, 2 * r + 1, h1, w1)
-    assert np.allclose(out, ref, atol=1e-12)
+    # the reference forms the bilinear weights in float32, grid_sample in float64
+    assert np.allclose(out, ref, atol=1e-6)
     assert np.all(out[1, :, :, 1, 1] == 0)
 
 
