@@ -39,6 +39,8 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
+    ap.add_argument('--only', choices=['all', 'lookup'], default='all',
+                    help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters)')
     return ap.parse_args()
 
 
@@ -78,6 +80,28 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+
+    if args.only == 'lookup':
+        f = fm32.to(torch.bfloat16)
+        blk = hcorr.CorrBlock(f[:, p.ii], f[:, p.jj], dtype=torch.bfloat16)
+        del f
+        for _ in range(args.warmup):
+            blk(coords)
+        torch.cuda.synchronize()
+        ev = []
+        for _ in range(args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            blk(coords)
+            e1.record()
+            ev.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        gbps = LOOKUP_BYTES_BF16 * lookups / (ms / 1e3) / 1e9
+        print(json.dumps({'metric': 'corr-lookup GB/s (algorithmic, 912 B per bf16 lookup)',
+                          'value': round(gbps, 1), 'unit': 'GB/s', 'lookup_ms': round(ms, 4),
+                          'lookups_per_launch': lookups, 'tiled': blk.tiled}), flush=True)
+        return
 
     def ba_call(poses, disps, precision):
         return db.ba_ex(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, iters, p.lm, p.ep,

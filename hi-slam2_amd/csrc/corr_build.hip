@@ -50,12 +50,18 @@ template <> struct Mfma<float> {
   }
 };
 
+// Offset of (y, x) in a plane of width w stored as row-major 8x8 tiles (the layout the
+// pyramid lookup reads when asked for tiled volumes).
+__device__ __forceinline__ int plane_offset(int y, int x, int w) {
+  return (((y >> 3) * (w >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
+}
+
 // grid: (p tiles, q tiles = (h/8) * (w/WT), batch)
 template <typename Ti, typename To>
 __global__ void __launch_bounds__(kThreads)
 corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __restrict__ l0,
                   To* __restrict__ l1, To* __restrict__ l2, To* __restrict__ l3, int C, int h,
-                  int w, int WT, int levels) {
+                  int w, int WT, int levels, int tiled) {
   constexpr int KP = kKC + 4;                      // padded LDS row (elements)
   extern __shared__ __align__(16) unsigned char smem[];
   Ti* s1 = reinterpret_cast<Ti*>(smem);            // [kTP][KP]
@@ -126,11 +132,21 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   }
   __syncthreads();
 
-  // level 0: rows of WT contiguous elements
+  // level 0: rows of WT contiguous elements (or whole 8x8 tiles in the tiled layout)
+  const bool tile0 = tiled && (h % 8 == 0) && (w % 8 == 0);
   for (int idx = t; idx < kTP * NQ; idx += kThreads) {
     const int p = idx / NQ, q = idx % NQ;
-    const int qy = y0 + q / WT, qx = x0 + q % WT;
-    l0[((int64_t)n * HW + p0 + p) * HW + qy * w + qx] = from_f<To>(sout[idx]);
+    int qy, qx;
+    if (tile0) {            // consecutive idx walk the tiles of the 8-row band in memory order
+      const int tx = q >> 6, e = q & 63;
+      qy = y0 + (e >> 3);
+      qx = x0 + tx * 8 + (e & 7);
+    } else {
+      qy = y0 + q / WT;
+      qx = x0 + q % WT;
+    }
+    const int off = tile0 ? plane_offset(qy, qx, w) : qy * w + qx;
+    l0[((int64_t)n * HW + p0 + p) * HW + off] = from_f<To>(sout[p * NQ + (qy - y0) * WT + (qx - x0)]);
   }
   // pooled levels: level L averages 2^L x 2^L blocks of level 0
   To* outs[3] = {l1, l2, l3};
@@ -147,14 +163,16 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
       for (int dy = 0; dy < s; ++dy)
         for (int dx = 0; dx < s; ++dx) sum += sout[p * NQ + (yy * s + dy) * WT + xx * s + dx];
       const int gy = (y0 >> L) + yy, gx = (x0 >> L) + xx;
-      outs[L - 1][((int64_t)n * HW + p0 + p) * hl * wl + gy * wl + gx] = from_f<To>(sum * inv);
+      const bool tl = tiled && (hl % 8 == 0) && (wl % 8 == 0);
+      outs[L - 1][((int64_t)n * HW + p0 + p) * hl * wl + (tl ? plane_offset(gy, gx, wl) : gy * wl + gx)] =
+          from_f<To>(sum * inv);
     }
   }
 }
 
 template <typename Ti, typename To>
 int launch(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
-           int64_t h, int64_t w, int L, hipStream_t s) {
+           int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
   const int WT = w % 64 == 0 ? 64 : static_cast<int>(w);
   HSL2_REQUIRE((kRows * WT) % 128 == 0, HSL2_ERR_UNSUPPORTED,
                "corr build: width must be a multiple of 64 or 16..63 with 8*w % 128 == 0");
@@ -170,7 +188,7 @@ int launch(const void* f1, const void* f2, void* const* levels, int64_t batch, i
   for (int i = 0; i < L; ++i) l[i] = static_cast<To*>(levels[i]);
   hipLaunchKernelGGL((corr_build_kernel<Ti, To>), grid, dim3(kThreads), lds, s,
                      static_cast<const Ti*>(f1), static_cast<const Ti*>(f2), l[0], l[1], l[2], l[3],
-                     static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), WT, L);
+                     static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), WT, L, tiled);
   HSL2_LAUNCH_CHECK();
   return HSL2_OK;
 }
@@ -182,7 +200,7 @@ using namespace hsl2;
 
 extern "C" int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype,
                                void* const* levels, int out_dtype, int64_t batch, int64_t channels,
-                               int64_t h, int64_t w, int num_levels, void* stream) {
+                               int64_t h, int64_t w, int num_levels, int tiled, void* stream) {
   HSL2_REQUIRE(fmap1 && fmap2 && levels, HSL2_ERR_ARGUMENT, "corr build: null pointer");
   HSL2_REQUIRE(num_levels >= 1 && num_levels <= 4, HSL2_ERR_ARGUMENT, "corr build: 1..4 levels");
   for (int i = 0; i < num_levels; ++i)
@@ -195,15 +213,15 @@ extern "C" int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtyp
   HSL2_REQUIRE(batch < 65536, HSL2_ERR_SHAPE, "corr build: batch too large for one launch");
   hipStream_t s = as_stream(stream);
   if (out_dtype == HSL2_BF16) {
-    if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
-    if (in_dtype == HSL2_F16) return launch<__half, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
-    if (in_dtype == HSL2_F32) return launch<float, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
+    if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
+    if (in_dtype == HSL2_F16) return launch<__half, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
+    if (in_dtype == HSL2_F32) return launch<float, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
   } else if (out_dtype == HSL2_F32) {
-    if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, float>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
-    if (in_dtype == HSL2_F16) return launch<__half, float>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
-    if (in_dtype == HSL2_F32) return launch<float, float>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
+    if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, float>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
+    if (in_dtype == HSL2_F16) return launch<__half, float>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
+    if (in_dtype == HSL2_F32) return launch<float, float>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
   } else if (out_dtype == HSL2_F16) {
-    if (in_dtype == HSL2_F16) return launch<__half, __half>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, s);
+    if (in_dtype == HSL2_F16) return launch<__half, __half>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
   }
   set_error("corr build: unsupported dtype combination");
   return HSL2_ERR_UNSUPPORTED;

@@ -40,9 +40,17 @@ template <typename T, typename A> __device__ __forceinline__ T store_acc(A v) {
 struct LevelTable {
   const void* vol[4];
   int h2[4], w2[4];
+  int tiled[4];            // 1: plane stored in 8x8 tiles (see tiled_offset)
   float scale[4];          // coordinate scale of the level (1 / 2^level)
   int64_t out_offset[4];   // element offset of the level's first output channel
 };
+
+// Offset of (y, x) inside an h2 x w2 plane stored as row-major 8x8 tiles, each tile
+// row-major (64 contiguous elements).  A radius-3 window (8x8 taps) then touches at most
+// four 64-element tiles instead of eight plane rows.
+__device__ __forceinline__ int tiled_offset(int y, int x, int w2) {
+  return (((y >> 3) * (w2 >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
+}
 
 // Coordinates of pixel p of item n: x = coords[n * cb + p * cp], y = x-address + cy.
 // Output: out + n * out_batch_stride + level offset + o * hw1 + pix, o = a * (2r+1) + b.
@@ -60,6 +68,7 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   const int level = blockIdx.y;
   const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
   const int h2 = lv.h2[level], w2 = lv.w2[level];
+  const bool tiled = lv.tiled[level] != 0;
   const float scale = lv.scale[level];
   const int n = blockIdx.x / tiles_per_item;
   const int pix0 = (blockIdx.x % tiles_per_item) * kPix;
@@ -99,7 +108,8 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     const int y1 = s_y0[p] + ty;
     const int x1 = s_x0[p] + tx;
     A v = 0;
-    if (y1 >= 0 && y1 < h2 && x1 >= 0 && x1 < w2) v = load_acc<T>(vbase + p * plane + y1 * w2 + x1);
+    if (y1 >= 0 && y1 < h2 && x1 >= 0 && x1 < w2)
+      v = load_acc<T>(vbase + p * plane + (tiled ? tiled_offset(y1, x1, w2) : y1 * w2 + x1));
     taps[p * TS + tap] = v;
   }
   __syncthreads();
@@ -271,6 +281,7 @@ extern "C" int hsl2_corr_index_forward(const void* volume, const float* coords, 
   lv.vol[0] = volume;
   lv.h2[0] = static_cast<int>(h2);
   lv.w2[0] = static_cast<int>(w2);
+  lv.tiled[0] = 0;
   lv.scale[0] = 1.0f;
   lv.out_offset[0] = 0;
   // planar coords [batch][2][h1*w1]
@@ -280,7 +291,8 @@ extern "C" int hsl2_corr_index_forward(const void* volume, const float* coords, 
 
 extern "C" int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels,
                                         const float* coords, void* corr, int dtype, int64_t batch,
-                                        int64_t h1, int64_t w1, int radius, void* stream) {
+                                        int64_t h1, int64_t w1, int radius, int tiled,
+                                        void* stream) {
   HSL2_REQUIRE(levels && num_levels >= 1 && num_levels <= 4, HSL2_ERR_ARGUMENT,
                "pyramid lookup: 1..4 levels");
   LevelTable lv{};
@@ -292,6 +304,7 @@ extern "C" int hsl2_corr_pyramid_lookup(const void* const* levels, int num_level
     lv.vol[i] = levels[i];
     lv.h2[i] = static_cast<int>(h2);
     lv.w2[i] = static_cast<int>(w2);
+    lv.tiled[i] = (tiled && h2 % 8 == 0 && w2 % 8 == 0) ? 1 : 0;
     lv.scale[i] = 1.0f / static_cast<float>(1 << i);
     lv.out_offset[i] = i * R * R * h1 * w1;
   }

@@ -203,7 +203,8 @@ std::vector<torch::Tensor> corr_index_backward(torch::Tensor volume, torch::Tens
 }
 
 // Fused CorrBlock.__call__ (hislam2/modules/corr.py:40-50): all levels, one launch.
-torch::Tensor corr_pyramid_lookup(std::vector<torch::Tensor> levels, torch::Tensor coords, int radius) {
+torch::Tensor corr_pyramid_lookup(std::vector<torch::Tensor> levels, torch::Tensor coords, int radius,
+                                  bool tiled) {
   CHECK_CONTIGUOUS(coords);
   require(!levels.empty() && levels.size() <= 4, "corr_pyramid_lookup: 1..4 levels");
   require(coords.dim() == 5 && coords.size(4) == 2 && coords.scalar_type() == torch::kFloat32,
@@ -225,7 +226,7 @@ torch::Tensor corr_pyramid_lookup(std::vector<torch::Tensor> levels, torch::Tens
                                    levels[0].options());
   raise_status(hsl2_corr_pyramid_lookup(ptrs.data(), static_cast<int>(levels.size()), coords.data_ptr<float>(),
                                         out.data_ptr(), dtype_code(levels[0]), B * N, h, w, radius,
-                                        current_stream()),
+                                        tiled ? 1 : 0, current_stream()),
                "corr_pyramid_lookup");
   return out;
 }
@@ -315,7 +316,7 @@ std::vector<torch::Tensor> altcorr_backward(torch::Tensor fmap1, torch::Tensor f
 
 // ---- correlation build (hislam2/modules/corr.py:24-38, 63-71) ---------------------------
 std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, int num_levels,
-                                      py::object out_dtype) {
+                                      py::object out_dtype, bool tiled) {
   CHECK_CONTIGUOUS(fmap1);
   CHECK_CONTIGUOUS(fmap2);
   require_device(fmap1, "fmap1");
@@ -333,7 +334,7 @@ std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, 
   }
   int odc = od == torch::kBFloat16 ? HSL2_BF16 : od == torch::kFloat32 ? HSL2_F32 : od == torch::kFloat16 ? HSL2_F16 : -1;
   raise_status(hsl2_corr_build(fmap1.data_ptr(), fmap2.data_ptr(), dtype_code(fmap1), ptrs.data(), odc, B * N, C,
-                               h, w, num_levels, current_stream()),
+                               h, w, num_levels, tiled ? 1 : 0, current_stream()),
                "corr_build");
   return levels;
 }
@@ -352,9 +353,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   // additions of this build
   m.def("ba_ex", &ba_ex, "ba with an explicit precision mode (0 split, 1 all-FP32); returns [dx, dz, None, ok]");
   m.def("ba_index", &ba_index, "kx and kk_exp of a ba call (host-side planner)");
-  m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch");
+  m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch",
+        py::arg("levels"), py::arg("coords"), py::arg("radius"), py::arg("tiled") = false);
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
-        py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none());
+        py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none(),
+        py::arg("tiled") = false);
   m.attr("PRECISION_SPLIT") = py::int_(static_cast<int>(HSL2_BA_SPLIT));
   m.attr("PRECISION_FP32") = py::int_(static_cast<int>(HSL2_BA_FP32));
 }

@@ -78,22 +78,32 @@ class CorrBuild(torch.autograd.Function):
 
 
 class CorrBlock:
-    """corr.py:23-71.  ``dtype`` selects the volume element type (default: fmap dtype)."""
+    """corr.py:23-71.  ``dtype`` selects the volume element type (default: fmap dtype).
 
-    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, dtype=None):
+    Without autograd the levels are stored as 8x8 tiles (``self.tiled``), which the
+    fused lookup reads with half the memory transactions of row-major planes; with
+    autograd (feature maps that require grad) the reference's row-major planes are
+    kept so that ``CorrSampler`` and its adjoint apply.  ``cat``/``__getitem__`` act
+    on the first dimension and work for both layouts."""
+
+    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, dtype=None, tiled=None):
         self.num_levels = num_levels
         self.radius = radius
-        if torch.is_grad_enabled() and (fmap1.requires_grad or fmap2.requires_grad):
+        grad = torch.is_grad_enabled() and (fmap1.requires_grad or fmap2.requires_grad)
+        self.tiled = (not grad) if tiled is None else bool(tiled)
+        if grad:
+            if self.tiled:
+                raise ValueError('the tiled layout has no autograd path; use tiled=False')
             self.corr_pyramid = list(CorrBuild.apply(fmap1, fmap2, num_levels, dtype))
         else:
             self.corr_pyramid = _db.corr_build(fmap1.contiguous(), fmap2.contiguous(),
-                                               num_levels, dtype)
+                                               num_levels, dtype, self.tiled)
 
     def __call__(self, coords):
         if not (torch.is_grad_enabled() and any(v.requires_grad for v in self.corr_pyramid)):
             # all levels in one launch, written straight into the concatenated output
             return _db.corr_pyramid_lookup(self.corr_pyramid, coords.float().contiguous(),
-                                           self.radius)
+                                           self.radius, self.tiled)
         out_pyramid = []
         batch, num, ht, wd, _ = coords.shape
         coords = coords.permute(0, 1, 4, 2, 3)
@@ -117,7 +127,7 @@ class CorrBlock:
     def corr(fmap1, fmap2):
         """All-pairs correlation [B, N, h, w, h, w] (corr.py:63-71), level 0 of corr_build."""
         batch, num, dim, ht, wd = fmap1.shape
-        lvl0, = _db.corr_build(fmap1.contiguous(), fmap2.contiguous(), 1, None)
+        lvl0, = _db.corr_build(fmap1.contiguous(), fmap2.contiguous(), 1, None, False)
         return lvl0.view(batch, num, ht, wd, ht, wd)
 
 

@@ -71,21 +71,25 @@ int hsl2_corr_index_backward(const float* coords, const void* corr_grad, void* v
  * level on coords / 2^level, then torch.cat) with one launch.  levels[i] is the level-i volume
  * [batch, h1, w1, h1 >> i, w1 >> i] of `dtype`; coords [batch, h1, w1, 2] f32 (x, y
  * interleaved, the CorrBlock call layout); corr [batch, L*(2r+1)^2, h1, w1] of `dtype`, channel
- * order [level][x-offset][y-offset], fully written. */
+ * order [level][x-offset][y-offset], fully written.  tiled != 0: every level whose plane sides
+ * are multiples of 8 is stored as row-major 8x8 tiles (as hsl2_corr_build writes it with
+ * tiled != 0); other levels are row-major. */
 int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const float* coords,
                              void* corr, int dtype, int64_t batch, int64_t h1, int64_t w1,
-                             int radius, void* stream);
+                             int radius, int tiled, void* stream);
 
 /* ---- correlation build ------------------------------------------------------ */
 
 /* Replaces CorrBlock.__init__ + CorrBlock.corr (hislam2/modules/corr.py:24-38, 63-71):
  * all-pairs correlation (f1/4)^T (f2/4) of fmap1/fmap2 [batch, C, h, w] (`in_dtype`
  * F32/BF16/F16) followed by `num_levels` 2x2 average-pool levels.  levels[i] points at
- * [batch, h, w, h >> i, w >> i] of `out_dtype` (F32/BF16).  Requires h, w divisible by
- * 2^(num_levels-1) and C a multiple of 32. */
+ * [batch, h, w, h >> i, w >> i] of `out_dtype` (F32/BF16).  Requires h, w multiples of 8 and
+ * C a multiple of 32.  tiled != 0 stores every level plane whose sides are multiples of 8 as
+ * row-major 8x8 tiles (read by hsl2_corr_pyramid_lookup with tiled != 0); tiled == 0 gives the
+ * reference's row-major planes. */
 int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype, void* const* levels,
                     int out_dtype, int64_t batch, int64_t channels, int64_t h, int64_t w,
-                    int num_levels, void* stream);
+                    int num_levels, int tiled, void* stream);
 
 /* ---- on-the-fly correlation ------------------------------------------------- */
 

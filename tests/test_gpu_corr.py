@@ -86,6 +86,33 @@ def test_corr_build_pyramid(db, hw, dtype):
             assert np.max(np.abs(got - ref[i])) <= 2e-2
 
 
+def untile(level):
+    """[..., h, w] stored as row-major 8x8 tiles -> row-major planes (numpy)."""
+    a = np.asarray(level)
+    h, w = a.shape[-2:]
+    flat = a.reshape(a.shape[:-2] + (h // 8, w // 8, 8, 8))
+    return np.swapaxes(flat, -3, -2).reshape(a.shape)
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
+def test_corr_build_tiled_layout(db, dtype):
+    rng = np.random.default_rng(31)
+    f1 = rounded(rng.standard_normal((1, 2, 64, 48, 64)) / 8, dtype).cuda()
+    f2 = rounded(rng.standard_normal((1, 2, 64, 48, 64)) / 8, dtype).cuda()
+    rm = db.corr_build(f1, f2, 4, dtype, False)
+    tl = db.corr_build(f1, f2, 4, dtype, True)
+    for i in range(4):
+        a, b = rm[i].float().cpu().numpy(), tl[i].float().cpu().numpy()
+        h2, w2 = a.shape[-2:]
+        if h2 % 8 == 0 and w2 % 8 == 0:
+            b = untile(b)
+        assert np.array_equal(a, b), i
+    coords = torch.from_numpy(np.random.default_rng(32).uniform(-5, 70, (1, 2, 48, 64, 2)).astype(np.float32)).cuda()
+    o_rm = db.corr_pyramid_lookup(rm, coords, 3, False)
+    o_tl = db.corr_pyramid_lookup(tl, coords, 3, True)
+    assert torch.equal(o_rm, o_tl)
+
+
 def test_corrblock_small_config(db):
     """4-level radius-3 CorrBlock lookup at 48x64 (196 channels), fp32 and bf16 volumes."""
     from hislam2_amd import corr as hc
