@@ -21,8 +21,8 @@ INCLUDE = os.path.join(ROOT, 'include')
 ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 ARCH = 'gfx950'
 
-LIB_SOURCES = ['status.cpp', 'ba_plan.cpp', 'ba.hip', 'corr_lookup.hip', 'corr_build.hip',
-               'altcorr.hip']
+LIB_SOURCES = ['status.cpp', 'ba_plan.cpp', 'ba.hip', 'chol.hip', 'corr_lookup.hip',
+               'corr_build.hip', 'altcorr.hip']
 
 HIP_FLAGS = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}', '-mcode-object-version=5',
@@ -59,11 +59,10 @@ def compile_lib(jobs=8):
         list(ex.map(run, cmds))
     tl = torch_lib_dir()
     out = os.path.join(HERE, 'libhsl2.so')
-    # Resolve the HIP runtime and rocSOLVER from torch's bundled ROCm first so that a
-    # process that also imports torch loads one runtime only.
+    # Resolve the HIP runtime from torch's bundled ROCm first so that a process that also
+    # imports torch loads one runtime only.
     run([hipcc, '-shared', f'--offload-arch={ARCH}', '-o', out + '.tmp'] + objs +
-        [f'-L{tl}', f'-Wl,-rpath,{tl}', f'-L{ROCM}/lib', f'-Wl,-rpath,{ROCM}/lib',
-         '-lrocsolver', '-lrocblas', '-lamdhip64'])
+        [f'-L{tl}', f'-Wl,-rpath,{tl}', f'-L{ROCM}/lib', f'-Wl,-rpath,{ROCM}/lib', '-lamdhip64'])
     os.replace(out + '.tmp', out)
     return out
 

@@ -13,20 +13,21 @@
 //                        (EEt6x6 / Ev6x1, 1466-1560).
 //   4. ba_assemble       the damped reduced camera system A - S, b - bS in FP64
 //                        (FP32 in the plain mode), written densely in column-major
-//                        order (SparseBlock, 1584-1686).
-//   5. rocSOLVER potrf / potrs; info != 0 gives dx = 0 (1669-1677).
+//                        order, padded to a multiple of 64 with an identity block
+//                        (SparseBlock, 1584-1686).
+//   5. cholesky_solve    tiled Cholesky factorisation and solve (chol.hip); a non-positive
+//                        pivot gives dx = 0 (1669-1677).
 //   6. ba_backsub        dz = Q (w - sum E^T dx) with the reference's t0 skip, and
 //                        the disparity update (EvT6x1, disp_retr, 1562-1582, 1398-1411).
 //   7. ba_pose_retr      T <- Exp(dx) T (1342-1396).
-#include <mutex>
-#include <unordered_map>
-
-#include <rocsolver/rocsolver.h>
-
 #include "ba_plan.hpp"
 #include "common.hpp"
 
 namespace hsl2 {
+
+template <typename R>
+int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_t s);
+
 namespace {
 
 constexpr float kMinDepth = 0.25f;   // src/droid_kernels.cu:26
@@ -328,46 +329,55 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
             const int* __restrict__ blk_ptr, const int* __restrict__ blk_src, int E, int P,
-            float lm, float ep, R* __restrict__ A) {
+            int64_t n_pad, float lm, float ep, R* __restrict__ A, int* __restrict__ info) {
   const int64_t n = 6 * (int64_t)P;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int64_t col = idx / n;
-  const int64_t row = idx - col * n;
+  if (idx == 0) info[0] = 0;                     // this iteration's factorisation status
+  if (idx >= n_pad * n_pad) return;
+  const int64_t col = idx / n_pad;
+  const int64_t row = idx - col * n_pad;
+  if (row >= n || col >= n) {                    // identity padding up to a multiple of 64
+    A[idx] = row == col ? R(1) : R(0);
+    return;
+  }
   const int a = static_cast<int>(row / 6), i = static_cast<int>(row % 6);
   const int b = static_cast<int>(col / 6), j = static_cast<int>(col % 6);
   const int64_t blk = (int64_t)a * P + b;
-  R v = 0;
+  double v = 0;
   for (int q = blk_ptr[blk]; q < blk_ptr[blk + 1]; ++q) {
     const int code = blk_src[q];
     const int kind = code & 7;
     const int64_t id = code >> 3;
-    if (kind < 4) v += static_cast<R>(Hs[((int64_t)kind * E + id) * 36 + i * 6 + j]);
-    else if (kind == kBlkS) v -= static_cast<R>(S[id * 36 + i * 6 + j]);
-    else v -= static_cast<R>(S[id * 36 + j * 6 + i]);
+    if (kind < 4) v += Hs[((int64_t)kind * E + id) * 36 + i * 6 + j];
+    else if (kind == kBlkS) v -= S[id * 36 + i * 6 + j];
+    else v -= S[id * 36 + j * 6 + i];
   }
-  if (row == col) v += static_cast<R>(ep) + static_cast<R>(lm) * v;   // src/droid_kernels.cu:1663-1664
-  A[idx] = v;
+  if (row == col) v += static_cast<double>(ep) + static_cast<double>(lm) * v;   // src/droid_kernels.cu:1663-1664
+  A[idx] = static_cast<R>(v);
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256)
 ba_assemble_rhs(const double* __restrict__ vs, const double* __restrict__ Ev,
                 const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
-                R* __restrict__ bvec) {
+                int n_pad, R* __restrict__ bvec) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 6 * P) return;
+  if (idx >= n_pad) return;
+  if (idx >= 6 * P) {
+    bvec[idx] = R(0);
+    return;
+  }
   const int a = idx / 6, i = idx % 6;
-  R v = 0;
+  double v = 0;
   for (int q = rhs_ptr[a]; q < rhs_ptr[a + 1]; ++q) {
     const int code = rhs_src[q];
     const int kind = code & 3;
     const int64_t id = code >> 2;
-    if (kind == kRhsVi) v += static_cast<R>(vs[id * 6 + i]);
-    else if (kind == kRhsVj) v += static_cast<R>(vs[((int64_t)E + id) * 6 + i]);
-    else v -= static_cast<R>(Ev[id * 6 + i]);
+    if (kind == kRhsVi) v += vs[id * 6 + i];
+    else if (kind == kRhsVj) v += vs[((int64_t)E + id) * 6 + i];
+    else v -= Ev[id * 6 + i];
   }
-  bvec[idx] = v;
+  bvec[idx] = static_cast<R>(v);
 }
 
 // ---- 6. back-substitution and disparity update -------------------------------
@@ -468,36 +478,6 @@ __global__ void ba_clear_flags(int* info) {
   if (threadIdx.x < 4) info[threadIdx.x] = 0;
 }
 
-// ---- rocSOLVER handle cache (one per device) ------------------------------------
-rocblas_handle solver_handle(int device) {
-  static std::mutex mu;
-  static std::unordered_map<int, rocblas_handle> handles;
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = handles.find(device);
-  if (it != handles.end()) return it->second;
-  rocblas_handle h = nullptr;
-  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-  handles[device] = h;
-  return h;
-}
-
-template <typename R>
-int solve_reduced(rocblas_handle h, int n, R* A, R* b, int* info);
-
-template <>
-int solve_reduced<double>(rocblas_handle h, int n, double* A, double* b, int* info) {
-  if (rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, info) != rocblas_status_success) return 1;
-  if (rocsolver_dpotrs(h, rocblas_fill_lower, n, 1, A, n, b, n) != rocblas_status_success) return 1;
-  return 0;
-}
-
-template <>
-int solve_reduced<float>(rocblas_handle h, int n, float* A, float* b, int* info) {
-  if (rocsolver_spotrf(h, rocblas_fill_lower, n, A, n, info) != rocblas_status_success) return 1;
-  if (rocsolver_spotrs(h, rocblas_fill_lower, n, 1, A, n, b, n) != rocblas_status_success) return 1;
-  return 0;
-}
-
 template <typename R>
 int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const float* intr,
                    const float* targets, const float* weights, const float* eta, int eta_rows,
@@ -516,19 +496,14 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
   double* Ev = reinterpret_cast<double*>(ws + p->w_Ev);
   R* A = reinterpret_cast<R*>(ws + p->w_A);
   R* bvec = reinterpret_cast<R*>(ws + p->w_b);
+  R* xvec = reinterpret_cast<R*>(ws + p->w_x);
+  R* Ld = reinterpret_cast<R*>(ws + p->w_Ld);
   int* info = reinterpret_cast<int*>(ws + p->w_info);
   const int E = static_cast<int>(p->E), P = p->P, HW = static_cast<int>(p->HW);
   const int F = static_cast<int>(p->F);
   const int wd = static_cast<int>(p->wd);
   const bool schur = !p->motion_only;
-  const int n = 6 * P;
-
-  int dev = 0;
-  HSL2_HIP(hipGetDevice(&dev));
-  rocblas_handle h = solver_handle(dev);
-  HSL2_REQUIRE(h != nullptr, HSL2_ERR_HIP, "ba: rocblas_create_handle failed");
-  HSL2_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, HSL2_ERR_HIP,
-               "ba: rocblas_set_stream failed");
+  const int64_t n_pad = p->n_pad;
 
   hipLaunchKernelGGL(ba_clear_flags, dim3(1), dim3(64), 0, s, info);
   HSL2_LAUNCH_CHECK();
@@ -560,22 +535,26 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
         HSL2_LAUNCH_CHECK();
       }
     }
-    const int64_t nn = (int64_t)n * n;
+    const int64_t nn = n_pad * n_pad;
     hipLaunchKernelGGL(ba_assemble<R>, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, s,
-                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, lm, ep, A);
+                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, A, info);
     HSL2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3((n + 255) / 256), dim3(256), 0, s, vs, Ev,
-                       blob + p->o_rhs_ptr, blob + p->o_rhs_src, E, P, bvec);
+    hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3(static_cast<unsigned>((n_pad + 255) / 256)), dim3(256), 0, s,
+                       vs, Ev, blob + p->o_rhs_ptr, blob + p->o_rhs_src, E, P,
+                       static_cast<int>(n_pad), bvec);
     HSL2_LAUNCH_CHECK();
-    HSL2_REQUIRE(solve_reduced<R>(h, n, A, bvec, info) == 0, HSL2_ERR_HIP, "ba: rocSOLVER call failed");
+    {
+      const int st = cholesky_solve<R>(A, n_pad, Ld, bvec, xvec, info, s);
+      if (st) return st;
+    }
     if (schur) {
       hipLaunchKernelGGL(ba_backsub<R>, dim3((HW + 255) / 256, F), dim3(256), 0, s, Eall, Q, wv,
-                         bvec, info, blob + p->o_slot_ptr, blob + p->o_slot_row,
+                         xvec, info, blob + p->o_slot_ptr, blob + p->o_slot_row,
                          blob + p->o_slot_pose, blob + p->o_frame_kx, HW, disps,
                          last ? dz : nullptr);
       HSL2_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(ba_pose_retr<R>, dim3((P + 63) / 64), dim3(64), 0, s, poses, bvec, info,
+    hipLaunchKernelGGL(ba_pose_retr<R>, dim3((P + 63) / 64), dim3(64), 0, s, poses, xvec, info,
                        p->t0, P, last ? dx : nullptr);
     HSL2_LAUNCH_CHECK();
   }

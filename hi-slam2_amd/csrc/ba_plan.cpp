@@ -215,8 +215,11 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
   p->w_vs = take(2 * E * 6 * f8);
   p->w_S = take(p->NPAIR * 36 * f8);
   p->w_Ev = take(p->NS * 6 * f8);
-  p->w_A = take(36 * P * P * real);
-  p->w_b = take(6 * P * real);
+  p->n_pad = ((6 * P + 63) / 64) * 64;
+  p->w_A = take(static_cast<size_t>(p->n_pad) * p->n_pad * real);
+  p->w_b = take(static_cast<size_t>(p->n_pad) * real);
+  p->w_x = take(static_cast<size_t>(p->n_pad) * real);
+  p->w_Ld = take(static_cast<size_t>(p->n_pad) * 64 * real);
   p->w_info = take(4 * sizeof(int32_t));
   p->total = off;
   *out = p;

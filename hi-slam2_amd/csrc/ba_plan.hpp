@@ -48,8 +48,11 @@ struct hsl2_ba_plan {
   size_t w_vs = 0;       // f64 [2][E][6]
   size_t w_S = 0;        // f64 [NPAIR][6][6]
   size_t w_Ev = 0;       // f64 [NS][6]
-  size_t w_A = 0;        // f64 or f32 [6P][6P], column-major
-  size_t w_b = 0;        // f64 or f32 [6P]
+  int64_t n_pad = 0;     // 6P rounded up to a multiple of 64 (identity padding)
+  size_t w_A = 0;        // f64 or f32 [n_pad][n_pad], column-major
+  size_t w_b = 0;        // f64 or f32 [n_pad]: rhs, then the forward-substitution result
+  size_t w_x = 0;        // f64 or f32 [n_pad]: solution
+  size_t w_Ld = 0;       // f64 or f32 [n_pad / 64][64][64]: diagonal Cholesky factors
   size_t w_info = 0;     // int32 [2]: solver info, all-ok flag
   size_t total = 0;
 };

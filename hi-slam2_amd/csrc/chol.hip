@@ -1,0 +1,322 @@
+// Dense Cholesky factorisation and solve of the reduced camera system on the GPU.
+//
+// Replaces the host-side Eigen SimplicialLLT of SparseBlock::solve
+// (src/droid_kernels.cu:1659-1680).  The system is held densely, column-major, padded to a
+// multiple of 64 with an identity block (so every tile is full) and factored in place by a
+// right-looking tiled algorithm over 64x64 tiles:
+//
+//   for each tile column k:
+//     chol_panel   one block per tile row i >= k: every block factors the diagonal tile in
+//                  LDS (redundantly, so no inter-block wait is needed); block 0 stores L_kk
+//                  in a separate diagonal-factor buffer (A_kk is still being read by the
+//                  other blocks) and y_k = L_kk^-1 b_k, the others store L_ik = A_ik L_kk^-T
+//     chol_update  one block per lower trailing tile (i, j), k < j <= i: A_ij -= L_ik L_jk^T,
+//                  plus one block per tile row i > k: b_i -= L_ik y_k
+//   for each tile row k from the bottom:
+//     chol_back    block 0 writes x_k = L_kk^-T y_k; blocks 1..k recompute x_k and apply
+//                  y_j -= L_kj^T x_k for j < k
+//
+// A pivot that is not positive (or NaN) sets info = 1, as Eigen's LLT reports a
+// NumericalIssue; the caller then treats the solution as zero.  Templated on the element
+// type: double for the split-precision path, float for the plain FP32 path.
+#include "common.hpp"
+
+namespace hsl2 {
+
+constexpr int kTile = 64;
+constexpr int kLd = kTile + 1;      // padded LDS row
+
+namespace {
+
+// Load the lower triangle of the diagonal tile (k, k) and factor it in LDS (L overwrites A).
+template <typename R>
+__device__ void factor_diag(const R* __restrict__ A, int64_t lda, int k, R (*L)[kLd], bool report,
+                            int* __restrict__ info) {
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)k * kTile;
+  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
+    const int c = idx / kTile, r = idx % kTile;       // column-major walk: coalesced
+    L[r][c] = r >= c ? A[(base + r) + (base + c) * lda] : R(0);
+  }
+  __syncthreads();
+  const int s = t & 63, rg = t >> 6;
+  for (int c = 0; c < kTile; ++c) {
+    if (t == 0) {
+      R d = L[c][c];
+      if (!(d > R(0))) {          // not positive definite (or NaN): report and keep going safely
+        if (report) info[0] = 1;
+        d = R(1);
+      }
+      L[c][c] = sqrt(d);
+    }
+    __syncthreads();
+    if (t > c && t < kTile) L[t][c] /= L[c][c];
+    __syncthreads();
+    // trailing update of the lower triangle: rows r > c, columns c < s <= r
+    if (s > c) {
+      const R lsc = L[s][c];
+      for (int r = c + 1 + rg; r < kTile; r += 4)
+        if (s <= r) L[r][s] -= L[r][c] * lsc;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
+           int* __restrict__ info) {
+  __shared__ R L[kTile][kLd];
+  __shared__ R X[kTile][kLd];
+  const int t = threadIdx.x;
+  const int i = k + blockIdx.x;
+  factor_diag<R>(A, lda, k, L, blockIdx.x == 0, info);
+  const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
+  if (blockIdx.x == 0) {
+    R* Lk = Ld + (int64_t)k * kTile * kTile;          // column-major 64x64 factor of tile k
+    for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
+      const int c = idx / kTile, r = idx % kTile;
+      Lk[idx] = L[r][c];
+    }
+    // y_k = L_kk^-1 b_k by forward substitution
+    __shared__ R y[kTile];
+    if (t < kTile) y[t] = b[bk + t];
+    __syncthreads();
+    for (int c = 0; c < kTile; ++c) {
+      if (t == 0) y[c] /= L[c][c];
+      __syncthreads();
+      if (t > c && t < kTile) y[t] -= L[t][c] * y[c];
+      __syncthreads();
+    }
+    if (t < kTile) b[bk + t] = y[t];
+    return;
+  }
+  // L_ik = A_ik L_kk^-T: solve X L_kk^T = A_ik column by column (right-looking)
+  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
+    const int c = idx / kTile, r = idx % kTile;
+    X[r][c] = A[(bi + r) + (bk + c) * lda];
+  }
+  __syncthreads();
+  const int r = t & 63, mg = t >> 6;
+  for (int c = 0; c < kTile; ++c) {
+    if (t < kTile) X[t][c] /= L[c][c];
+    __syncthreads();
+    const R xrc = X[r][c];
+    for (int m = c + 1 + mg; m < kTile; m += 4) X[r][m] -= xrc * L[m][c];
+    __syncthreads();
+  }
+  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
+    const int c = idx / kTile, rr = idx % kTile;
+    A[(bi + rr) + (bk + c) * lda] = X[rr][c];
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
+  __shared__ R Li[kTile][kLd];
+  __shared__ R Lj[kTile][kLd];
+  const int t = threadIdx.x;
+  const int ntiles = m * (m + 1) / 2;
+  const int task = blockIdx.x;
+  const int64_t bk = (int64_t)k * kTile;
+  if (task >= ntiles) {
+    // rhs: b_i -= L_ik y_k
+    const int i = k + 1 + (task - ntiles);
+    const int64_t bi = (int64_t)i * kTile;
+    __shared__ R part[4][kTile];
+    const int r = t & 63, cg = t >> 6;
+    R acc = 0;
+    for (int c = cg; c < kTile; c += 4) acc += A[(bi + r) + (bk + c) * lda] * b[bk + c];
+    part[cg][r] = acc;
+    __syncthreads();
+    if (t < kTile) b[bi + t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    return;
+  }
+  // lower-triangular task index -> (ii, jj), jj <= ii
+  int ii = static_cast<int>((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
+  while ((ii + 1) * (ii + 2) / 2 <= task) ++ii;
+  while (ii * (ii + 1) / 2 > task) --ii;
+  const int jj = task - ii * (ii + 1) / 2;
+  const int64_t bi = (int64_t)(k + 1 + ii) * kTile, bj = (int64_t)(k + 1 + jj) * kTile;
+  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
+    const int c = idx / kTile, r = idx % kTile;
+    Li[r][c] = A[(bi + r) + (bk + c) * lda];
+    Lj[r][c] = A[(bj + r) + (bk + c) * lda];
+  }
+  __syncthreads();
+  // each thread: 4 consecutive rows x 4 columns of the tile
+  const int tr = t & 15, tc = t >> 4;
+  R acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = 0;
+  for (int kk = 0; kk < kTile; ++kk) {
+    R a[4], bb[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      a[p] = Li[4 * tr + p][kk];
+      bb[p] = Lj[4 * tc + p][kk];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * bb[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) A[(bi + 4 * tr + p) + (bj + 4 * tc + q) * lda] -= acc[p][q];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __restrict__ y,
+          R* __restrict__ x, int k) {
+  __shared__ R L[kTile][kLd];
+  __shared__ R xs[kTile];
+  __shared__ R part[4][kTile];
+  const int t = threadIdx.x;
+  const int64_t bk = (int64_t)k * kTile;
+  const R* Lk = Ld + (int64_t)k * kTile * kTile;
+  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
+    const int c = idx / kTile, r = idx % kTile;
+    L[r][c] = Lk[idx];
+  }
+  if (t < kTile) xs[t] = y[bk + t];
+  __syncthreads();
+  // x_k = L_kk^-T y_k (backward substitution)
+  for (int c = kTile - 1; c >= 0; --c) {
+    if (t == 0) xs[c] /= L[c][c];
+    __syncthreads();
+    if (t < c) xs[t] -= L[c][t] * xs[c];
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    if (t < kTile) x[bk + t] = xs[t];
+    return;
+  }
+  // y_j -= L_kj^T x_k for the block's tile column j < k
+  const int j = blockIdx.x - 1;
+  const int64_t bj = (int64_t)j * kTile;
+  const int c = t & 63, rg = t >> 6;
+  R acc = 0;
+  for (int r = rg; r < kTile; r += 4) acc += A[(bk + r) + (bj + c) * lda] * xs[r];
+  part[rg][c] = acc;
+  __syncthreads();
+  if (t < kTile) y[bj + t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
+}
+
+}  // namespace
+
+// Factor the padded n_pad x n_pad matrix A (column-major, lda = n_pad, n_pad % 64 == 0) and
+// solve A x = b.  A's strictly-lower tiles receive L, Ld (n_pad * 64 elements) the diagonal
+// factors; b is overwritten by the forward-substitution result; x receives the solution;
+// info[0] is set to 1 on a non-positive pivot (it must be cleared by the caller).
+template <typename R>
+int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_t s) {
+  HSL2_REQUIRE(n_pad % kTile == 0, HSL2_ERR_SHAPE, "cholesky: size must be a multiple of 64");
+  const int T = static_cast<int>(n_pad / kTile);
+  for (int k = 0; k < T; ++k) {
+    hipLaunchKernelGGL(chol_panel<R>, dim3(T - k), dim3(256), 0, s, A, n_pad, Ld, b, k, info);
+    HSL2_LAUNCH_CHECK();
+    const int m = T - k - 1;
+    if (m > 0) {
+      hipLaunchKernelGGL(chol_update<R>, dim3(m * (m + 1) / 2 + m), dim3(256), 0, s, A, n_pad, b, k, m);
+      HSL2_LAUNCH_CHECK();
+    }
+  }
+  for (int k = T - 1; k >= 0; --k) {
+    hipLaunchKernelGGL(chol_back<R>, dim3(k + 1), dim3(256), 0, s, A, n_pad, Ld, b, x, k);
+    HSL2_LAUNCH_CHECK();
+  }
+  return HSL2_OK;
+}
+
+template int cholesky_solve<double>(double*, int64_t, double*, double*, double*, int*, hipStream_t);
+template int cholesky_solve<float>(float*, int64_t, float*, float*, float*, int*, hipStream_t);
+
+namespace {
+
+template <typename R>
+__global__ void spd_pad(const R* __restrict__ A, const R* __restrict__ b, int64_t n, int64_t n_pad,
+                        R* __restrict__ Ap, R* __restrict__ bp, int* __restrict__ info) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx == 0) info[0] = 0;
+  if (idx < n_pad) bp[idx] = idx < n ? b[idx] : R(0);
+  if (idx >= n_pad * n_pad) return;
+  const int64_t c = idx / n_pad, r = idx - c * n_pad;
+  Ap[idx] = (r < n && c < n) ? A[r + c * n] : (r == c ? R(1) : R(0));
+}
+
+template <typename R>
+__global__ void spd_unpad(const R* __restrict__ xp, const int* __restrict__ info, int64_t n,
+                          R* __restrict__ x) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) x[idx] = info[0] == 0 ? xp[idx] : R(0);
+}
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
+}  // namespace hsl2
+
+using namespace hsl2;
+
+extern "C" size_t hsl2_spd_solve_workspace_bytes(int dtype, int64_t n) {
+  const size_t es = dtype == HSL2_F64 ? 8 : 4;
+  const size_t np = static_cast<size_t>((n + 63) / 64) * 64;
+  return align256(np * np * es) + 3 * align256(np * es) + align256(np * 64 * es) + 256;
+}
+
+extern "C" int hsl2_spd_solve(int dtype, const void* A, const void* b, void* x, int64_t n,
+                              void* workspace, size_t workspace_bytes, int* ok, void* stream) {
+  HSL2_REQUIRE(A && b && x && workspace, HSL2_ERR_ARGUMENT, "spd_solve: null pointer");
+  HSL2_REQUIRE(n > 0 && n < (1 << 16), HSL2_ERR_SHAPE, "spd_solve: n must be in [1, 65535]");
+  HSL2_REQUIRE(dtype == HSL2_F64 || dtype == HSL2_F32, HSL2_ERR_ARGUMENT, "spd_solve: dtype must be F32 or F64");
+  HSL2_REQUIRE(workspace_bytes >= hsl2_spd_solve_workspace_bytes(dtype, n), HSL2_ERR_WORKSPACE,
+               "spd_solve: workspace too small");
+  HSL2_REQUIRE(reinterpret_cast<uintptr_t>(workspace) % 256 == 0, HSL2_ERR_ARGUMENT,
+               "spd_solve: workspace must be 256-byte aligned");
+  hipStream_t s = as_stream(stream);
+  const int64_t np = ((n + 63) / 64) * 64;
+  const size_t es = dtype == HSL2_F64 ? 8 : 4;
+  char* w = static_cast<char*>(workspace);
+  char* Ap = w;
+  char* bp = Ap + align256(np * np * es);
+  char* xp = bp + align256(np * es);
+  char* Ld = xp + align256(np * es);
+  int* info = reinterpret_cast<int*>(Ld + align256(np * 64 * es));
+  const unsigned g1 = static_cast<unsigned>((np * np + 255) / 256), g2 = static_cast<unsigned>((n + 255) / 256);
+  int st;
+  if (dtype == HSL2_F64) {
+    hipLaunchKernelGGL(spd_pad<double>, dim3(g1), dim3(256), 0, s, static_cast<const double*>(A),
+                       static_cast<const double*>(b), n, np, reinterpret_cast<double*>(Ap),
+                       reinterpret_cast<double*>(bp), info);
+    HSL2_LAUNCH_CHECK();
+    st = cholesky_solve<double>(reinterpret_cast<double*>(Ap), np, reinterpret_cast<double*>(Ld),
+                                reinterpret_cast<double*>(bp), reinterpret_cast<double*>(xp), info, s);
+    if (st) return st;
+    hipLaunchKernelGGL(spd_unpad<double>, dim3(g2), dim3(256), 0, s, reinterpret_cast<double*>(xp), info, n,
+                       static_cast<double*>(x));
+  } else {
+    hipLaunchKernelGGL(spd_pad<float>, dim3(g1), dim3(256), 0, s, static_cast<const float*>(A),
+                       static_cast<const float*>(b), n, np, reinterpret_cast<float*>(Ap),
+                       reinterpret_cast<float*>(bp), info);
+    HSL2_LAUNCH_CHECK();
+    st = cholesky_solve<float>(reinterpret_cast<float*>(Ap), np, reinterpret_cast<float*>(Ld),
+                               reinterpret_cast<float*>(bp), reinterpret_cast<float*>(xp), info, s);
+    if (st) return st;
+    hipLaunchKernelGGL(spd_unpad<float>, dim3(g2), dim3(256), 0, s, reinterpret_cast<float*>(xp), info, n,
+                       static_cast<float*>(x));
+  }
+  HSL2_LAUNCH_CHECK();
+  if (ok) {
+    int h = 0;
+    HSL2_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HSL2_HIP(hipStreamSynchronize(s));
+    *ok = h == 0 ? 1 : 0;
+  }
+  return HSL2_OK;
+}

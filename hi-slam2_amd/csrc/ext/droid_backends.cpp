@@ -158,6 +158,29 @@ std::vector<torch::Tensor> ba_index(torch::Tensor ii, torch::Tensor jj, int t0, 
   return {kx, kk};
 }
 
+// Dense SPD solve with the library's tiled Cholesky (the reduced-system solver of ba).
+std::vector<torch::Tensor> spd_solve(torch::Tensor A, torch::Tensor b) {
+  require_device(A, "A");
+  require(A.dim() == 2 && A.size(0) == A.size(1) && b.numel() == A.size(0), "spd_solve: A [n, n], b [n]");
+  require(A.scalar_type() == b.scalar_type() &&
+              (A.scalar_type() == torch::kFloat64 || A.scalar_type() == torch::kFloat32),
+          "spd_solve: float64 or float32");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(A.device());
+  const int64_t n = A.size(0);
+  torch::Tensor Acm = A.t().contiguous();            // column-major
+  torch::Tensor bc = b.contiguous();
+  const int dt = dtype_code(A);
+  torch::Tensor ws = torch::empty({static_cast<int64_t>(hsl2_spd_solve_workspace_bytes(dt, n)) + 256},
+                                  A.options().dtype(torch::kUInt8));
+  void* wp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(ws.data_ptr()) + 255) & ~uintptr_t(255));
+  torch::Tensor x = torch::empty({n}, A.options());
+  int ok = 0;
+  raise_status(hsl2_spd_solve(dt, Acm.data_ptr(), bc.data_ptr(), x.data_ptr(), n, wp,
+                              static_cast<size_t>(ws.numel() - 256), &ok, current_stream()),
+               "spd_solve");
+  return {x, torch::tensor({ok}, torch::kInt32)};
+}
+
 // ---- corr_index (src/droid.cpp:254-275) ----------------------------------------------
 std::vector<torch::Tensor> corr_index_forward(torch::Tensor volume, torch::Tensor coords, int radius) {
   CHECK_CONTIGUOUS(volume);
@@ -353,6 +376,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   // additions of this build
   m.def("ba_ex", &ba_ex, "ba with an explicit precision mode (0 split, 1 all-FP32); returns [dx, dz, None, ok]");
   m.def("ba_index", &ba_index, "kx and kk_exp of a ba call (host-side planner)");
+  m.def("spd_solve", &spd_solve, "dense SPD solve with the tiled GPU Cholesky; returns [x, ok]");
   m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch",
         py::arg("levels"), py::arg("coords"), py::arg("radius"), py::arg("tiled") = false);
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),

@@ -6,8 +6,8 @@
  * the last failure on the calling thread is available from hsl2_last_error().
  * Device pointers must be allocated on the device that is current when the
  * call is made; `stream` is a hipStream_t passed as void* (NULL = the null
- * stream).  No entry point allocates device memory except through rocSOLVER's
- * own handle (see hsl2_ba_run); scratch is always supplied by the caller.
+ * stream).  No entry point allocates device memory: scratch is always supplied by the
+ * caller.
  *
  * Each function names the reference interface it replaces (paths relative to
  * the HI-SLAM2 source tree).
@@ -156,6 +156,15 @@ int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps, const floa
                 const float* targets, const float* weights, const float* eta, int64_t eta_rows,
                 int iterations, float lm, float ep, float* dx, float* dz,
                 void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
+
+/* Dense symmetric positive-definite solve A x = b (column-major n x n, F64 or F32) with the
+ * tiled Cholesky that ba uses for its reduced camera system (replaces the host Eigen
+ * SimplicialLLT of SparseBlock::solve, src/droid_kernels.cu:1659-1680).  A is not modified;
+ * x = 0 and *ok = 0 (host, may be NULL; reading it synchronizes) when a pivot is not
+ * positive.  The workspace (256-byte aligned) holds the padded copy and the factors. */
+size_t hsl2_spd_solve_workspace_bytes(int dtype, int64_t n);
+int hsl2_spd_solve(int dtype, const void* A, const void* b, void* x, int64_t n, void* workspace,
+                   size_t workspace_bytes, int* ok, void* stream);
 
 /* Debug/test view of the last iteration's intermediates inside a workspace after
  * hsl2_ba_run: per-edge Hessian blocks Hs [4, E, 6, 6] and gradients vs [2, E, 6], the
