@@ -7,14 +7,15 @@
 //
 //   for each tile column k:
 //     chol_panel   one block per tile row i >= k: every block factors the diagonal tile in
-//                  LDS (redundantly, so no inter-block wait is needed); block 0 stores L_kk
-//                  in a separate diagonal-factor buffer (A_kk is still being read by the
-//                  other blocks) and y_k = L_kk^-1 b_k, the others store L_ik = A_ik L_kk^-T
+//                  registers and inverts the factor (redundantly, so no inter-block wait is
+//                  needed); block 0 stores L_kk^-1 in a separate diagonal buffer (A_kk is
+//                  still being read by the other blocks) and y_k = L_kk^-1 b_k, the others
+//                  store L_ik = A_ik (L_kk^-1)^T (a 64x64x64 product, no substitution)
 //     chol_update  one block per lower trailing tile (i, j), k < j <= i: A_ij -= L_ik L_jk^T,
 //                  plus one block per tile row i > k: b_i -= L_ik y_k
 //   for each tile row k from the bottom:
-//     chol_back    block 0 writes x_k = L_kk^-T y_k; blocks 1..k recompute x_k and apply
-//                  y_j -= L_kj^T x_k for j < k
+//     chol_back    block 0 writes x_k = (L_kk^-1)^T y_k; blocks 1..k recompute x_k and
+//                  apply y_j -= L_kj^T x_k for j < k
 //
 // A pivot that is not positive (or NaN) sets info = 1, as Eigen's LLT reports a
 // NumericalIssue; the caller then treats the solution as zero.  Templated on the element
@@ -28,87 +29,182 @@ constexpr int kLd = kTile + 1;      // padded LDS row
 
 namespace {
 
-// Load the lower triangle of the diagonal tile (k, k) and factor it in LDS (L overwrites A).
+// Thread t of a 256-thread block owns rows 4*(t>>4) .. +3 and columns 4*(t&15) .. +3 of a
+// 64x64 tile, in registers.  Register arrays are only indexed with compile-time indices;
+// a runtime column or row is selected with unrolled predicated loops.
+
+// Factor the diagonal tile (k, k) in registers, then invert the factor.  On return
+// Lw[r][c] (LDS) holds L_kk^-1 (lower triangular, zeros above).  A non-positive pivot sets
+// info[0] = 1 when `report` and is replaced by 1 so that the arithmetic stays finite.
 template <typename R>
-__device__ void factor_diag(const R* __restrict__ A, int64_t lda, int k, R (*L)[kLd], bool report,
-                            int* __restrict__ info) {
+__device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R (*Lw)[kLd],
+                                  bool report, int* __restrict__ info) {
+  __shared__ R colc[2][kTile];
+  __shared__ R rowx[2][kTile];
+  __shared__ R diag[kTile];
   const int t = threadIdx.x;
+  const int tr = t >> 4, ts = t & 15;
   const int64_t base = (int64_t)k * kTile;
-  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
-    const int c = idx / kTile, r = idx % kTile;       // column-major walk: coalesced
-    L[r][c] = r >= c ? A[(base + r) + (base + c) * lda] : R(0);
+  R a[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int r = 4 * tr + p, c = 4 * ts + q;
+      a[p][q] = r >= c ? A[(base + r) + (base + c) * lda] : R(0);
+    }
+  if (tr == ts) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) diag[4 * tr + p] = a[p][p];
   }
   __syncthreads();
-  const int s = t & 63, rg = t >> 6;
+  // right-looking factorisation, two barriers per column
   for (int c = 0; c < kTile; ++c) {
-    if (t == 0) {
-      R d = L[c][c];
-      if (!(d > R(0))) {          // not positive definite (or NaN): report and keep going safely
-        if (report) info[0] = 1;
+    const int buf = c & 1;
+    if (ts == (c >> 2)) {
+      R d = diag[c];
+      if (!(d > R(0))) {
+        if (report && t == 0) info[0] = 1;
         d = R(1);
       }
-      L[c][c] = sqrt(d);
+      d = sqrt(d);
+      const R inv = R(1) / d;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q != (c & 3)) continue;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int r = 4 * tr + p;
+          if (r > c) {
+            a[p][q] *= inv;
+            colc[buf][r] = a[p][q];
+            Lw[r][c] = a[p][q];
+          } else if (r == c) {
+            a[p][q] = d;
+            Lw[c][c] = d;
+          }
+        }
+      }
     }
     __syncthreads();
-    if (t > c && t < kTile) L[t][c] /= L[c][c];
-    __syncthreads();
-    // trailing update of the lower triangle: rows r > c, columns c < s <= r
-    if (s > c) {
-      const R lsc = L[s][c];
-      for (int r = c + 1 + rg; r < kTile; r += 4)
-        if (s <= r) L[r][s] -= L[r][c] * lsc;
+    R cr[4], cs[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      cr[p] = colc[buf][4 * tr + p];
+      cs[p] = colc[buf][4 * ts + p];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = 4 * tr + p, sc = 4 * ts + q;
+        if (sc > c && r >= sc) a[p][q] -= cr[p] * cs[q];
+      }
+    if (tr == ts) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        if (4 * tr + p == c + 1) diag[c + 1] = a[p][p];
     }
     __syncthreads();
   }
+  // Lw now holds L (lower, diagonal included); its upper triangle is never read.
+  // Invert: X = L^-1 by right-looking elimination on the rows of X, one barrier per row.
+  R x[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[p][q] = (4 * tr + p == 4 * ts + q) ? R(1) : R(0);
+  for (int c = 0; c < kTile; ++c) {
+    const int buf = c & 1;
+    if (tr == (c >> 2)) {
+      const R inv = R(1) / Lw[c][c];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        if (p != (c & 3)) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          x[p][q] *= inv;
+          rowx[buf][4 * ts + q] = x[p][q];
+        }
+      }
+    }
+    __syncthreads();
+    R rx[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rx[q] = rowx[buf][4 * ts + q];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int r = 4 * tr + p;
+      if (r > c) {
+        const R lrc = Lw[r][c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[p][q] -= lrc * rx[q];
+      }
+    }
+  }
+  __syncthreads();                 // every read of L in Lw is done
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Lw[4 * tr + p][4 * ts + q] = x[p][q];
+  __syncthreads();
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256)
 chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
            int* __restrict__ info) {
-  __shared__ R L[kTile][kLd];
+  __shared__ R Li[kTile][kLd];     // L_kk^-1
   __shared__ R X[kTile][kLd];
+  __shared__ R part[4][kTile];
   const int t = threadIdx.x;
   const int i = k + blockIdx.x;
-  factor_diag<R>(A, lda, k, L, blockIdx.x == 0, info);
+  factor_and_invert<R>(A, lda, k, Li, blockIdx.x == 0, info);
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
   if (blockIdx.x == 0) {
-    R* Lk = Ld + (int64_t)k * kTile * kTile;          // column-major 64x64 factor of tile k
+    // the diagonal factor is kept as its inverse (column-major) for the back-substitution
+    R* Lk = Ld + (int64_t)k * kTile * kTile;
     for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
       const int c = idx / kTile, r = idx % kTile;
-      Lk[idx] = L[r][c];
+      Lk[idx] = Li[r][c];
     }
-    // y_k = L_kk^-1 b_k by forward substitution
-    __shared__ R y[kTile];
-    if (t < kTile) y[t] = b[bk + t];
+    // y_k = L_kk^-1 b_k
+    const int r = t & 63, cg = t >> 6;
+    R acc = 0;
+    for (int c = cg; c < kTile; c += 4) acc += Li[r][c] * b[bk + c];
+    part[cg][r] = acc;
     __syncthreads();
-    for (int c = 0; c < kTile; ++c) {
-      if (t == 0) y[c] /= L[c][c];
-      __syncthreads();
-      if (t > c && t < kTile) y[t] -= L[t][c] * y[c];
-      __syncthreads();
-    }
-    if (t < kTile) b[bk + t] = y[t];
+    if (t < kTile) b[bk + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
     return;
   }
-  // L_ik = A_ik L_kk^-T: solve X L_kk^T = A_ik column by column (right-looking)
+  // L_ik = A_ik L_kk^-T
   for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
     const int c = idx / kTile, r = idx % kTile;
     X[r][c] = A[(bi + r) + (bk + c) * lda];
   }
   __syncthreads();
-  const int r = t & 63, mg = t >> 6;
-  for (int c = 0; c < kTile; ++c) {
-    if (t < kTile) X[t][c] /= L[c][c];
-    __syncthreads();
-    const R xrc = X[r][c];
-    for (int m = c + 1 + mg; m < kTile; m += 4) X[r][m] -= xrc * L[m][c];
-    __syncthreads();
+  const int tr = t >> 4, ts = t & 15;
+  R acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = 0;
+  for (int m = 0; m < kTile; ++m) {
+    R av[4], lv[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      av[p] = X[4 * tr + p][m];
+      lv[p] = Li[4 * ts + p][m];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += av[p] * lv[q];
   }
-  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
-    const int c = idx / kTile, rr = idx % kTile;
-    A[(bi + rr) + (bk + c) * lda] = X[rr][c];
-  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) A[(bi + 4 * tr + p) + (bk + 4 * ts + q) * lda] = acc[p][q];
 }
 
 template <typename R>
@@ -145,8 +241,7 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
     Lj[r][c] = A[(bj + r) + (bk + c) * lda];
   }
   __syncthreads();
-  // each thread: 4 consecutive rows x 4 columns of the tile
-  const int tr = t & 15, tc = t >> 4;
+  const int tr = t >> 4, tc = t & 15;
   R acc[4][4];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
@@ -174,25 +269,29 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __restrict__ y,
           R* __restrict__ x, int k) {
-  __shared__ R L[kTile][kLd];
+  __shared__ R Li[kTile][kLd];
+  __shared__ R ys[kTile];
   __shared__ R xs[kTile];
   __shared__ R part[4][kTile];
   const int t = threadIdx.x;
   const int64_t bk = (int64_t)k * kTile;
-  const R* Lk = Ld + (int64_t)k * kTile * kTile;
+  const R* Lk = Ld + (int64_t)k * kTile * kTile;     // L_kk^-1, column-major
   for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
     const int c = idx / kTile, r = idx % kTile;
-    L[r][c] = Lk[idx];
+    Li[r][c] = Lk[idx];
   }
-  if (t < kTile) xs[t] = y[bk + t];
+  if (t < kTile) ys[t] = y[bk + t];
   __syncthreads();
-  // x_k = L_kk^-T y_k (backward substitution)
-  for (int c = kTile - 1; c >= 0; --c) {
-    if (t == 0) xs[c] /= L[c][c];
-    __syncthreads();
-    if (t < c) xs[t] -= L[c][t] * xs[c];
-    __syncthreads();
+  // x_k = L_kk^-T y_k = (L_kk^-1)^T y_k
+  {
+    const int c = t & 63, rg = t >> 6;
+    R acc = 0;
+    for (int r = rg; r < kTile; r += 4) acc += Li[r][c] * ys[r];
+    part[rg][c] = acc;
   }
+  __syncthreads();
+  if (t < kTile) xs[t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+  __syncthreads();
   if (blockIdx.x == 0) {
     if (t < kTile) x[bk + t] = xs[t];
     return;
@@ -203,6 +302,7 @@ chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __r
   const int c = t & 63, rg = t >> 6;
   R acc = 0;
   for (int r = rg; r < kTile; r += 4) acc += A[(bk + r) + (bj + c) * lda] * xs[r];
+  __syncthreads();
   part[rg][c] = acc;
   __syncthreads();
   if (t < kTile) y[bj + t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
@@ -211,8 +311,8 @@ chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __r
 }  // namespace
 
 // Factor the padded n_pad x n_pad matrix A (column-major, lda = n_pad, n_pad % 64 == 0) and
-// solve A x = b.  A's strictly-lower tiles receive L, Ld (n_pad * 64 elements) the diagonal
-// factors; b is overwritten by the forward-substitution result; x receives the solution;
+// solve A x = b.  A's strictly-lower tiles receive L, Ld (n_pad * 64 elements) the inverses
+// of the diagonal factors; b is overwritten by the forward-substitution result; x receives the solution;
 // info[0] is set to 1 on a non-positive pivot (it must be cleared by the caller).
 template <typename R>
 int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_t s) {

@@ -132,70 +132,96 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   }
 }
 
-// One thread per (pixel, level) with the tap window in registers: the radius is a
-// compile-time constant, so there is no integer division and no LDS round trip.  Per row
-// and per column of the window the plane offset and the in-bounds predicate are formed
-// once; a tap costs one add, one predicate and one load.  Output rows [o][pixel] are
-// written coalesced because consecutive lanes own consecutive pixels.
+// Radius known at compile time: 64 pixels of one level per 256-thread block.
+// Phase 1: each wave gathers whole tap windows, one window per load instruction (lanes run
+// along the window rows, so an instruction touches at most four 8x8 tiles of a tiled
+// plane); taps land in LDS with a padded per-pixel stride.  Phase 2: four threads per
+// pixel (pixels on the fast lane axis) read the tap columns they need and write the
+// [a][b][pixel] output rows coalesced.  All index arithmetic is shifts or constant
+// divisions.
 template <typename T, int RAD>
 __global__ void __launch_bounds__(256)
-corr_lookup_px_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
-                      int64_t cy_off, T* __restrict__ out, int hw1, int64_t out_batch_stride,
-                      int total) {
+corr_lookup_win_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
+                       int64_t cy_off, T* __restrict__ out, int hw1, int64_t out_batch_stride,
+                       int tiles_per_item) {
   using A = typename Acc<T>::type;
-  constexpr int D = 2 * RAD + 2;
-  constexpr int R = 2 * RAD + 1;
+  constexpr int D = 2 * RAD + 2, D2 = D * D, R = 2 * RAD + 1;
+  constexpr int TS = D2 + 1;                    // padded per-pixel tap stride
+  __shared__ A taps[kPix * TS];
+  __shared__ int s_x0[kPix], s_y0[kPix];
+  __shared__ float s_dx[kPix], s_dy[kPix];
+
   const int level = blockIdx.y;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= total) return;
-  const int n = gid / hw1;
-  const int pix = gid - n * hw1;
   const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
   const int h2 = lv.h2[level], w2 = lv.w2[level];
   const bool tiled = lv.tiled[level] != 0;
   const float scale = lv.scale[level];
+  const int n = blockIdx.x / tiles_per_item;
+  const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
+  const int npix = min(kPix, hw1 - pix0);
+  const int t = threadIdx.x;
 
-  const float* cp_ = coords + (int64_t)n * cb + (int64_t)pix * cp;
-  const float cx = cp_[0] * scale;
-  const float cy = cp_[cy_off] * scale;
-  const float flx = floorf(cx), fly = floorf(cy);
-  const int x0 = static_cast<int>(flx) - RAD;
-  const int y0 = static_cast<int>(fly) - RAD;
-  const float fx = cx - flx, fy = cy - fly;
-
-  int rowoff[D], coloff[D];
-  bool rowok[D], colok[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    const int y = y0 + k, x = x0 + k;
-    rowok[k] = y >= 0 && y < h2;
-    colok[k] = x >= 0 && x < w2;
-    rowoff[k] = tiled ? ((((y >> 3) * (w2 >> 3)) << 6) + ((y & 7) << 3)) : y * w2;
-    coloff[k] = tiled ? (((x >> 3) << 6) + (x & 7)) : x;
+  if (t < kPix) {
+    int x0 = 0, y0 = 0;
+    float fx = 0.f, fy = 0.f;
+    if (t < npix) {
+      const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
+      const float cx = cp_[0] * scale;
+      const float cy = cp_[cy_off] * scale;
+      const float flx = floorf(cx), fly = floorf(cy);
+      x0 = static_cast<int>(flx) - RAD;
+      y0 = static_cast<int>(fly) - RAD;
+      fx = cx - flx;
+      fy = cy - fly;
+    }
+    s_x0[t] = x0;
+    s_y0[t] = y0;
+    s_dx[t] = fx;
+    s_dy[t] = fy;
   }
-  const T* plane = volume + ((int64_t)n * hw1 + pix) * ((int64_t)h2 * w2);
-  A tap[D][D];
-#pragma unroll
-  for (int ty = 0; ty < D; ++ty)
-#pragma unroll
-    for (int tx = 0; tx < D; ++tx) {
-      // out-of-plane taps read the plane's first element (always in bounds) and are zeroed
-      const bool ok = rowok[ty] && colok[tx];
-      const A v = load_acc<T>(plane + (ok ? rowoff[ty] + coloff[tx] : 0));
-      tap[ty][tx] = ok ? v : A(0);
-    }
+  __syncthreads();
 
-  // bilinear weights in float, as the reference forms them (src/correlation_kernels.cu:41-63)
-  const A w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy);
-  const A w01 = (1.f - fx) * fy, w11 = fx * fy;
-  T* o = out + (int64_t)n * out_batch_stride + lv.out_offset[level] + pix;
+  const int wave = t >> 6, lane = t & 63;
+  const int64_t plane = (int64_t)h2 * w2;
+  const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
+  for (int p = wave; p < npix; p += 4) {
+    const int xs = s_x0[p], ys = s_y0[p];
+    const T* pl = vbase + p * plane;
 #pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-      const A v = w00 * tap[b][a] + w10 * tap[b][a + 1] + w01 * tap[b + 1][a] + w11 * tap[b + 1][a + 1];
-      o[(int64_t)(a * R + b) * hw1] = store_acc<T>(v);
+    for (int k = lane; k < D2; k += 64) {
+      const int ty = k / D, tx = k - ty * D;    // constant divisor
+      const int y = ys + ty, x = xs + tx;
+      const bool ok = (unsigned)y < (unsigned)h2 && (unsigned)x < (unsigned)w2;
+      const int off = tiled ? (((((y >> 3) * (w2 >> 3)) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7))
+                            : y * w2 + x;
+      const A v = load_acc<T>(pl + (ok ? off : 0));   // out-of-plane taps read element 0, then zero
+      taps[p * TS + k] = ok ? v : A(0);
     }
+  }
+  __syncthreads();
+
+  const int p = lane, g = wave;
+  if (p < npix) {
+    const float fx = s_dx[p], fy = s_dy[p];
+    const A w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy);
+    const A w01 = (1.f - fx) * fy, w11 = fx * fy;
+    const A* tp = taps + p * TS;
+    T* o = out + (int64_t)n * out_batch_stride + lv.out_offset[level] + pix0 + p;
+#pragma unroll
+    for (int a = g; a < R; a += 4) {
+      A c0[D], c1[D];
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        c0[b] = tp[b * D + a];
+        c1[b] = tp[b * D + a + 1];
+      }
+#pragma unroll
+      for (int b = 0; b < R; ++b) {
+        const A v = w00 * c0[b] + w10 * c1[b] + w01 * c0[b + 1] + w11 * c1[b + 1];
+        o[(int64_t)(a * R + b) * hw1] = store_acc<T>(v);
+      }
+    }
+  }
 }
 
 // Adjoint: writes the complete volume_grad planes of the block's 64 pixels.
@@ -278,20 +304,19 @@ int launch_fwd(const LevelTable& lv, int levels, const float* coords, int64_t cb
                int64_t cy_off, void* corr, int64_t batch, int64_t hw1, int r,
                int64_t out_batch_stride, hipStream_t s) {
   if (batch * hw1 == 0) return HSL2_OK;
-  if ((r == 3 || r == 4) && batch * hw1 < (1ll << 31)) {
-    const int total = static_cast<int>(batch * hw1);
-    dim3 grid(static_cast<unsigned>((total + 255) / 256), levels);
+  const int tiles = static_cast<int>((hw1 + kPix - 1) / kPix);
+  if ((r == 3 || r == 4) && batch * tiles < (1ll << 31)) {
+    dim3 grid(static_cast<unsigned>(batch * tiles), levels);
     if (r == 3)
-      hipLaunchKernelGGL((corr_lookup_px_kernel<T, 3>), grid, dim3(256), 0, s, lv, coords, cb, cp,
-                         cy_off, static_cast<T*>(corr), static_cast<int>(hw1), out_batch_stride, total);
+      hipLaunchKernelGGL((corr_lookup_win_kernel<T, 3>), grid, dim3(256), 0, s, lv, coords, cb, cp,
+                         cy_off, static_cast<T*>(corr), static_cast<int>(hw1), out_batch_stride, tiles);
     else
-      hipLaunchKernelGGL((corr_lookup_px_kernel<T, 4>), grid, dim3(256), 0, s, lv, coords, cb, cp,
-                         cy_off, static_cast<T*>(corr), static_cast<int>(hw1), out_batch_stride, total);
+      hipLaunchKernelGGL((corr_lookup_win_kernel<T, 4>), grid, dim3(256), 0, s, lv, coords, cb, cp,
+                         cy_off, static_cast<T*>(corr), static_cast<int>(hw1), out_batch_stride, tiles);
     HSL2_LAUNCH_CHECK();
     return HSL2_OK;
   }
-  // other radii: 64 pixels per block, windows staged through LDS
-  const int tiles = static_cast<int>((hw1 + kPix - 1) / kPix);
+  // other radii: the same scheme with the radius as a runtime value
   const int64_t blocks = batch * tiles;
   HSL2_REQUIRE(blocks < (1ll << 31), HSL2_ERR_SHAPE, "corr lookup: grid too large");
   const int D = 2 * r + 2;
