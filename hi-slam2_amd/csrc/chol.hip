@@ -33,15 +33,22 @@ namespace {
 // 64x64 tile, in registers.  Register arrays are only indexed with compile-time indices;
 // a runtime column or row is selected with unrolled predicated loops.
 
-// Factor the diagonal tile (k, k) in registers, then invert the factor.  On return
-// Lw[r][c] (LDS) holds L_kk^-1 (lower triangular, zeros above).  A non-positive pivot sets
-// info[0] = 1 when `report` and is replaced by 1 so that the arithmetic stays finite.
+// Factor the diagonal tile (k, k) in registers, then invert the factor.  On return Xi
+// (LDS) holds L_kk^-1 (lower triangular, zeros above) and Lw holds L_kk (lower part).
+// A non-positive pivot sets info[0] = 1 when `report` and is replaced by 1 so that the
+// arithmetic stays finite.
+//
+// Factorisation: right-looking, ONE barrier per column.  The owners of column c publish
+// it unscaled; every thread then takes the pivot from the published column and applies
+// a[r][s] -= a[r][c] a[s][c] / pivot, while the owners scale their column by 1/sqrt(pivot).
+// Inverse: 4x4 diagonal blocks inverted in registers, then recursive doubling
+// inv([[L11, 0], [L21, L22]]) = [[X11, 0], [-X22 L21 X11, X22]] over block sizes 4..32,
+// i.e. two small products and two barriers per level instead of 64 dependent steps.
 template <typename R>
 __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R (*Lw)[kLd],
-                                  bool report, int* __restrict__ info) {
+                                  R (*Xi)[kLd], bool report, int* __restrict__ info) {
   __shared__ R colc[2][kTile];
-  __shared__ R rowx[2][kTile];
-  __shared__ R diag[kTile];
+  __shared__ R tmp[32 * 32];
   const int t = threadIdx.x;
   const int tr = t >> 4, ts = t & 15;
   const int64_t base = (int64_t)k * kTile;
@@ -52,23 +59,29 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
     for (int p = 0; p < 4; ++p) {
       const int r = 4 * tr + p, c = 4 * ts + q;
       a[p][q] = r >= c ? A[(base + r) + (base + c) * lda] : R(0);
+      Xi[r][c] = R(0);
     }
-  if (tr == ts) {
-#pragma unroll
-    for (int p = 0; p < 4; ++p) diag[4 * tr + p] = a[p][p];
-  }
-  __syncthreads();
-  // right-looking factorisation, two barriers per column
   for (int c = 0; c < kTile; ++c) {
     const int buf = c & 1;
-    if (ts == (c >> 2)) {
-      R d = diag[c];
-      if (!(d > R(0))) {
-        if (report && t == 0) info[0] = 1;
-        d = R(1);
+    const bool owner = ts == (c >> 2);
+    if (owner) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q != (c & 3)) continue;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          if (4 * tr + p >= c) colc[buf][4 * tr + p] = a[p][q];
       }
-      d = sqrt(d);
-      const R inv = R(1) / d;
+    }
+    __syncthreads();
+    R piv = colc[buf][c];
+    if (!(piv > R(0))) {
+      if (report && t == 0) info[0] = 1;
+      piv = R(1);
+    }
+    if (owner) {
+      const R d = sqrt(piv);
+      const R invd = R(1) / d;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (q != (c & 3)) continue;
@@ -76,8 +89,7 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
         for (int p = 0; p < 4; ++p) {
           const int r = 4 * tr + p;
           if (r > c) {
-            a[p][q] *= inv;
-            colc[buf][r] = a[p][q];
+            a[p][q] *= invd;
             Lw[r][c] = a[p][q];
           } else if (r == c) {
             a[p][q] = d;
@@ -86,11 +98,11 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
         }
       }
     }
-    __syncthreads();
+    const R ip = R(1) / piv;
     R cr[4], cs[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-      cr[p] = colc[buf][4 * tr + p];
+      cr[p] = colc[buf][4 * tr + p] * ip;
       cs[p] = colc[buf][4 * ts + p];
     }
 #pragma unroll
@@ -100,87 +112,85 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
         const int r = 4 * tr + p, sc = 4 * ts + q;
         if (sc > c && r >= sc) a[p][q] -= cr[p] * cs[q];
       }
-    if (tr == ts) {
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-        if (4 * tr + p == c + 1) diag[c + 1] = a[p][p];
-    }
-    __syncthreads();
   }
-  // Lw now holds L (lower, diagonal included); its upper triangle is never read.
-  // Invert: X = L^-1 by right-looking elimination on the rows of X, one barrier per row.
-  R x[4][4];
+  __syncthreads();            // Lw complete
+  // 4x4 diagonal blocks of the inverse, in registers
+  if (tr == ts) {
+    R l[4][4], x[4][4];
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < 4; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x[p][q] = (4 * tr + p == 4 * ts + q) ? R(1) : R(0);
-  for (int c = 0; c < kTile; ++c) {
-    const int buf = c & 1;
-    if (tr == (c >> 2)) {
-      const R inv = R(1) / Lw[c][c];
+      for (int q = 0; q < 4; ++q) l[p][q] = q <= p ? Lw[4 * tr + p][4 * tr + q] : R(0);
 #pragma unroll
-      for (int p = 0; p < 4; ++p) {
-        if (p != (c & 3)) continue;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          x[p][q] *= inv;
-          rowx[buf][4 * ts + q] = x[p][q];
-        }
+      for (int i = 0; i < 4; ++i) {
+        R v = i == j ? R(1) : R(0);
+#pragma unroll
+        for (int m = 0; m < i; ++m) v -= l[i][m] * x[m][j];
+        x[i][j] = i >= j ? v / l[i][i] : R(0);
       }
-    }
-    __syncthreads();
-    R rx[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) rx[q] = rowx[buf][4 * ts + q];
+    for (int p = 0; p < 4; ++p)
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int r = 4 * tr + p;
-      if (r > c) {
-        const R lrc = Lw[r][c];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) x[p][q] -= lrc * rx[q];
-      }
-    }
+      for (int q = 0; q <= p; ++q) Xi[4 * tr + p][4 * tr + q] = x[p][q];
   }
-  __syncthreads();                 // every read of L in Lw is done
-#pragma unroll
-  for (int p = 0; p < 4; ++p)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) Lw[4 * tr + p][4 * ts + q] = x[p][q];
   __syncthreads();
+  // recursive doubling: blocks of size 2m from blocks of size m
+  for (int lg = 2; lg <= 5; ++lg) {
+    const int m = 1 << lg, mm = m * m;
+    const int total = 32 * m;               // (64 / 2m) blocks of m x m
+    for (int e = t; e < total; e += 256) {
+      const int b = e >> (2 * lg), i = (e >> lg) & (m - 1), j = e & (m - 1);
+      const int o = 2 * m * b;
+      R v = 0;                              // (L21 X11)[i][j], X11 lower: kk >= j
+      for (int kk = j; kk < m; ++kk) v += Lw[o + m + i][o + kk] * Xi[o + kk][o + j];
+      tmp[e] = v;
+    }
+    __syncthreads();
+    for (int e = t; e < total; e += 256) {
+      const int b = e >> (2 * lg), i = (e >> lg) & (m - 1), j = e & (m - 1);
+      const int o = 2 * m * b;
+      R v = 0;                              // -(X22 T)[i][j], X22 lower: kk <= i
+      for (int kk = 0; kk <= i; ++kk) v += Xi[o + m + i][o + m + kk] * tmp[(b << (2 * lg)) + (kk << lg) + j];
+      Xi[o + m + i][o + j] = -v;
+    }
+    __syncthreads();
+    (void)mm;
+  }
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256)
 chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
            int* __restrict__ info) {
-  __shared__ R Li[kTile][kLd];     // L_kk^-1
-  __shared__ R X[kTile][kLd];
+  __shared__ R Lw[kTile][kLd];     // L_kk, then A_ik
+  __shared__ R Xi[kTile][kLd];     // L_kk^-1
   __shared__ R part[4][kTile];
   const int t = threadIdx.x;
   const int i = k + blockIdx.x;
-  factor_and_invert<R>(A, lda, k, Li, blockIdx.x == 0, info);
+  factor_and_invert<R>(A, lda, k, Lw, Xi, blockIdx.x == 0, info);
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
   if (blockIdx.x == 0) {
     // the diagonal factor is kept as its inverse (column-major) for the back-substitution
     R* Lk = Ld + (int64_t)k * kTile * kTile;
     for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
       const int c = idx / kTile, r = idx % kTile;
-      Lk[idx] = Li[r][c];
+      Lk[idx] = Xi[r][c];
     }
     // y_k = L_kk^-1 b_k
     const int r = t & 63, cg = t >> 6;
     R acc = 0;
-    for (int c = cg; c < kTile; c += 4) acc += Li[r][c] * b[bk + c];
+    for (int c = cg; c < kTile; c += 4) acc += Xi[r][c] * b[bk + c];
     part[cg][r] = acc;
     __syncthreads();
     if (t < kTile) b[bk + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
     return;
   }
-  // L_ik = A_ik L_kk^-T
+  // L_ik = A_ik (L_kk^-1)^T, with A_ik staged in Lw (L_kk is no longer needed)
   for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
     const int c = idx / kTile, r = idx % kTile;
-    X[r][c] = A[(bi + r) + (bk + c) * lda];
+    Lw[r][c] = A[(bi + r) + (bk + c) * lda];
   }
   __syncthreads();
   const int tr = t >> 4, ts = t & 15;
@@ -193,8 +203,8 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
     R av[4], lv[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-      av[p] = X[4 * tr + p][m];
-      lv[p] = Li[4 * ts + p][m];
+      av[p] = Lw[4 * tr + p][m];
+      lv[p] = Xi[4 * ts + p][m];
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
