@@ -168,12 +168,23 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   const float* dsp = disps + (int64_t)ix * HW;
   const float* tg = targets + (int64_t)e * 2 * HW;
   const float* wt = weights + (int64_t)e * 2 * HW;
-  for (int k = threadIdx.x; k < HW; k += kLinThreads) {
+  // software pipelining: the next pixel's five inputs are loaded before this pixel's math
+  int k = threadIdx.x;
+  float in_d = 0.f, in_tu = 0.f, in_tv = 0.f, in_wu = 0.f, in_wv = 0.f;
+  if (k < HW) {
+    in_d = dsp[k]; in_tu = tg[k]; in_tv = tg[HW + k]; in_wu = wt[k]; in_wv = wt[HW + k];
+  }
+  for (; k < HW; k += kLinThreads) {
+    const float d = in_d, tu = in_tu, tv = in_tv, wu = in_wu, wv = in_wv;
+    const int kn = k + kLinThreads;
+    if (kn < HW) {
+      in_d = dsp[kn]; in_tu = tg[kn]; in_tv = tg[HW + kn]; in_wu = wt[kn]; in_wv = wt[HW + kn];
+    }
     const int row = k / wd;
     const int col = k - row * wd;
     RowJac ru, rv;
-    const bool ok = pixel_rows(g, static_cast<float>(col), static_cast<float>(row), dsp[k],
-                               tg[k], tg[HW + k], wt[k], wt[HW + k], ru, rv);
+    const bool ok = pixel_rows(g, static_cast<float>(col), static_cast<float>(row), d, tu, tv, wu,
+                               wv, ru, rv);
     if (kSchur) {
       float c = 0.f, b = 0.f, ei[6] = {0, 0, 0, 0, 0, 0}, ej[6] = {0, 0, 0, 0, 0, 0};
       if (ok) {
@@ -245,7 +256,22 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
   if (k >= HW) return;
   const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
   double c = 0.0, b = 0.0;
-  for (int q = e0; q < e1; ++q) {
+  int q = e0;
+  for (; q + 4 <= e1; q += 4) {                  // four edges' loads in flight together
+    float cv[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = frame_edges[q + u];
+      cv[u] = Cii[(int64_t)e * HW + k];
+      bv[u] = bz[(int64_t)e * HW + k];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c += cv[u];
+      b += bv[u];
+    }
+  }
+  for (; q < e1; ++q) {
     const int e = frame_edges[q];
     c += Cii[(int64_t)e * HW + k];
     b += bz[(int64_t)e * HW + k];
@@ -256,8 +282,22 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
   const int p = frame_pose[f];
   if (p >= 0) {
     double ei[6] = {0, 0, 0, 0, 0, 0};
-    for (int q = e0; q < e1; ++q) {
-      const int e = frame_edges[q];
+    int q2 = e0;
+    for (; q2 + 2 <= e1; q2 += 2) {
+      float v[2][6];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = frame_edges[q2 + u];
+#pragma unroll
+        for (int n = 0; n < 6; ++n) v[u][n] = Eii[((int64_t)e * 6 + n) * HW + k];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int n = 0; n < 6; ++n) ei[n] += v[u][n];
+    }
+    for (; q2 < e1; ++q2) {
+      const int e = frame_edges[q2];
 #pragma unroll
       for (int n = 0; n < 6; ++n) ei[n] += Eii[((int64_t)e * 6 + n) * HW + k];
     }

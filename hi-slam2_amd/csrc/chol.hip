@@ -29,6 +29,24 @@ constexpr int kLd = kTile + 1;      // padded LDS row
 
 namespace {
 
+// Copy a 64x64 column-major tile (column stride ld) into LDS T[r][c]: every thread issues
+// its 16 loads before any LDS store, so they are in flight together.
+template <typename R>
+__device__ __forceinline__ void tile_to_lds(const R* __restrict__ src, int64_t ld, R (*T)[kLd]) {
+  const int t = threadIdx.x;
+  R v[16];
+#pragma unroll
+  for (int it = 0; it < 16; ++it) {
+    const int idx = t + 256 * it, c = idx >> 6, r = idx & 63;
+    v[it] = src[r + c * ld];
+  }
+#pragma unroll
+  for (int it = 0; it < 16; ++it) {
+    const int idx = t + 256 * it, c = idx >> 6, r = idx & 63;
+    T[r][c] = v[it];
+  }
+}
+
 // Thread t of a 256-thread block owns rows 4*(t>>4) .. +3 and columns 4*(t&15) .. +3 of a
 // 64x64 tile, in registers.  Register arrays are only indexed with compile-time indices;
 // a runtime column or row is selected with unrolled predicated loops.
@@ -174,8 +192,9 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
   if (blockIdx.x == 0) {
     // the diagonal factor is kept as its inverse (column-major) for the back-substitution
     R* Lk = Ld + (int64_t)k * kTile * kTile;
-    for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
-      const int c = idx / kTile, r = idx % kTile;
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+      const int idx = t + 256 * it, c = idx >> 6, r = idx & 63;
       Lk[idx] = Xi[r][c];
     }
     // y_k = L_kk^-1 b_k
@@ -188,10 +207,7 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
     return;
   }
   // L_ik = A_ik (L_kk^-1)^T, with A_ik staged in Lw (L_kk is no longer needed)
-  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
-    const int c = idx / kTile, r = idx % kTile;
-    Lw[r][c] = A[(bi + r) + (bk + c) * lda];
-  }
+  tile_to_lds<R>(A + bi + bk * lda, lda, Lw);
   __syncthreads();
   const int tr = t >> 4, ts = t & 15;
   R acc[4][4];
@@ -233,6 +249,7 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
     __shared__ R part[4][kTile];
     const int r = t & 63, cg = t >> 6;
     R acc = 0;
+#pragma unroll
     for (int c = cg; c < kTile; c += 4) acc += A[(bi + r) + (bk + c) * lda] * b[bk + c];
     part[cg][r] = acc;
     __syncthreads();
@@ -245,11 +262,8 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
   while (ii * (ii + 1) / 2 > task) --ii;
   const int jj = task - ii * (ii + 1) / 2;
   const int64_t bi = (int64_t)(k + 1 + ii) * kTile, bj = (int64_t)(k + 1 + jj) * kTile;
-  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
-    const int c = idx / kTile, r = idx % kTile;
-    Li[r][c] = A[(bi + r) + (bk + c) * lda];
-    Lj[r][c] = A[(bj + r) + (bk + c) * lda];
-  }
+  tile_to_lds<R>(A + bi + bk * lda, lda, Li);
+  tile_to_lds<R>(A + bj + bk * lda, lda, Lj);
   __syncthreads();
   const int tr = t >> 4, tc = t & 15;
   R acc[4][4];
@@ -269,10 +283,15 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * bb[q];
   }
+  R old[4][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int p = 0; p < 4; ++p) A[(bi + 4 * tr + p) + (bj + 4 * tc + q) * lda] -= acc[p][q];
+    for (int p = 0; p < 4; ++p) old[p][q] = A[(bi + 4 * tr + p) + (bj + 4 * tc + q) * lda];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) A[(bi + 4 * tr + p) + (bj + 4 * tc + q) * lda] = old[p][q] - acc[p][q];
 }
 
 template <typename R>
@@ -286,10 +305,7 @@ chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __r
   const int t = threadIdx.x;
   const int64_t bk = (int64_t)k * kTile;
   const R* Lk = Ld + (int64_t)k * kTile * kTile;     // L_kk^-1, column-major
-  for (int idx = t; idx < kTile * kTile; idx += blockDim.x) {
-    const int c = idx / kTile, r = idx % kTile;
-    Li[r][c] = Lk[idx];
-  }
+  tile_to_lds<R>(Lk, kTile, Li);
   if (t < kTile) ys[t] = y[bk + t];
   __syncthreads();
   // x_k = L_kk^-T y_k = (L_kk^-1)^T y_k
@@ -311,6 +327,7 @@ chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __r
   const int64_t bj = (int64_t)j * kTile;
   const int c = t & 63, rg = t >> 6;
   R acc = 0;
+#pragma unroll
   for (int r = rg; r < kTile; r += 4) acc += A[(bk + r) + (bj + c) * lda] * xs[r];
   __syncthreads();
   part[rg][c] = acc;

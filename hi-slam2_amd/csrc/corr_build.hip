@@ -88,16 +88,39 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
 
+  constexpr int EPL = 16 / sizeof(Ti);           // elements per 16-byte load
+  struct alignas(16) Vec { Ti v[EPL]; };
+  const int qpr = WT / EPL;                        // 16-byte loads per image-2 row of the tile
+  const int nb_loads = kKC * NQ / EPL;             // B-tile loads per chunk (a multiple of 256)
   for (int c0 = 0; c0 < C; c0 += kKC) {
-    // stage [pixel][channel] chunks; global reads run along pixels (coalesced)
-    for (int idx = t; idx < kKC * kTP; idx += kThreads) {
-      const int c = idx / kTP, p = idx % kTP;
-      s1[p * KP + c] = a[(int64_t)(c0 + c) * HW + p0 + p];
+    // stage [pixel][channel] chunks with 16-byte loads along pixels; all of a thread's
+    // loads are issued before the transposing LDS stores
+    if (t < kKC * kTP / EPL) {
+      const int c = t / (kTP / EPL), pv = t % (kTP / EPL);
+      const Vec va = *reinterpret_cast<const Vec*>(a + (int64_t)(c0 + c) * HW + p0 + pv * EPL);
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) s1[(pv * EPL + e) * KP + c] = va.v[e];
     }
-    for (int idx = t; idx < kKC * NQ; idx += kThreads) {
-      const int c = idx / NQ, q = idx % NQ;
-      const int qy = y0 + q / WT, qx = x0 + q % WT;
-      s2[q * KP + c] = b[(int64_t)(c0 + c) * HW + qy * w + qx];
+    for (int base = 0; base < nb_loads; base += 8 * kThreads) {
+      Vec vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + u * kThreads + t;
+        if (idx < nb_loads) {
+          const int c = idx / (NQ / EPL), qv = idx % (NQ / EPL);
+          const int qy = y0 + qv / qpr, qx = x0 + (qv % qpr) * EPL;
+          vb[u] = *reinterpret_cast<const Vec*>(b + (int64_t)(c0 + c) * HW + qy * w + qx);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + u * kThreads + t;
+        if (idx < nb_loads) {
+          const int c = idx / (NQ / EPL), qv = idx % (NQ / EPL);
+#pragma unroll
+          for (int e = 0; e < EPL; ++e) s2[(qv * EPL + e) * KP + c] = vb[u].v[e];
+        }
+      }
     }
     __syncthreads();
     const int i = lane % 32;
@@ -132,21 +155,26 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   }
   __syncthreads();
 
-  // level 0: rows of WT contiguous elements (or whole 8x8 tiles in the tiled layout)
+  // level 0: groups of 8 elements that are consecutive in memory (a row segment of the
+  // tile, or one row of an 8x8 block in the tiled layout), one vector store each
   const bool tile0 = tiled && (h % 8 == 0) && (w % 8 == 0);
-  for (int idx = t; idx < kTP * NQ; idx += kThreads) {
-    const int p = idx / NQ, q = idx % NQ;
+  struct alignas(8 * sizeof(To)) Out8 { To v[8]; };
+  for (int g = t; g < kTP * NQ / 8; g += kThreads) {
+    const int p = g / (NQ / 8), q8 = g % (NQ / 8);
     int qy, qx;
-    if (tile0) {            // consecutive idx walk the tiles of the 8-row band in memory order
-      const int tx = q >> 6, e = q & 63;
-      qy = y0 + (e >> 3);
-      qx = x0 + tx * 8 + (e & 7);
+    if (tile0) {            // consecutive groups walk the 8x8 blocks of the 8-row band in memory order
+      qy = y0 + (q8 & 7);
+      qx = x0 + (q8 >> 3) * 8;
     } else {
-      qy = y0 + q / WT;
-      qx = x0 + q % WT;
+      qy = y0 + (q8 * 8) / WT;
+      qx = x0 + (q8 * 8) % WT;
     }
+    const float* src = sout + p * NQ + (qy - y0) * WT + (qx - x0);
+    Out8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o.v[e] = from_f<To>(src[e]);
     const int off = tile0 ? plane_offset(qy, qx, w) : qy * w + qx;
-    l0[((int64_t)n * HW + p0 + p) * HW + off] = from_f<To>(sout[p * NQ + (qy - y0) * WT + (qx - x0)]);
+    *reinterpret_cast<Out8*>(l0 + ((int64_t)n * HW + p0 + p) * HW + off) = o;
   }
   // pooled levels: level L averages 2^L x 2^L blocks of level 0
   To* outs[3] = {l1, l2, l3};

@@ -184,20 +184,36 @@ corr_lookup_win_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   const int wave = t >> 6, lane = t & 63;
   const int64_t plane = (int64_t)h2 * w2;
   const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
-  for (int p = wave; p < npix; p += 4) {
-    const int xs = s_x0[p], ys = s_y0[p];
-    const T* pl = vbase + p * plane;
+  // each wave gathers the windows of pixels wave, wave+4, ...: all of a lane's loads are
+  // issued before any is consumed, so up to 16 * KT gathers are in flight per lane
+  constexpr int PPW = kPix / 4;
+  constexpr int KT = (D2 + 63) / 64;
+  A v[PPW][KT];
+  bool okv[PPW][KT];
 #pragma unroll
-    for (int k = lane; k < D2; k += 64) {
+  for (int j = 0; j < PPW; ++j) {
+    const int p = wave + 4 * j;
+    const int xs = s_x0[p], ys = s_y0[p];
+    const T* pl = vbase + (p < npix ? p : 0) * plane;
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) {
+      const int k = lane + 64 * kt;
       const int ty = k / D, tx = k - ty * D;    // constant divisor
       const int y = ys + ty, x = xs + tx;
-      const bool ok = (unsigned)y < (unsigned)h2 && (unsigned)x < (unsigned)w2;
+      const bool ok = p < npix && k < D2 && (unsigned)y < (unsigned)h2 && (unsigned)x < (unsigned)w2;
       const int off = tiled ? (((((y >> 3) * (w2 >> 3)) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7))
                             : y * w2 + x;
-      const A v = load_acc<T>(pl + (ok ? off : 0));   // out-of-plane taps read element 0, then zero
-      taps[p * TS + k] = ok ? v : A(0);
+      v[j][kt] = load_acc<T>(pl + (ok ? off : 0));   // out-of-plane taps read element 0, then zero
+      okv[j][kt] = ok;
     }
   }
+#pragma unroll
+  for (int j = 0; j < PPW; ++j)
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) {
+      const int k = lane + 64 * kt;
+      if (k < D2) taps[(wave + 4 * j) * TS + k] = okv[j][kt] ? v[j][kt] : A(0);
+    }
   __syncthreads();
 
   const int p = lane, g = wave;
