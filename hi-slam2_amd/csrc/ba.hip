@@ -168,18 +168,8 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   const float* dsp = disps + (int64_t)ix * HW;
   const float* tg = targets + (int64_t)e * 2 * HW;
   const float* wt = weights + (int64_t)e * 2 * HW;
-  // software pipelining: the next pixel's five inputs are loaded before this pixel's math
-  int k = threadIdx.x;
-  float in_d = 0.f, in_tu = 0.f, in_tv = 0.f, in_wu = 0.f, in_wv = 0.f;
-  if (k < HW) {
-    in_d = dsp[k]; in_tu = tg[k]; in_tv = tg[HW + k]; in_wu = wt[k]; in_wv = wt[HW + k];
-  }
-  for (; k < HW; k += kLinThreads) {
-    const float d = in_d, tu = in_tu, tv = in_tv, wu = in_wu, wv = in_wv;
-    const int kn = k + kLinThreads;
-    if (kn < HW) {
-      in_d = dsp[kn]; in_tu = tg[kn]; in_tv = tg[HW + kn]; in_wu = wt[kn]; in_wv = wt[HW + kn];
-    }
+  for (int k = threadIdx.x; k < HW; k += kLinThreads) {
+    const float d = dsp[k], tu = tg[k], tv = tg[HW + k], wu = wt[k], wv = wt[HW + k];
     const int row = k / wd;
     const int col = k - row * wd;
     RowJac ru, rv;

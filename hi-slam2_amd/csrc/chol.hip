@@ -51,6 +51,70 @@ __device__ __forceinline__ void tile_to_lds(const R* __restrict__ src, int64_t l
 // 64x64 tile, in registers.  Register arrays are only indexed with compile-time indices;
 // a runtime column or row is selected with unrolled predicated loops.
 
+// 1/x and 1/sqrt(x) from the hardware estimates refined by Newton steps (two for double,
+// reaching full precision from the ~2^-26 estimate; one for float), instead of the IEEE
+// division / square-root sequences, which sit on the factorisation's critical path.
+__device__ __forceinline__ double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  r = r * (2.0 - x * r);
+  return r;
+}
+__device__ __forceinline__ float fast_rcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return r * (2.f - x * r);
+}
+__device__ __forceinline__ double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y;
+}
+__device__ __forceinline__ float fast_rsqrt(float x) {
+  float y = __builtin_amdgcn_rsqf(x);
+  return y * (1.5f - 0.5f * x * y * y);
+}
+
+// One level of the recursive-doubling inverse (block size m = 2^LG -> 2m).
+template <typename R, int LG>
+__device__ __forceinline__ void inverse_level(R (*Lw)[kLd], R (*Xi)[kLd], R* tmp) {
+  constexpr int m = 1 << LG;
+  constexpr int total = 32 * m;                  // (64 / 2m) blocks of m x m
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int e0 = 0; e0 < total; e0 += 256) {
+    const int e = e0 + t;
+    if (total >= 256 || e < total) {
+      const int b = e >> (2 * LG), i = (e >> LG) & (m - 1), j = e & (m - 1);
+      const int o = 2 * m * b;
+      R v0 = 0, v1 = 0;                          // (L21 X11)[i][j]; X11 lower, so kk >= j
+#pragma unroll
+      for (int kk = 0; kk < m; kk += 2) {
+        v0 += kk >= j ? Lw[o + m + i][o + kk] * Xi[o + kk][o + j] : R(0);
+        v1 += kk + 1 >= j ? Lw[o + m + i][o + kk + 1] * Xi[o + kk + 1][o + j] : R(0);
+      }
+      tmp[e] = v0 + v1;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e0 = 0; e0 < total; e0 += 256) {
+    const int e = e0 + t;
+    if (total >= 256 || e < total) {
+      const int b = e >> (2 * LG), i = (e >> LG) & (m - 1), j = e & (m - 1);
+      const int o = 2 * m * b;
+      R v0 = 0, v1 = 0;                          // -(X22 T)[i][j]; X22 lower, so kk <= i
+#pragma unroll
+      for (int kk = 0; kk < m; kk += 2) {
+        v0 += kk <= i ? Xi[o + m + i][o + m + kk] * tmp[(b << (2 * LG)) + (kk << LG) + j] : R(0);
+        v1 += kk + 1 <= i ? Xi[o + m + i][o + m + kk + 1] * tmp[(b << (2 * LG)) + ((kk + 1) << LG) + j] : R(0);
+      }
+      Xi[o + m + i][o + j] = -(v0 + v1);
+    }
+  }
+  __syncthreads();
+}
+
 // Factor the diagonal tile (k, k) in registers, then invert the factor.  On return Xi
 // (LDS) holds L_kk^-1 (lower triangular, zeros above) and Lw holds L_kk (lower part).
 // A non-positive pivot sets info[0] = 1 when `report` and is replaced by 1 so that the
@@ -98,8 +162,8 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
       piv = R(1);
     }
     if (owner) {
-      const R d = sqrt(piv);
-      const R invd = R(1) / d;
+      const R invd = fast_rsqrt(piv);
+      const R d = piv * invd;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (q != (c & 3)) continue;
@@ -116,7 +180,7 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
         }
       }
     }
-    const R ip = R(1) / piv;
+    const R ip = fast_rcp(piv);
     R cr[4], cs[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
@@ -139,6 +203,9 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
     for (int p = 0; p < 4; ++p)
 #pragma unroll
       for (int q = 0; q < 4; ++q) l[p][q] = q <= p ? Lw[4 * tr + p][4 * tr + q] : R(0);
+    R dinv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dinv[i] = fast_rcp(l[i][i]);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -146,7 +213,7 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
         R v = i == j ? R(1) : R(0);
 #pragma unroll
         for (int m = 0; m < i; ++m) v -= l[i][m] * x[m][j];
-        x[i][j] = i >= j ? v / l[i][i] : R(0);
+        x[i][j] = i >= j ? v * dinv[i] : R(0);
       }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -155,27 +222,10 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
   }
   __syncthreads();
   // recursive doubling: blocks of size 2m from blocks of size m
-  for (int lg = 2; lg <= 5; ++lg) {
-    const int m = 1 << lg, mm = m * m;
-    const int total = 32 * m;               // (64 / 2m) blocks of m x m
-    for (int e = t; e < total; e += 256) {
-      const int b = e >> (2 * lg), i = (e >> lg) & (m - 1), j = e & (m - 1);
-      const int o = 2 * m * b;
-      R v = 0;                              // (L21 X11)[i][j], X11 lower: kk >= j
-      for (int kk = j; kk < m; ++kk) v += Lw[o + m + i][o + kk] * Xi[o + kk][o + j];
-      tmp[e] = v;
-    }
-    __syncthreads();
-    for (int e = t; e < total; e += 256) {
-      const int b = e >> (2 * lg), i = (e >> lg) & (m - 1), j = e & (m - 1);
-      const int o = 2 * m * b;
-      R v = 0;                              // -(X22 T)[i][j], X22 lower: kk <= i
-      for (int kk = 0; kk <= i; ++kk) v += Xi[o + m + i][o + m + kk] * tmp[(b << (2 * lg)) + (kk << lg) + j];
-      Xi[o + m + i][o + j] = -v;
-    }
-    __syncthreads();
-    (void)mm;
-  }
+  inverse_level<R, 2>(Lw, Xi, tmp);
+  inverse_level<R, 3>(Lw, Xi, tmp);
+  inverse_level<R, 4>(Lw, Xi, tmp);
+  inverse_level<R, 5>(Lw, Xi, tmp);
 }
 
 template <typename R>
@@ -215,6 +265,7 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = 0;
+#pragma unroll 8
   for (int m = 0; m < kTile; ++m) {
     R av[4], lv[4];
 #pragma unroll
@@ -271,6 +322,7 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = 0;
+#pragma unroll 8
   for (int kk = 0; kk < kTile; ++kk) {
     R a[4], bb[4];
 #pragma unroll
