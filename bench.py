@@ -172,11 +172,7 @@ def main():
     value = iters * K / split['t_total']
     lookup_s = split['lookup_ms'] / 1e3
     achieved = LOOKUP_BYTES_BF16 * lookups / lookup_s / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, 'profiles', 'lookup_traffic.json')
-    if os.path.exists(tf):
-        with open(tf) as fh:
-            traffic = json.load(fh).get('bytes_per_launch')
+    traffic = lookup_traffic()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -206,7 +202,7 @@ def main():
         'reproj_rms_split_px': round(res['split']['rms'], 6),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': traffic,
-                     'kernel': 'corr_lookup_fwd_kernel<bf16> (4 levels, 9.216M lookups/launch, '
+                     'kernel': 'corr_lookup_win_kernel<bf16, 3> (4 levels, 9.216M lookups/launch, '
                                '912 B/lookup algorithmic)'},
         'cpu_baseline': cpu,
     }
@@ -221,6 +217,19 @@ def main():
             'reproj_diff_px': round(abs(res['split']['rms'] - res['plain']['rms']), 7),
         })
     print(json.dumps(line), flush=True)
+
+
+def lookup_traffic():
+    """HBM bytes of one bf16 lookup launch, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a 2 GiB device copy); None if absent."""
+    path = os.path.join(ROOT, 'profiles', 'corr_lookup_bf16_global300_pmc.md')
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for ln in fh:
+            if ln.startswith('bytes_per_launch'):
+                return int(ln.split(':')[1])
+    return None
 
 
 def cpu_baseline(p, fm32, coords):

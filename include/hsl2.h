@@ -28,7 +28,7 @@ enum {
   HSL2_ERR_ARGUMENT = 1,   /* null pointer, unknown dtype, bad scalar argument */
   HSL2_ERR_SHAPE = 2,      /* sizes inconsistent with each other */
   HSL2_ERR_INDEX = 3,      /* an edge / frame index outside its valid range */
-  HSL2_ERR_HIP = 4,        /* a HIP runtime or rocSOLVER call failed */
+  HSL2_ERR_HIP = 4,        /* a HIP runtime call failed */
   HSL2_ERR_WORKSPACE = 5,  /* caller-supplied workspace too small */
   HSL2_ERR_UNSUPPORTED = 6 /* valid request this build does not implement */
 };
