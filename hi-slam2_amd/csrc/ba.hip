@@ -20,493 +20,10 @@
 //   6. ba_backsub        dz = Q (w - sum E^T dx) with the reference's t0 skip, and
 //                        the disparity update (EvT6x1, disp_retr, 1562-1582, 1398-1411).
 //   7. ba_pose_retr      T <- Exp(dx) T (1342-1396).
-#include "ba_plan.hpp"
-#include "common.hpp"
+#include "ba_kernels.hpp"
 
 namespace hsl2 {
-
-template <typename R>
-int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_t s);
-
 namespace {
-
-constexpr float kMinDepth = 0.25f;   // src/droid_kernels.cu:26
-constexpr int kLinThreads = 256;
-
-// ---- SE3 helpers (xyzw quaternions, tangent [tau, phi]) ----------------------
-__device__ __forceinline__ void quat_rotate(const float q[4], const float X[3], float Y[3]) {
-  // Y = X + 2 w (v x X) + 2 v x (v x X)   (src/droid_kernels.cu:74-84)
-  const float uvx = 2.f * (q[1] * X[2] - q[2] * X[1]);
-  const float uvy = 2.f * (q[2] * X[0] - q[0] * X[2]);
-  const float uvz = 2.f * (q[0] * X[1] - q[1] * X[0]);
-  Y[0] = X[0] + q[3] * uvx + (q[1] * uvz - q[2] * uvy);
-  Y[1] = X[1] + q[3] * uvy + (q[2] * uvx - q[0] * uvz);
-  Y[2] = X[2] + q[3] * uvz + (q[0] * uvy - q[1] * uvx);
-}
-
-// G_ij = G_j G_i^-1 (src/droid_kernels.cu:148-159)
-__device__ __forceinline__ void relative_pose(const float* pi, const float* pj, float tij[3],
-                                              float qij[4]) {
-  const float* qi = pi + 3;
-  const float* qj = pj + 3;
-  qij[0] = -qj[3] * qi[0] + qj[0] * qi[3] - qj[1] * qi[2] + qj[2] * qi[1];
-  qij[1] = -qj[3] * qi[1] + qj[1] * qi[3] - qj[2] * qi[0] + qj[0] * qi[2];
-  qij[2] = -qj[3] * qi[2] + qj[2] * qi[3] - qj[0] * qi[1] + qj[1] * qi[0];
-  qij[3] = qj[3] * qi[3] + qj[0] * qi[0] + qj[1] * qi[1] + qj[2] * qi[2];
-  float r[3];
-  quat_rotate(qij, pi, r);
-  tij[0] = pj[0] - r[0];
-  tij[1] = pj[1] - r[1];
-  tij[2] = pj[2] - r[2];
-}
-
-__device__ __forceinline__ void quat_to_rot(const float q[4], float R[9]) {
-  const float x = q[0], y = q[1], z = q[2], w = q[3];
-  R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - z * w);     R[2] = 2 * (x * z + y * w);
-  R[3] = 2 * (x * y + z * w);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - x * w);
-  R[6] = 2 * (x * z - y * w);     R[7] = 2 * (y * z + x * w);     R[8] = 1 - 2 * (x * x + y * y);
-}
-
-struct EdgeGeom {
-  float R[9];
-  float t[3];
-  float fx, fy, cx, cy;
-};
-
-__device__ __forceinline__ void load_edge_geom(const float* __restrict__ poses,
-                                               const float* __restrict__ intr, int ix, int jx,
-                                               EdgeGeom& g) {
-  float pi[7], pj[7], q[4];
-#pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    pi[k] = poses[ix * 7 + k];
-    pj[k] = poses[jx * 7 + k];
-  }
-  relative_pose(pi, pj, g.t, q);
-  quat_to_rot(q, g.R);
-  g.fx = intr[0];
-  g.fy = intr[1];
-  g.cx = intr[2];
-  g.cy = intr[3];
-}
-
-// One residual row (u or v) of one pixel: weight, residual, Jacobians.
-struct RowJac {
-  float w, r, Jz;
-  float Ji[6], Jj[6];
-};
-
-// Per-pixel geometry of projective_transform_kernel (src/droid_kernels.cu:451-561).
-// Returns false for an invalid pixel (transformed depth < 0.25): then every weight is 0.
-__device__ __forceinline__ bool pixel_rows(const EdgeGeom& g, float u, float v, float d,
-                                           float tu, float tv, float wtu, float wtv,
-                                           RowJac& ru, RowJac& rv) {
-  const float X0 = (u - g.cx) / g.fx;
-  const float Y0 = (v - g.cy) / g.fy;
-  const float x = g.R[0] * X0 + g.R[1] * Y0 + g.R[2] + d * g.t[0];
-  const float y = g.R[3] * X0 + g.R[4] * Y0 + g.R[5] + d * g.t[1];
-  const float zr = g.R[6] * X0 + g.R[7] * Y0 + g.R[8] + d * g.t[2];
-  if (!(zr >= kMinDepth)) return false;
-  const float z = zr;
-  const float dinv = 1.f / zr;
-  const float d2 = dinv * dinv;
-  ru.w = 0.001f * wtu;
-  rv.w = 0.001f * wtv;
-  ru.r = tu - (g.fx * (x * dinv) + g.cx);
-  rv.r = tv - (g.fy * (y * dinv) + g.cy);
-  const float P0 = g.fx * dinv, P2 = -g.fx * x * d2;   // u row: [P0, 0, P2]
-  const float Q1 = g.fy * dinv, Q2 = -g.fy * y * d2;   // v row: [0, Q1, Q2]
-  ru.Jj[0] = P0 * d;  ru.Jj[1] = 0.f;    ru.Jj[2] = P2 * d;
-  ru.Jj[3] = P2 * y;  ru.Jj[4] = P0 * z - P2 * x;  ru.Jj[5] = -P0 * y;
-  rv.Jj[0] = 0.f;     rv.Jj[1] = Q1 * d; rv.Jj[2] = Q2 * d;
-  rv.Jj[3] = -Q1 * z + Q2 * y;  rv.Jj[4] = -Q2 * x;  rv.Jj[5] = Q1 * x;
-  ru.Jz = P0 * g.t[0] + P2 * g.t[2];
-  rv.Jz = Q1 * g.t[1] + Q2 * g.t[2];
-  // Ji = -Adj(G_ij)^T Jj: tau part -R^T Jt, phi part -R^T (Jr - t x Jt) (src/droid_kernels.cu:104-119)
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    RowJac& J = k == 0 ? ru : rv;
-    const float a0 = J.Jj[0], a1 = J.Jj[1], a2 = J.Jj[2];
-    const float c0 = J.Jj[3] - (g.t[1] * a2 - g.t[2] * a1);
-    const float c1 = J.Jj[4] - (g.t[2] * a0 - g.t[0] * a2);
-    const float c2 = J.Jj[5] - (g.t[0] * a1 - g.t[1] * a0);
-    J.Ji[0] = -(g.R[0] * a0 + g.R[3] * a1 + g.R[6] * a2);
-    J.Ji[1] = -(g.R[1] * a0 + g.R[4] * a1 + g.R[7] * a2);
-    J.Ji[2] = -(g.R[2] * a0 + g.R[5] * a1 + g.R[8] * a2);
-    J.Ji[3] = -(g.R[0] * c0 + g.R[3] * c1 + g.R[6] * c2);
-    J.Ji[4] = -(g.R[1] * c0 + g.R[4] * c1 + g.R[7] * c2);
-    J.Ji[5] = -(g.R[2] * c0 + g.R[5] * c1 + g.R[8] * c2);
-  }
-  return true;
-}
-
-__device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
-
-// ---- 1. per-edge linearization ----------------------------------------------
-template <bool kSchur>
-__global__ void __launch_bounds__(kLinThreads)
-ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
-             const float* __restrict__ intr, const float* __restrict__ targets,
-             const float* __restrict__ weights, const int* __restrict__ edge_ii,
-             const int* __restrict__ edge_jj, int HW, int wd, int E,
-             double* __restrict__ Hs, double* __restrict__ vs, float* __restrict__ Eij,
-             float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz) {
-  __shared__ double lds[16 * kLinThreads];
-  __shared__ double red[90];
-  const int e = blockIdx.x;
-  const int ix = edge_ii[e], jx = edge_jj[e];
-  const bool self = ix == jx;
-  EdgeGeom g;
-  load_edge_geom(poses, intr, ix, jx, g);
-
-  // Per-pixel products are formed in FP32; their sums are kept in FP64 (the FP64 VALU
-  // rate equals the FP32 rate on this part, so this costs registers, not throughput).
-  double acc[90];
-#pragma unroll
-  for (int k = 0; k < 90; ++k) acc[k] = 0.0;
-
-  const float* dsp = disps + (int64_t)ix * HW;
-  const float* tg = targets + (int64_t)e * 2 * HW;
-  const float* wt = weights + (int64_t)e * 2 * HW;
-  for (int k = threadIdx.x; k < HW; k += kLinThreads) {
-    const float d = dsp[k], tu = tg[k], tv = tg[HW + k], wu = wt[k], wv = wt[HW + k];
-    const int row = k / wd;
-    const int col = k - row * wd;
-    RowJac ru, rv;
-    const bool ok = pixel_rows(g, static_cast<float>(col), static_cast<float>(row), d, tu, tv, wu,
-                               wv, ru, rv);
-    if (kSchur) {
-      float c = 0.f, b = 0.f, ei[6] = {0, 0, 0, 0, 0, 0}, ej[6] = {0, 0, 0, 0, 0, 0};
-      if (ok) {
-        c = ru.w * ru.Jz * ru.Jz + rv.w * rv.Jz * rv.Jz;
-        b = ru.w * ru.r * ru.Jz + rv.w * rv.r * rv.Jz;
-        if (!self) {
-#pragma unroll
-          for (int n = 0; n < 6; ++n) {
-            ei[n] = ru.w * ru.Jz * ru.Ji[n] + rv.w * rv.Jz * rv.Ji[n];
-            ej[n] = ru.w * ru.Jz * ru.Jj[n] + rv.w * rv.Jz * rv.Jj[n];
-          }
-        }
-      }
-      Cii[(int64_t)e * HW + k] = c;
-      bz[(int64_t)e * HW + k] = b;
-#pragma unroll
-      for (int n = 0; n < 6; ++n) {
-        Eii[((int64_t)e * 6 + n) * HW + k] = ei[n];
-        Eij[((int64_t)e * 6 + n) * HW + k] = ej[n];
-      }
-    }
-    if (ok && !self) {
-      float Ju[12], Jv[12];
-#pragma unroll
-      for (int n = 0; n < 6; ++n) {
-        Ju[n] = ru.Ji[n];
-        Ju[6 + n] = ru.Jj[n];
-        Jv[n] = rv.Ji[n];
-        Jv[6 + n] = rv.Jj[n];
-      }
-#pragma unroll
-      for (int n = 0; n < 12; ++n) {
-        const float wun = ru.w * Ju[n], wvn = rv.w * Jv[n];
-#pragma unroll
-        for (int m = 0; m <= n; ++m) acc[tri(n, m)] += static_cast<double>(wun * Ju[m] + wvn * Jv[m]);
-      }
-      const float wru = ru.w * ru.r, wrv = rv.w * rv.r;
-#pragma unroll
-      for (int n = 0; n < 12; ++n) acc[78 + n] += static_cast<double>(wru * Ju[n] + wrv * Jv[n]);
-    }
-  }
-
-  block_reduce_store<90, kLinThreads, double>(acc, lds, red);
-  __syncthreads();
-  const int t = threadIdx.x;
-  if (t < 144) {
-    const int q = t / 36, i = (t % 36) / 6, j = t % 6;
-    int l;
-    if (q == 0) l = tri(max(i, j), min(i, j));
-    else if (q == 1) l = tri(6 + j, i);
-    else if (q == 2) l = tri(6 + i, j);
-    else l = tri(6 + max(i, j), 6 + min(i, j));
-    Hs[((int64_t)q * E + e) * 36 + i * 6 + j] = red[l];
-  } else if (t < 156) {
-    const int k = t - 144;
-    vs[((int64_t)(k / 6) * E + e) * 6 + (k % 6)] = red[78 + k];
-  }
-}
-
-// ---- 2. per-frame sums: C, w, Q and the pose-i Schur block Ei ----------------
-__global__ void __launch_bounds__(256)
-ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
-                const float* __restrict__ Eii, const float* __restrict__ eta, int eta_rows,
-                const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
-                const int* __restrict__ frame_pose, int HW, float* __restrict__ Q,
-                float* __restrict__ w, float* __restrict__ Eall) {
-  const int f = blockIdx.y;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= HW) return;
-  const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
-  double c = 0.0, b = 0.0;
-  int q = e0;
-  for (; q + 4 <= e1; q += 4) {                  // four edges' loads in flight together
-    float cv[4], bv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = frame_edges[q + u];
-      cv[u] = Cii[(int64_t)e * HW + k];
-      bv[u] = bz[(int64_t)e * HW + k];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      c += cv[u];
-      b += bv[u];
-    }
-  }
-  for (; q < e1; ++q) {
-    const int e = frame_edges[q];
-    c += Cii[(int64_t)e * HW + k];
-    b += bz[(int64_t)e * HW + k];
-  }
-  const float et = eta[(int64_t)(eta_rows == 1 ? 0 : f) * HW + k];
-  Q[(int64_t)f * HW + k] = static_cast<float>(1.0 / (c + et));
-  w[(int64_t)f * HW + k] = static_cast<float>(b);
-  const int p = frame_pose[f];
-  if (p >= 0) {
-    double ei[6] = {0, 0, 0, 0, 0, 0};
-    int q2 = e0;
-    for (; q2 + 2 <= e1; q2 += 2) {
-      float v[2][6];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int e = frame_edges[q2 + u];
-#pragma unroll
-        for (int n = 0; n < 6; ++n) v[u][n] = Eii[((int64_t)e * 6 + n) * HW + k];
-      }
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int n = 0; n < 6; ++n) ei[n] += v[u][n];
-    }
-    for (; q2 < e1; ++q2) {
-      const int e = frame_edges[q2];
-#pragma unroll
-      for (int n = 0; n < 6; ++n) ei[n] += Eii[((int64_t)e * 6 + n) * HW + k];
-    }
-#pragma unroll
-    for (int n = 0; n < 6; ++n) Eall[((int64_t)p * 6 + n) * HW + k] = static_cast<float>(ei[n]);
-  }
-}
-
-// ---- 3. Schur Gram blocks and Ev, one wave per task --------------------------
-__global__ void __launch_bounds__(256)
-ba_schur_gram(const float* __restrict__ Eall, const float* __restrict__ Q,
-              const float* __restrict__ w, const int* __restrict__ tasks,
-              const int* __restrict__ slot_row, int NT, int NPAIR, int HW, int task_blocks8,
-              double* __restrict__ S, double* __restrict__ Ev) {
-  // XCD-aware order: the 8 XCDs receive blocks round-robin, so give each XCD a
-  // contiguous run of tasks (tasks of one depth frame share their E rows in L2).
-  const int b = blockIdx.x;
-  const int tb = (b % 8) * task_blocks8 + b / 8;
-  const int lane = threadIdx.x % kWave;
-  const int task = tb * 4 + threadIdx.x / kWave;
-  if (task >= NT) return;
-  const int kind = tasks[task * 4 + 0];
-  const int sa = tasks[task * 4 + 1];
-  const int sb = tasks[task * 4 + 2];
-  const int f = tasks[task * 4 + 3];
-  const float* Qf = Q + (int64_t)f * HW;
-  const float* Ea = Eall + (int64_t)slot_row[sa] * 6 * HW;
-  if (kind == kTaskPair) {
-    const float* Eb = Eall + (int64_t)slot_row[sb] * 6 * HW;
-    double acc[36];
-#pragma unroll
-    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
-    for (int k = lane; k < HW; k += kWave) {
-      const double q = Qf[k];
-      double ea[6], eb[6];
-#pragma unroll
-      for (int n = 0; n < 6; ++n) {
-        ea[n] = Ea[n * HW + k] * q;
-        eb[n] = Eb[n * HW + k];
-      }
-#pragma unroll
-      for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int j = 0; j < 6; ++j) acc[i * 6 + j] += ea[i] * eb[j];
-    }
-#pragma unroll
-    for (int k = 0; k < 36; ++k) {
-      const double v = wave_sum(acc[k]);
-      if (lane == k) S[(int64_t)(task) * 36 + k] = v;
-    }
-  } else {
-    const float* wf = w + (int64_t)f * HW;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = lane; k < HW; k += kWave) {
-      const double qw = static_cast<double>(Qf[k]) * wf[k];
-#pragma unroll
-      for (int n = 0; n < 6; ++n) acc[n] += qw * Ea[n * HW + k];
-    }
-#pragma unroll
-    for (int n = 0; n < 6; ++n) {
-      const double v = wave_sum(acc[n]);
-      if (lane == n) Ev[(int64_t)sa * 6 + n] = v;
-    }
-  }
-}
-
-// ---- 4. reduced camera system, dense column-major ----------------------------
-template <typename R>
-__global__ void __launch_bounds__(256)
-ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
-            const int* __restrict__ blk_ptr, const int* __restrict__ blk_src, int E, int P,
-            int64_t n_pad, float lm, float ep, R* __restrict__ A, int* __restrict__ info) {
-  const int64_t n = 6 * (int64_t)P;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx == 0) info[0] = 0;                     // this iteration's factorisation status
-  if (idx >= n_pad * n_pad) return;
-  const int64_t col = idx / n_pad;
-  const int64_t row = idx - col * n_pad;
-  if (row >= n || col >= n) {                    // identity padding up to a multiple of 64
-    A[idx] = row == col ? R(1) : R(0);
-    return;
-  }
-  const int a = static_cast<int>(row / 6), i = static_cast<int>(row % 6);
-  const int b = static_cast<int>(col / 6), j = static_cast<int>(col % 6);
-  const int64_t blk = (int64_t)a * P + b;
-  double v = 0;
-  for (int q = blk_ptr[blk]; q < blk_ptr[blk + 1]; ++q) {
-    const int code = blk_src[q];
-    const int kind = code & 7;
-    const int64_t id = code >> 3;
-    if (kind < 4) v += Hs[((int64_t)kind * E + id) * 36 + i * 6 + j];
-    else if (kind == kBlkS) v -= S[id * 36 + i * 6 + j];
-    else v -= S[id * 36 + j * 6 + i];
-  }
-  if (row == col) v += static_cast<double>(ep) + static_cast<double>(lm) * v;   // src/droid_kernels.cu:1663-1664
-  A[idx] = static_cast<R>(v);
-}
-
-template <typename R>
-__global__ void __launch_bounds__(256)
-ba_assemble_rhs(const double* __restrict__ vs, const double* __restrict__ Ev,
-                const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
-                int n_pad, R* __restrict__ bvec) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n_pad) return;
-  if (idx >= 6 * P) {
-    bvec[idx] = R(0);
-    return;
-  }
-  const int a = idx / 6, i = idx % 6;
-  double v = 0;
-  for (int q = rhs_ptr[a]; q < rhs_ptr[a + 1]; ++q) {
-    const int code = rhs_src[q];
-    const int kind = code & 3;
-    const int64_t id = code >> 2;
-    if (kind == kRhsVi) v += vs[id * 6 + i];
-    else if (kind == kRhsVj) v += vs[((int64_t)E + id) * 6 + i];
-    else v -= Ev[id * 6 + i];
-  }
-  bvec[idx] = static_cast<R>(v);
-}
-
-// ---- 6. back-substitution and disparity update -------------------------------
-template <typename R>
-__global__ void __launch_bounds__(256)
-ba_backsub(const float* __restrict__ Eall, const float* __restrict__ Q,
-           const float* __restrict__ w, const R* __restrict__ x, const int* __restrict__ info,
-           const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
-           const int* __restrict__ slot_pose, const int* __restrict__ frame_kx, int HW,
-           float* __restrict__ disps, float* __restrict__ dz) {
-  const int f = blockIdx.y;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= HW) return;
-  const bool solved = info[0] == 0;
-  R acc = 0;
-  if (solved) {
-    for (int s = slot_ptr[f]; s < slot_ptr[f + 1]; ++s) {
-      const int a = slot_pose[s];
-      if (a <= 0) continue;                       // t0 skip, src/droid_kernels.cu:1572
-      const float* Er = Eall + (int64_t)slot_row[s] * 6 * HW + k;
-#pragma unroll
-      for (int n = 0; n < 6; ++n) acc += static_cast<R>(Er[(int64_t)n * HW]) * x[6 * a + n];
-    }
-  }
-  const float qv = Q[(int64_t)f * HW + k];
-  const float d = static_cast<float>(static_cast<R>(qv) * (static_cast<R>(w[(int64_t)f * HW + k]) - acc));
-  if (dz) dz[(int64_t)f * HW + k] = d;
-  disps[(int64_t)frame_kx[f] * HW + k] += d;
-}
-
-// ---- 7. pose retraction -------------------------------------------------------
-__device__ void retr_se3(const float xi[6], float* pose) {
-  // Exp(xi) * T (src/droid_kernels.cu:179-244, 1342-1360)
-  const float* tau = xi;
-  const float* phi = xi + 3;
-  const float th2 = phi[0] * phi[0] + phi[1] * phi[1] + phi[2] * phi[2];
-  const float th = sqrtf(th2);
-  float imag, real;
-  if (th2 < 1e-8f) {
-    imag = 0.5f - (1.f / 48.f) * th2 + (1.f / 3840.f) * th2 * th2;
-    real = 1.f - (1.f / 8.f) * th2 + (1.f / 384.f) * th2 * th2;
-  } else {
-    imag = sinf(0.5f * th) / th;
-    real = cosf(0.5f * th);
-  }
-  const float dq[4] = {imag * phi[0], imag * phi[1], imag * phi[2], real};
-  float dt[3] = {tau[0], tau[1], tau[2]};
-  if (th > 1e-4f) {
-    const float a = (1.f - cosf(th)) / th2;
-    const float b = (th - sinf(th)) / (th * th2);
-    const float c1[3] = {phi[1] * tau[2] - phi[2] * tau[1], phi[2] * tau[0] - phi[0] * tau[2],
-                         phi[0] * tau[1] - phi[1] * tau[0]};
-    const float c2[3] = {phi[1] * c1[2] - phi[2] * c1[1], phi[2] * c1[0] - phi[0] * c1[2],
-                         phi[0] * c1[1] - phi[1] * c1[0]};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) dt[k] += a * c1[k] + b * c2[k];
-  }
-  const float* q = pose + 3;
-  float q1[4];
-  q1[0] = dq[3] * q[0] + dq[0] * q[3] + dq[1] * q[2] - dq[2] * q[1];
-  q1[1] = dq[3] * q[1] + dq[1] * q[3] + dq[2] * q[0] - dq[0] * q[2];
-  q1[2] = dq[3] * q[2] + dq[2] * q[3] + dq[0] * q[1] - dq[1] * q[0];
-  q1[3] = dq[3] * q[3] - dq[0] * q[0] - dq[1] * q[1] - dq[2] * q[2];
-  float t1[3];
-  quat_rotate(dq, pose, t1);
-  pose[0] = t1[0] + dt[0];
-  pose[1] = t1[1] + dt[1];
-  pose[2] = t1[2] + dt[2];
-  pose[3] = q1[0];
-  pose[4] = q1[1];
-  pose[5] = q1[2];
-  pose[6] = q1[3];
-}
-
-template <typename R>
-__global__ void ba_pose_retr(float* __restrict__ poses, const R* __restrict__ x,
-                             int* __restrict__ info, int t0, int P, float* __restrict__ dx) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool solved = info[0] == 0;
-  if (a == 0 && !solved) info[1] = 1;            // remember a failed solve
-  if (a >= P) return;
-  float xi[6];
-#pragma unroll
-  for (int n = 0; n < 6; ++n) xi[n] = solved ? static_cast<float>(x[6 * a + n]) : 0.f;
-  if (dx) {
-#pragma unroll
-    for (int n = 0; n < 6; ++n) dx[6 * a + n] = xi[n];
-  }
-  float pose[7];
-#pragma unroll
-  for (int n = 0; n < 7; ++n) pose[n] = poses[(int64_t)(t0 + a) * 7 + n];
-  retr_se3(xi, pose);
-#pragma unroll
-  for (int n = 0; n < 7; ++n) poses[(int64_t)(t0 + a) * 7 + n] = pose[n];
-}
-
-__global__ void ba_clear_flags(int* info) {
-  if (threadIdx.x < 4) info[threadIdx.x] = 0;
-}
 
 template <typename R>
 int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const float* intr,
@@ -541,33 +58,32 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
     const bool last = it == iterations - 1;
     if (E > 0) {
       if (schur)
-        hipLaunchKernelGGL(ba_linearize<true>, dim3(E), dim3(kLinThreads), 0, s, poses, disps, intr,
-                           targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
+        hipLaunchKernelGGL((ba_linearize<true, kSemCuda>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+                           intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
                            Hs, vs, Eall + (int64_t)P * 6 * HW, Eii, Cii, bz);
       else
-        hipLaunchKernelGGL(ba_linearize<false>, dim3(E), dim3(kLinThreads), 0, s, poses, disps, intr,
-                           targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
+        hipLaunchKernelGGL((ba_linearize<false, kSemCuda>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+                           intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
                            Hs, vs, nullptr, nullptr, nullptr, nullptr);
       HSL2_LAUNCH_CHECK();
     }
     if (schur) {
       hipLaunchKernelGGL(ba_frame_reduce, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii,
                          eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
-                         blob + p->o_frame_pose, HW, Q, wv, Eall);
+                         blob + p->o_frame_pose, HW, 0.f, Q, wv, Eall);
       HSL2_LAUNCH_CHECK();
       const int nt = static_cast<int>(p->NT);
       if (nt > 0) {
         const int tblocks = (nt + 3) / 4;
         const int tb8 = (tblocks + 7) / 8;
         hipLaunchKernelGGL(ba_schur_gram, dim3(tb8 * 8), dim3(256), 0, s, Eall, Q, wv,
-                           blob + p->o_task, blob + p->o_slot_row, nt, static_cast<int>(p->NPAIR),
-                           HW, tb8, S, Ev);
+                           blob + p->o_task, blob + p->o_slot_row, 0, nt, HW, tb8, S, Ev);
         HSL2_LAUNCH_CHECK();
       }
     }
     const int64_t nn = n_pad * n_pad;
     hipLaunchKernelGGL(ba_assemble<R>, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, s,
-                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, A, info);
+                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, 0, A, info);
     HSL2_LAUNCH_CHECK();
     hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3(static_cast<unsigned>((n_pad + 255) / 256)), dim3(256), 0, s,
                        vs, Ev, blob + p->o_rhs_ptr, blob + p->o_rhs_src, E, P,
@@ -580,12 +96,12 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
     if (schur) {
       hipLaunchKernelGGL(ba_backsub<R>, dim3((HW + 255) / 256, F), dim3(256), 0, s, Eall, Q, wv,
                          xvec, info, blob + p->o_slot_ptr, blob + p->o_slot_row,
-                         blob + p->o_slot_pose, blob + p->o_frame_kx, HW, disps,
+                         blob + p->o_slot_pose, blob + p->o_frame_kx, HW, 1, disps,
                          last ? dz : nullptr);
       HSL2_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(ba_pose_retr<R>, dim3((P + 63) / 64), dim3(64), 0, s, poses, xvec, info,
-                       p->t0, P, last ? dx : nullptr);
+                       p->t0, P, 0, last ? dx : nullptr);
     HSL2_LAUNCH_CHECK();
   }
   return HSL2_OK;

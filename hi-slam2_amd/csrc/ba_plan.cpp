@@ -20,6 +20,14 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t num_edges,
                                    int64_t num_poses, int64_t ht, int64_t wd, int t0, int t1,
                                    int motion_only, int precision, hsl2_ba_plan** out) {
+  return hsl2_ba_plan_create_ex(ii, jj, num_edges, num_poses, ht, wd, t0, t1, motion_only,
+                                precision, HSL2_SEM_CUDA, out);
+}
+
+extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int64_t num_edges,
+                                      int64_t num_poses, int64_t ht, int64_t wd, int t0, int t1,
+                                      int motion_only, int precision, int semantics,
+                                      hsl2_ba_plan** out) {
   using hsl2::set_error;
   if (!out) { set_error("ba plan: null output pointer"); return HSL2_ERR_ARGUMENT; }
   *out = nullptr;
@@ -38,6 +46,14 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
   if (precision != HSL2_BA_SPLIT && precision != HSL2_BA_FP32) {
     set_error("ba plan: unknown precision");
     return HSL2_ERR_ARGUMENT;
+  }
+  if (semantics != HSL2_SEM_CUDA && semantics != HSL2_SEM_TORCH) {
+    set_error("ba plan: unknown semantics");
+    return HSL2_ERR_ARGUMENT;
+  }
+  if (semantics == HSL2_SEM_TORCH && precision != HSL2_BA_SPLIT) {
+    set_error("ba plan: the torch-semantics step solves in FP64 only");
+    return HSL2_ERR_UNSUPPORTED;
   }
   if (num_edges >= (1ll << 27)) { set_error("ba plan: too many edges"); return HSL2_ERR_SHAPE; }
   for (int64_t e = 0; e < num_edges; ++e) {
@@ -60,9 +76,13 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
   p->P = t1 - t0;
   p->motion_only = motion_only ? 1 : 0;
   p->precision = precision;
+  p->semantics = semantics;
   const int64_t E = num_edges, P = p->P, HW = p->HW;
 
   // ---- kx = sorted unique([t0, t1) ++ ii), kk_exp (src/droid_kernels.cu:1841-1849) ----
+  // The torch formulation's depth frames are unique(ii) only (hislam2/geom/ba.py:70); a
+  // window frame without edges then has no depth block and its pseudo-edge is unused
+  // (kk_exp = -1).
   std::vector<int64_t> ii_exp(P + E), jj_exp(P + E);
   for (int64_t k = 0; k < P; ++k) ii_exp[k] = jj_exp[k] = t0 + k;
   for (int64_t e = 0; e < E; ++e) {
@@ -70,7 +90,7 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
     jj_exp[P + e] = jj[e];
   }
   std::vector<int32_t> frame_of(static_cast<size_t>(t1), -1);
-  for (int64_t n = 0; n < P + E; ++n) frame_of[ii_exp[n]] = 0;
+  for (int64_t n = (semantics == HSL2_SEM_TORCH ? P : 0); n < P + E; ++n) frame_of[ii_exp[n]] = 0;
   for (int64_t k = 0; k < t1; ++k)
     if (frame_of[k] == 0) {
       frame_of[k] = static_cast<int32_t>(p->kx.size());
@@ -118,7 +138,7 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
     std::vector<std::vector<int32_t>> per_frame(F);
     for (int64_t n = 0; n < P + E; ++n) {
       const int64_t a = jj_exp[n] - t0;
-      if (a >= 0 && a < P) per_frame[p->kk_exp[n]].push_back(static_cast<int32_t>(n));
+      if (a >= 0 && a < P && p->kk_exp[n] >= 0) per_frame[p->kk_exp[n]].push_back(static_cast<int32_t>(n));
     }
     for (int64_t f = 0; f < F; ++f) {
       slot_ptr[f + 1] = slot_ptr[f] + static_cast<int32_t>(per_frame[f].size());
@@ -162,14 +182,16 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
     while (B.size() % 64) B.push_back(0);
     return off;
   };
-  std::vector<int32_t> eii(E), ejj(E), fkx(F);
+  std::vector<int32_t> eii(E), ejj(E), efr(E), fkx(F);
   for (int64_t e = 0; e < E; ++e) {
     eii[e] = static_cast<int32_t>(ii[e]);
     ejj[e] = static_cast<int32_t>(jj[e]);
+    efr[e] = static_cast<int32_t>(p->kk_exp[P + e]);
   }
   for (int64_t f = 0; f < F; ++f) fkx[f] = static_cast<int32_t>(p->kx[f]);
   p->o_edge_ii = put(eii);
   p->o_edge_jj = put(ejj);
+  p->o_edge_frame = put(efr);
   p->o_frame_kx = put(fkx);
   p->o_frame_pose = put(frame_pose);
   p->o_frame_eptr = put(frame_eptr);
@@ -221,6 +243,12 @@ extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t
   p->w_x = take(static_cast<size_t>(p->n_pad) * real);
   p->w_Ld = take(static_cast<size_t>(p->n_pad) * 64 * real);
   p->w_info = take(4 * sizeof(int32_t));
+  if (semantics == HSL2_SEM_TORCH) {
+    p->w_lz = take(schur ? F * HW * f4 : 0);
+    p->w_dpart = take(E * HW * f4);
+    p->w_W = take(static_cast<size_t>(p->n_pad) * p->n_pad * f8);
+    p->w_Sinv = take(static_cast<size_t>(p->n_pad) * p->n_pad * f8);
+  }
   p->total = off;
   *out = p;
   return HSL2_OK;

@@ -16,6 +16,7 @@ struct hsl2_ba_plan {
   int t0 = 0, t1 = 0, P = 0;
   int motion_only = 0;
   int precision = 0;
+  int semantics = 0;      // HSL2_SEM_CUDA or HSL2_SEM_TORCH
   int64_t F = 0;          // unique depth frames |kx|
   int64_t NS = 0;         // Schur slots (expanded edges with a pose slot in [0, P))
   int64_t NPAIR = 0;      // slot pairs (s <= s') inside a frame
@@ -27,6 +28,7 @@ struct hsl2_ba_plan {
   std::vector<int32_t> blob;
   // offsets (in int32 elements) of each table inside blob
   int64_t o_edge_ii = 0, o_edge_jj = 0;        // [E]
+  int64_t o_edge_frame = 0;                    // [E] depth-frame index of ii[e]
   int64_t o_frame_kx = 0, o_frame_pose = 0;    // [F]
   int64_t o_frame_eptr = 0, o_frame_edges = 0; // [F+1], [E]
   int64_t o_slot_ptr = 0;                      // [F+1]
@@ -54,6 +56,11 @@ struct hsl2_ba_plan {
   size_t w_x = 0;        // f64 or f32 [n_pad]: solution
   size_t w_Ld = 0;       // f64 or f32 [n_pad / 64][64][64]: diagonal Cholesky factors
   size_t w_info = 0;     // int32 [2]: solver info, all-ok flag
+  // torch-semantics step only
+  size_t w_lz = 0;       // f32 [F][HW]: depth part of the adjoint solution
+  size_t w_dpart = 0;    // f32 [E][HW]: per-edge disparity-state gradients of the adjoint
+  size_t w_W = 0;        // f64 [n_pad][n_pad]: inverse of the Cholesky factor (dzcov)
+  size_t w_Sinv = 0;     // f64 [n_pad][n_pad]: inverse of the reduced system (dzcov)
   size_t total = 0;
 };
 

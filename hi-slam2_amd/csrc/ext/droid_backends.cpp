@@ -364,6 +364,205 @@ std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, 
 
 }  // namespace
 
+// ---- projective transform of the torch formulation (hislam2/geom/projective_ops.py:44-74) ----
+void check_projective(torch::Tensor& poses, torch::Tensor& disps, torch::Tensor& intrinsics, torch::Tensor& ii,
+                      torch::Tensor& jj) {
+  for (auto* t : {&poses, &disps, &intrinsics, &ii, &jj}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, "projective_transform tensors");
+  }
+  for (auto* t : {&poses, &disps, &intrinsics})
+    require(t->scalar_type() == torch::kFloat32, "projective_transform: poses, disps, intrinsics must be float32");
+  require(ii.scalar_type() == torch::kInt64 && jj.scalar_type() == torch::kInt64 && ii.dim() == 1 &&
+              jj.dim() == 1 && ii.size(0) == jj.size(0),
+          "projective_transform: ii, jj must be int64 [E]");
+  const int64_t N = poses.size(0);
+  require(poses.dim() == 2 && poses.size(1) == 7, "projective_transform: poses must be [N, 7]");
+  require(disps.dim() == 3 && disps.size(0) == N, "projective_transform: disps must be [N, h, w]");
+  require(intrinsics.numel() == N * 4, "projective_transform: intrinsics must be [N, 4]");
+  if (ii.size(0) > 0) {
+    auto lo = torch::stack({ii.min(), jj.min()}).min().item<int64_t>();
+    auto hi = torch::stack({ii.max(), jj.max()}).max().item<int64_t>();
+    require(lo >= 0 && hi < N, "projective_transform: an edge indexes a frame outside [0, N)");
+  }
+}
+
+std::vector<torch::Tensor> projective_transform(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                                torch::Tensor ii, torch::Tensor jj, bool jacobian,
+                                                bool return_depth) {
+  check_projective(poses, disps, intrinsics, ii, jj);
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  const int64_t E = ii.size(0), h = disps.size(1), w = disps.size(2);
+  auto opts = poses.options();
+  torch::Tensor coords = torch::empty({E, h, w, return_depth ? 3 : 2}, opts);
+  torch::Tensor valid = torch::empty({E, h, w, 1}, opts);
+  torch::Tensor Ji, Jj, Jz;
+  if (jacobian) {
+    Ji = torch::empty({E, h, w, 2, 6}, opts);
+    Jj = torch::empty({E, h, w, 2, 6}, opts);
+    Jz = torch::empty({E, h, w, 2, 1}, opts);
+  }
+  raise_status(hsl2_projective_transform(poses.data_ptr<float>(), disps.data_ptr<float>(),
+                                         intrinsics.data_ptr<float>(), ii.data_ptr<int64_t>(),
+                                         jj.data_ptr<int64_t>(), E, h, w, return_depth ? 1 : 0,
+                                         coords.data_ptr<float>(), valid.data_ptr<float>(),
+                                         jacobian ? Ji.data_ptr<float>() : nullptr,
+                                         jacobian ? Jj.data_ptr<float>() : nullptr,
+                                         jacobian ? Jz.data_ptr<float>() : nullptr, current_stream()),
+               "projective_transform");
+  return {coords, valid, Ji, Jj, Jz};
+}
+
+// returns [grad_poses (left tangent, [N, 6]), grad_disps [N, h, w]]
+std::vector<torch::Tensor> projective_transform_backward(torch::Tensor poses, torch::Tensor disps,
+                                                         torch::Tensor intrinsics, torch::Tensor ii,
+                                                         torch::Tensor jj, torch::Tensor grad_coords,
+                                                         bool return_depth) {
+  check_projective(poses, disps, intrinsics, ii, jj);
+  CHECK_CONTIGUOUS(grad_coords);
+  const int64_t N = poses.size(0), E = ii.size(0), h = disps.size(1), w = disps.size(2);
+  require(grad_coords.is_cuda() && grad_coords.scalar_type() == torch::kFloat32 &&
+              grad_coords.numel() == E * h * w * (return_depth ? 3 : 2),
+          "projective_transform backward: grad_coords must be float32 [E, h, w, 2 or 3]");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  auto opts = poses.options();
+  torch::Tensor gp = torch::empty({N, 6}, opts), gd = torch::empty({N, h, w}, opts);
+  const size_t bytes = hsl2_projective_transform_workspace_bytes(E, h, w);
+  torch::Tensor ws = torch::empty({static_cast<int64_t>(bytes) + 256}, opts.dtype(torch::kUInt8));
+  uintptr_t base = reinterpret_cast<uintptr_t>(ws.data_ptr());
+  void* ws_ptr = reinterpret_cast<void*>((base + 255) & ~uintptr_t(255));
+  raise_status(hsl2_projective_transform_backward(poses.data_ptr<float>(), disps.data_ptr<float>(),
+                                                  intrinsics.data_ptr<float>(), ii.data_ptr<int64_t>(),
+                                                  jj.data_ptr<int64_t>(), E, N, h, w, return_depth ? 1 : 0,
+                                                  grad_coords.data_ptr<float>(), gp.data_ptr<float>(),
+                                                  gd.data_ptr<float>(), ws_ptr, bytes, current_stream()),
+               "projective_transform backward");
+  return {gp, gd};
+}
+
+// ---- one GN step of the torch formulation (hislam2/geom/ba.py:32-158) -------------
+// Holds the plan of one (ii, jj, fixedp) graph and the workspace of its last forward step,
+// which the backward of the same step reads.
+class GNStep {
+ public:
+  GNStep(torch::Tensor ii, torch::Tensor jj, int64_t num_poses, int64_t ht, int64_t wd, int fixedp,
+         bool motion_only)
+      : N_(num_poses), ht_(ht), wd_(wd), motion_only_(motion_only) {
+    require(ii.dim() == 1 && jj.dim() == 1 && ii.size(0) == jj.size(0), "gn_step: ii, jj must be [E]");
+    require(ii.scalar_type() == torch::kInt64 && jj.scalar_type() == torch::kInt64,
+            "gn_step: ii, jj must be int64");
+    torch::Tensor ii_h = ii.to(torch::kCPU).contiguous(), jj_h = jj.to(torch::kCPU).contiguous();
+    E_ = ii.size(0);
+    raise_status(hsl2_ba_plan_create_ex(ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E_, num_poses,
+                                        ht, wd, fixedp, static_cast<int>(num_poses), motion_only ? 1 : 0,
+                                        HSL2_BA_SPLIT, HSL2_SEM_TORCH, &plan_),
+                 "gn_step");
+    F_ = hsl2_ba_plan_num_frames(plan_);
+    P_ = num_poses - fixedp;
+    kx_ = torch::empty({F_}, torch::kInt64);
+    raise_status(hsl2_ba_plan_frames(plan_, kx_.data_ptr<int64_t>()), "gn_step");
+  }
+  ~GNStep() { hsl2_ba_plan_destroy(plan_); }
+  GNStep(const GNStep&) = delete;
+  GNStep& operator=(const GNStep&) = delete;
+
+  torch::Tensor frames() const { return kx_; }
+
+  // poses [N, 7] and disps [N, ht, wd] are updated in place; returns [dx, dz, dzcov, ok]
+  std::vector<torch::Tensor> forward(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                     torch::Tensor targets, torch::Tensor weights, torch::Tensor eta,
+                                     double lm, double ep, bool want_cov) {
+    check_state(poses, disps, intrinsics, targets, weights);
+    const int64_t HW = ht_ * wd_;
+    const float* eta_ptr = nullptr;
+    if (!motion_only_) {
+      CHECK_CONTIGUOUS(eta);
+      require_device(eta, "eta");
+      require(eta.scalar_type() == torch::kFloat32 && eta.numel() == F_ * HW,
+              "gn_step: eta must be float32 with |unique(ii)| x ht x wd elements");
+      eta_ptr = eta.data_ptr<float>();
+    }
+    c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+    auto opts = poses.options();
+    ws_ = torch::empty({static_cast<int64_t>(hsl2_ba_plan_workspace_bytes(plan_)) + 256}, opts.dtype(torch::kUInt8));
+    torch::Tensor dx = torch::zeros({P_, 6}, opts);
+    torch::Tensor dz = motion_only_ ? torch::Tensor() : torch::zeros({F_, HW}, opts);
+    torch::Tensor cov = (!motion_only_ && want_cov) ? torch::zeros({F_, HW}, opts) : torch::Tensor();
+    int ok = 1;
+    raise_status(hsl2_gn_step(plan_, poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                              E_ ? targets.data_ptr<float>() : nullptr, E_ ? weights.data_ptr<float>() : nullptr,
+                              eta_ptr, static_cast<float>(lm), static_cast<float>(ep), dx.data_ptr<float>(),
+                              dz.defined() ? dz.data_ptr<float>() : nullptr,
+                              cov.defined() ? cov.data_ptr<float>() : nullptr, ws_ptr(), ws_bytes(), &ok,
+                              current_stream()),
+                 "gn_step");
+    return {dx, dz, cov, torch::tensor({ok}, torch::kInt32)};
+  }
+
+  // gradients of one step given the step's input state and outputs; returns
+  // [grad_targets, grad_weights, grad_eta, grad_poses (tangent, [N, 6]), grad_disps]
+  std::vector<torch::Tensor> backward(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                      torch::Tensor targets, torch::Tensor weights, double lm, double ep,
+                                      torch::Tensor dx, torch::Tensor dz, torch::Tensor grad_dx,
+                                      torch::Tensor grad_dz) {
+    require(ws_.defined(), "gn_step: backward before forward");
+    check_state(poses, disps, intrinsics, targets, weights);
+    const int64_t HW = ht_ * wd_;
+    for (auto* t : {&dx, &grad_dx}) {
+      CHECK_CONTIGUOUS(*t);
+      require(t->is_cuda() && t->scalar_type() == torch::kFloat32 && t->numel() == P_ * 6,
+              "gn_step: dx and its gradient must be float32 [N - fixedp, 6]");
+    }
+    if (!motion_only_)
+      for (auto* t : {&dz, &grad_dz}) {
+        CHECK_CONTIGUOUS(*t);
+        require(t->is_cuda() && t->scalar_type() == torch::kFloat32 && t->numel() == F_ * HW,
+                "gn_step: dz and its gradient must be float32 [|unique(ii)|, ht * wd]");
+      }
+    c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+    auto opts = poses.options();
+    torch::Tensor gt = torch::empty({E_, 2, ht_, wd_}, opts), gw = torch::empty({E_, 2, ht_, wd_}, opts);
+    torch::Tensor geta = motion_only_ ? torch::Tensor() : torch::empty({F_, ht_, wd_}, opts);
+    torch::Tensor gp = torch::empty({N_, 6}, opts), gd = torch::empty({N_, ht_, wd_}, opts);
+    raise_status(hsl2_gn_step_backward(
+                     plan_, poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                     E_ ? targets.data_ptr<float>() : nullptr, E_ ? weights.data_ptr<float>() : nullptr,
+                     static_cast<float>(lm), static_cast<float>(ep), dx.data_ptr<float>(),
+                     motion_only_ ? nullptr : dz.data_ptr<float>(), grad_dx.data_ptr<float>(),
+                     motion_only_ ? nullptr : grad_dz.data_ptr<float>(), E_ ? gt.data_ptr<float>() : nullptr,
+                     E_ ? gw.data_ptr<float>() : nullptr, motion_only_ ? nullptr : geta.data_ptr<float>(),
+                     gp.data_ptr<float>(), gd.data_ptr<float>(), ws_ptr(), ws_bytes(), current_stream()),
+                 "gn_step backward");
+    return {gt, gw, geta, gp, gd};
+  }
+
+ private:
+  void check_state(torch::Tensor& poses, torch::Tensor& disps, torch::Tensor& intrinsics, torch::Tensor& targets,
+                   torch::Tensor& weights) {
+    for (auto* t : {&poses, &disps, &intrinsics, &targets, &weights}) {
+      CHECK_CONTIGUOUS(*t);
+      require_device(*t, "gn_step tensors");
+      require(t->scalar_type() == torch::kFloat32, "gn_step: tensors must be float32");
+    }
+    require(poses.dim() == 2 && poses.size(0) == N_ && poses.size(1) == 7, "gn_step: poses must be [N, 7]");
+    require(disps.dim() == 3 && disps.size(0) == N_ && disps.size(1) == ht_ && disps.size(2) == wd_,
+            "gn_step: disps must be [N, ht, wd]");
+    require(intrinsics.numel() == N_ * 4, "gn_step: intrinsics must be [N, 4]");
+    require(targets.numel() == E_ * 2 * ht_ * wd_ && weights.numel() == E_ * 2 * ht_ * wd_,
+            "gn_step: targets and weights must be [E, 2, ht, wd]");
+  }
+  void* ws_ptr() const {
+    uintptr_t base = reinterpret_cast<uintptr_t>(ws_.data_ptr());
+    return reinterpret_cast<void*>((base + 255) & ~uintptr_t(255));
+  }
+  size_t ws_bytes() const { return static_cast<size_t>(ws_.numel() - 256); }
+
+  hsl2_ba_plan* plan_ = nullptr;
+  int64_t N_, ht_, wd_, E_ = 0, F_ = 0, P_ = 0;
+  bool motion_only_;
+  torch::Tensor kx_, ws_;
+};
+
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "MI355X (gfx950) dense-tracking kernels behind the droid_backends interface";
   // bundle adjustment (src/droid.cpp:327)
@@ -382,6 +581,20 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
         py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none(),
         py::arg("tiled") = false);
+  m.def("projective_transform", &projective_transform,
+        "torch-formulation projective transform: [coords, valid, Ji, Jj, Jz]", py::arg("poses"),
+        py::arg("disps"), py::arg("intrinsics"), py::arg("ii"), py::arg("jj"), py::arg("jacobian") = false,
+        py::arg("return_depth") = false);
+  m.def("projective_transform_backward", &projective_transform_backward,
+        "adjoint of projective_transform: [grad_poses (tangent), grad_disps]");
+  py::class_<GNStep>(m, "GNStep", "one Gauss-Newton step of the torch formulation (hislam2/geom/ba.py) "
+                                  "with its hand-written adjoint")
+      .def(py::init<torch::Tensor, torch::Tensor, int64_t, int64_t, int64_t, int, bool>(), py::arg("ii"),
+           py::arg("jj"), py::arg("num_poses"), py::arg("ht"), py::arg("wd"), py::arg("fixedp"),
+           py::arg("motion_only") = false)
+      .def("frames", &GNStep::frames)
+      .def("forward", &GNStep::forward)
+      .def("backward", &GNStep::backward);
   m.attr("PRECISION_SPLIT") = py::int_(static_cast<int>(HSL2_BA_SPLIT));
   m.attr("PRECISION_FP32") = py::int_(static_cast<int>(HSL2_BA_FP32));
 }

@@ -47,6 +47,12 @@ enum {
   HSL2_BA_FP32 = 1    /* the same kernels with an FP32 reduced Cholesky and back-substitution */
 };
 
+/* ---- BA semantics ------------------------------------------------------------- */
+enum {
+  HSL2_SEM_CUDA = 0,  /* ba_cuda of src/droid_kernels.cu:1820-1937 (the droid_backends.ba binding) */
+  HSL2_SEM_TORCH = 1  /* one GN step of the torch formulation, hislam2/geom/ba.py:32-158 + chol.py */
+};
+
 int hsl2_version(void);
 const char* hsl2_status_string(int status);
 const char* hsl2_last_error(void);
@@ -134,6 +140,13 @@ typedef struct hsl2_ba_plan hsl2_ba_plan;
 int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t num_edges,
                         int64_t num_poses, int64_t ht, int64_t wd, int t0, int t1,
                         int motion_only, int precision, hsl2_ba_plan** plan);
+/* hsl2_ba_plan_create with a choice of semantics.  HSL2_SEM_TORCH plans one step of the torch
+ * formulation (BA / MoBA of hislam2/geom/ba.py:32-158): t0 plays `fixedp` (poses before it are
+ * held fixed), t1 = num_poses, the depth frames are unique(ii) (ba.py:70) and the precision
+ * must be HSL2_BA_SPLIT.  motion_only = 1 plans MoBA (ba.py:110-158, rig = 1). */
+int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int64_t num_edges,
+                           int64_t num_poses, int64_t ht, int64_t wd, int t0, int t1,
+                           int motion_only, int precision, int semantics, hsl2_ba_plan** plan);
 void hsl2_ba_plan_destroy(hsl2_ba_plan* plan);
 /* number of unique depth frames |kx| (rows of eta and of dz) */
 int64_t hsl2_ba_plan_num_frames(const hsl2_ba_plan* plan);
@@ -156,6 +169,57 @@ int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps, const floa
                 const float* targets, const float* weights, const float* eta, int64_t eta_rows,
                 int iterations, float lm, float ep, float* dx, float* dz,
                 void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
+
+/* One Gauss-Newton step in the torch semantics: replaces BA (hislam2/geom/ba.py:32-107, with
+ * schur_solve and CholeskySolver of hislam2/geom/chol.py:5-78) and, for a motion-only plan,
+ * MoBA (ba.py:110-158 with block_solve).  Device arrays: poses [num_poses, 7] and disps
+ * [num_poses, ht, wd] f32 updated in place (disps > 10 -> 0, then clamped at 0.001 as in
+ * ba.py:104-105), intrinsics [num_poses, 4] f32 (one row per frame), targets / weights
+ * [num_edges, 2, ht, wd] f32, eta [num_frames, ht, wd] f32 (num_frames = |unique(ii)|).
+ * Outputs: dx [num_poses - t0, 6] f32; dz [num_frames, ht * wd] f32 (the raw Schur solution,
+ * before the clamp); dzcov [num_frames, ht * wd] f32 (may be NULL) = sum(square(L^-1 E Q)) + Q
+ * (chol.py:74-76).  A failed factorisation gives dx = 0 (chol.py:14-18) and dzcov = Q.
+ * The workspace must stay untouched until hsl2_gn_step_backward of the same step. */
+int hsl2_gn_step(const hsl2_ba_plan* plan, float* poses, float* disps, const float* intrinsics,
+                 const float* targets, const float* weights, const float* eta, float lm, float ep,
+                 float* dx, float* dz, float* dzcov, void* workspace, size_t workspace_bytes,
+                 int* solve_ok, void* stream);
+
+/* Adjoint of hsl2_gn_step: the backward that autograd forms through BA with CholeskySolver's
+ * implicit adjoint (hislam2/geom/chol.py:22-31), written out and evaluated per pixel with the
+ * Jacobians recomputed.  Inputs: the step's INPUT state (poses, disps), intrinsics, targets,
+ * weights, lm, ep, the step's outputs dx and dz, and the gradients grad_dx [num_poses-t0, 6] and
+ * grad_dz [num_frames, ht*wd] (grad_dz / dz unused for a motion-only plan).  Outputs:
+ * grad_targets / grad_weights [num_edges, 2, ht, wd], grad_eta [num_frames, ht, wd],
+ * grad_poses [num_poses, 6] (tangent: left perturbation Exp(xi) T of each input pose) and
+ * grad_disps [num_poses, ht, wd] (through the linearization only; the identity path of
+ * disps + dz is the caller's).  Uses the workspace the forward step left. */
+int hsl2_gn_step_backward(const hsl2_ba_plan* plan, const float* poses, const float* disps,
+                          const float* intrinsics, const float* targets, const float* weights,
+                          float lm, float ep, const float* dx, const float* dz,
+                          const float* grad_dx, const float* grad_dz, float* grad_targets,
+                          float* grad_weights, float* grad_eta, float* grad_poses,
+                          float* grad_disps, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Replaces projective_transform of the torch formulation (hislam2/geom/projective_ops.py:44-74
+ * with pinhole.py:6-53) for one batch element.  Device arrays: poses [N, 7], disps [N, ht, wd],
+ * intrinsics [N, 4] f32, ii / jj [num_edges] int64 (must index [0, N)).  Outputs: coords
+ * [num_edges, ht, wd, 2] (3 with return_depth: x, y, d / Z), valid [num_edges, ht, wd] (may be
+ * NULL), and, all or none, Ji / Jj [num_edges, ht, wd, 2, 6], Jz [num_edges, ht, wd, 2]. */
+int hsl2_projective_transform(const float* poses, const float* disps, const float* intrinsics,
+                              const int64_t* ii, const int64_t* jj, int64_t num_edges, int64_t ht,
+                              int64_t wd, int return_depth, float* coords, float* valid, float* Ji,
+                              float* Jj, float* Jz, void* stream);
+/* Adjoint of hsl2_projective_transform with respect to the poses (left tangent, grad_poses
+ * [N, 6]) and the disparities (grad_disps [N, ht, wd]), given grad_coords shaped like coords;
+ * the Z < 0.1 branch of the projection has zero derivative in Z, as in the reference's code.
+ * The workspace (256-byte aligned) holds per-edge partial sums. */
+size_t hsl2_projective_transform_workspace_bytes(int64_t num_edges, int64_t ht, int64_t wd);
+int hsl2_projective_transform_backward(const float* poses, const float* disps, const float* intrinsics,
+                                       const int64_t* ii, const int64_t* jj, int64_t num_edges,
+                                       int64_t num_poses, int64_t ht, int64_t wd, int return_depth,
+                                       const float* grad_coords, float* grad_poses, float* grad_disps,
+                                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* Dense symmetric positive-definite solve A x = b (column-major n x n, F64 or F32) with the
  * tiled Cholesky that ba uses for its reduced camera system (replaces the host Eigen

@@ -83,3 +83,30 @@ def test_extension_rejects_cpu_tensors(db):
     with pytest.raises(ValueError):
         db.ba(t(p.poses), t(p.disps), t(p.intrinsics), t(p.targets), t(p.weights), t(p.eta),
               t(p.ii), t(p.jj), p.t0, p.t1, 1, 1e-4, 0.1, False)
+
+
+def test_torch_semantics_planner_frames_are_unique_ii(db):
+    """The torch formulation's depth frames are unique(ii) (hislam2/geom/ba.py:70), not
+    unique([t0, t1) ++ ii): frame 5 (inside the window, no outgoing edge) is not one."""
+    ii = torch.tensor([0, 3, 1, 3, 0])
+    jj = torch.tensor([1, 0, 3, 5, 3])
+    step = db.GNStep(ii, jj, 6, 4, 8, 2, False)
+    assert step.frames().tolist() == torch.unique(ii).tolist() == [0, 1, 3]
+    kx, _ = db.ba_index(ii, jj, 2, 6, 6)
+    assert kx.tolist() == [0, 1, 2, 3, 4, 5]
+
+
+def test_torch_semantics_planner_rejects_bad_graphs(db):
+    with pytest.raises(ValueError):
+        db.GNStep(torch.tensor([0, 6]), torch.tensor([1, 0]), 6, 4, 8, 1, False)   # frame 6 >= N
+    with pytest.raises(ValueError):
+        db.GNStep(torch.tensor([0]), torch.tensor([1]), 6, 4, 8, 6, False)         # fixedp >= N
+
+
+def test_gn_step_rejects_cpu_tensors(db):
+    step = db.GNStep(torch.tensor([0, 1]), torch.tensor([1, 0]), 2, 4, 8, 1, False)
+    z = lambda *s: torch.zeros(*s)
+    with pytest.raises(ValueError):
+        step.forward(z(2, 7), z(2, 4, 8), z(2, 4), z(2, 2, 4, 8), z(2, 2, 4, 8), z(2, 4, 8), 1e-4, 0.1, True)
+    with pytest.raises(ValueError):
+        db.projective_transform(z(2, 7), z(2, 4, 8), z(2, 4), torch.tensor([0]), torch.tensor([1]))
