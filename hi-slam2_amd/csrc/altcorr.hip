@@ -227,6 +227,155 @@ altcorr_bwd_kernel(const T* __restrict__ f1, const T* __restrict__ f2,
   }
 }
 
+// ---- fp8 (OCP e4m3fn, gfx950's FP8 format) with per-channel FP32 scales -------------------
+// Quantisation: scale_c = max |x_c| / 448 over every vector of the tensor (1 where the channel
+// is all zero), q = e4m3(x / scale_c) rounded to nearest even by the hardware convert.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+constexpr float kFp8Max = 448.f;
+
+__device__ __forceinline__ void fp8x4_to_f32(uint32_t w, float o[4]) {
+  const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w), false);
+  const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(w), true);
+  o[0] = lo[0];
+  o[1] = lo[1];
+  o[2] = hi[0];
+  o[3] = hi[1];
+}
+
+__device__ __forceinline__ float fp8_scale(uint32_t amax_bits) {
+  const float a = __uint_as_float(amax_bits);
+  return a > 0.f ? a / kFp8Max : 1.f;
+}
+
+__global__ void fill_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// per-channel max |x| over `vectors` channel-last vectors of C channels; one block per chunk
+template <typename T>
+__global__ void __launch_bounds__(256)
+fp8_amax_kernel(const T* __restrict__ x, int64_t vectors, int C, int64_t chunk, uint32_t* __restrict__ amax) {
+  const int64_t v0 = (int64_t)blockIdx.x * chunk;
+  const int64_t v1 = min(vectors, v0 + chunk);
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    float m = 0.f;
+    for (int64_t v = v0; v < v1; ++v) m = fmaxf(m, fabsf(to_f<T>(x[v * C + c])));
+    // non-negative floats order like their bit patterns
+    atomicMax(amax + c, __float_as_uint(m));
+  }
+}
+
+// four consecutive channels per thread -> one 32-bit store
+template <typename T>
+__global__ void __launch_bounds__(256)
+fp8_quantize_kernel(const T* __restrict__ x, int64_t n4, int C, const uint32_t* __restrict__ amax,
+                    uint32_t* __restrict__ q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  const int c = static_cast<int>((i * 4) % C);
+  float v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = to_f<T>(x[i * 4 + k]) / fp8_scale(amax[c + k]);
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+  q[i] = static_cast<uint32_t>(w);
+}
+
+__global__ void fp8_scales_kernel(uint32_t* __restrict__ amax_then_scale, int C) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) amax_then_scale[c] = __float_as_uint(fp8_scale(amax_then_scale[c]));
+}
+
+// altcorr forward on fp8 feature maps: fmap1 vectors are dequantised into LDS with both
+// channel scales folded in (s1_c s2_c), the taps of fmap2 are read 16 channels (16 bytes) at a
+// time and converted by the hardware, the bilinear combine is that of altcorr_fwd_kernel.
+template <typename To>
+__global__ void __launch_bounds__(kThreads)
+altcorr_fwd_fp8_kernel(const uint8_t* __restrict__ f1, const float* __restrict__ s1,
+                       const uint8_t* __restrict__ f2, const float* __restrict__ s2,
+                       const float* __restrict__ coords, const int64_t* __restrict__ ii,
+                       const int64_t* __restrict__ jj, To* __restrict__ out, int N1, int N2, int C,
+                       int HW, int W, int H2, int W2, int M, int r, int tiles) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int D = 2 * r + 2, D2 = D * D, R = 2 * r + 1;
+  float* s_f1 = reinterpret_cast<float*>(smem_raw);          // [kPix][C]
+  float* s_tap = s_f1 + kPix * C;                            // [kPix][D2]
+  __shared__ PixCoord s_c[kPix];
+
+  const int tile = blockIdx.x % tiles;
+  const int bm = blockIdx.x / tiles;
+  const int m = bm % M;
+  const int b = bm / M;
+  const int pix0 = tile * kPix;
+  const int npix = min(kPix, HW - pix0);
+  const int t = threadIdx.x;
+  const int64_t i1 = ii[m], j2 = jj[m];
+  const bool edge_ok = i1 >= 0 && i1 < N1 && j2 >= 0 && j2 < N2;
+  const uint8_t* a = f1 + (((int64_t)b * N1 + (edge_ok ? i1 : 0)) * HW + pix0) * C;
+  const uint8_t* g = f2 + ((int64_t)b * N2 + (edge_ok ? j2 : 0)) * (int64_t)H2 * W2 * C;
+
+  for (int k = 4 * t; k < npix * C; k += 4 * kThreads) {
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (edge_ok) fp8x4_to_f32(*reinterpret_cast<const uint32_t*>(a + k), v);
+    const int c = k % C;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s_f1[k + u] = v[u] * (s1[c + u] * s2[c + u]);
+  }
+  if (t < kPix) {
+    PixCoord pc{0, 0, 0.f, 0.f};
+    if (t < npix) {
+      const float* cb = coords + (int64_t)bm * 2 * HW;
+      const float cx = cb[pix0 + t], cy = cb[HW + pix0 + t];
+      const float fx = floorf(cx), fy = floorf(cy);
+      pc.x0 = static_cast<int>(fx) - r;
+      pc.y0 = static_cast<int>(fy) - r;
+      pc.dx = cx - fx;
+      pc.dy = cy - fy;
+    }
+    s_c[t] = pc;
+  }
+  __syncthreads();
+
+  const int wave = t / kWave, lane = t % kWave;
+  for (int p = wave; p < npix; p += kThreads / kWave) {
+    const PixCoord pc = s_c[p];
+    const float* av = s_f1 + p * C;
+    for (int tap = lane; tap < D2; tap += kWave) {
+      const int ty = tap / D, tx = tap - ty * D;
+      const int y1 = pc.y0 + ty, x1 = pc.x0 + tx;
+      float acc = 0.f;
+      if (edge_ok && y1 >= 0 && y1 < H2 && x1 >= 0 && x1 < W2) {
+        const uint8_t* gv = g + ((int64_t)y1 * W2 + x1) * C;
+        for (int c = 0; c < C; c += 16) {
+          const uint4 w4 = *reinterpret_cast<const uint4*>(gv + c);
+          float bv[16];
+          fp8x4_to_f32(w4.x, bv);
+          fp8x4_to_f32(w4.y, bv + 4);
+          fp8x4_to_f32(w4.z, bv + 8);
+          fp8x4_to_f32(w4.w, bv + 12);
+#pragma unroll
+          for (int k = 0; k < 16; ++k) acc += av[c + k] * bv[k];
+        }
+      }
+      s_tap[p * D2 + tap] = acc * 0.0625f;
+    }
+  }
+  __syncthreads();
+
+  To* ob = out + (int64_t)bm * R * R * HW + pix0;
+  for (int idx = t; idx < npix * R * R; idx += kThreads) {
+    const int o = idx / npix;
+    const int p = idx - o * npix;
+    const int ax = o / R, by = o - ax * R;
+    const float fx = s_c[p].dx, fy = s_c[p].dy;
+    const float* tp = s_tap + p * D2;
+    const float v = (1 - fx) * (1 - fy) * tp[by * D + ax] + fx * (1 - fy) * tp[by * D + ax + 1] +
+                    (1 - fx) * fy * tp[(by + 1) * D + ax] + fx * fy * tp[(by + 1) * D + ax + 1];
+    ob[(int64_t)o * HW + p] = from_f<To>(v);
+  }
+}
+
 int check_alt(const void* f1, const void* f2, const float* coords, const int64_t* ii,
               const int64_t* jj, int64_t B, int64_t N1, int64_t N2, int64_t C, int64_t H,
               int64_t W, int64_t H2, int64_t W2, int64_t M, int radius) {
@@ -342,6 +491,77 @@ extern "C" int hsl2_altcorr_backward(const void* fmap1, const void* fmap2, const
     default: set_error("altcorr: unsupported dtype"); return HSL2_ERR_ARGUMENT;
   }
 #undef HSL2_ALT_BWD
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+extern "C" int hsl2_fp8_quantize(const void* x, int dtype, int64_t vectors, int64_t channels, uint8_t* q,
+                                 float* scales, void* stream) {
+  HSL2_REQUIRE(x && q && scales, HSL2_ERR_ARGUMENT, "fp8_quantize: null pointer");
+  HSL2_REQUIRE(vectors >= 0 && channels > 0 && channels % 4 == 0 && channels <= 4096, HSL2_ERR_SHAPE,
+               "fp8_quantize: channels must be a positive multiple of 4 (at most 4096)");
+  hipStream_t s = as_stream(stream);
+  const int C = static_cast<int>(channels);
+  uint32_t* amax = reinterpret_cast<uint32_t*>(scales);
+  hipLaunchKernelGGL(fill_u32, dim3((C + 255) / 256), dim3(256), 0, s, amax, C);
+  HSL2_LAUNCH_CHECK();
+  if (vectors > 0) {
+    const int64_t chunk = 64;
+    const int64_t blocks = (vectors + chunk - 1) / chunk;
+    HSL2_REQUIRE(blocks < (1ll << 31), HSL2_ERR_SHAPE, "fp8_quantize: too many vectors");
+    const int64_t n4 = vectors * C / 4;
+    const unsigned qb = static_cast<unsigned>((n4 + 255) / 256);
+    switch (dtype) {
+      case HSL2_F32:
+        hipLaunchKernelGGL(fp8_amax_kernel<float>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                           static_cast<const float*>(x), vectors, C, chunk, amax);
+        hipLaunchKernelGGL(fp8_quantize_kernel<float>, dim3(qb), dim3(256), 0, s, static_cast<const float*>(x),
+                           n4, C, amax, reinterpret_cast<uint32_t*>(q));
+        break;
+      case HSL2_BF16:
+        hipLaunchKernelGGL(fp8_amax_kernel<__hip_bfloat16>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                           static_cast<const __hip_bfloat16*>(x), vectors, C, chunk, amax);
+        hipLaunchKernelGGL(fp8_quantize_kernel<__hip_bfloat16>, dim3(qb), dim3(256), 0, s,
+                           static_cast<const __hip_bfloat16*>(x), n4, C, amax, reinterpret_cast<uint32_t*>(q));
+        break;
+      default: set_error("fp8_quantize: unsupported dtype (f32 or bf16)"); return HSL2_ERR_ARGUMENT;
+    }
+    HSL2_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(fp8_scales_kernel, dim3((C + 255) / 256), dim3(256), 0, s, amax, C);
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+extern "C" int hsl2_altcorr_forward_fp8(const uint8_t* fmap1, const float* scale1, const uint8_t* fmap2,
+                                        const float* scale2, const float* coords, const int64_t* ii,
+                                        const int64_t* jj, void* corr, int out_dtype, int64_t B, int64_t N1,
+                                        int64_t N2, int64_t C, int64_t H, int64_t W, int64_t H2, int64_t W2,
+                                        int64_t M, int radius, void* stream) {
+  int st = check_alt(fmap1, fmap2, coords, ii, jj, B, N1, N2, C, H, W, H2, W2, M, radius);
+  if (st) return st;
+  HSL2_REQUIRE(corr && scale1 && scale2, HSL2_ERR_ARGUMENT, "altcorr fp8: null pointer");
+  HSL2_REQUIRE(C % 16 == 0, HSL2_ERR_UNSUPPORTED, "altcorr fp8: channel count must be a multiple of 16");
+  const int HW = static_cast<int>(H * W);
+  const int tiles = (HW + kPix - 1) / kPix;
+  const int64_t blocks = B * M * tiles;
+  if (blocks == 0) return HSL2_OK;
+  const int D = 2 * radius + 2;
+  const size_t lds = sizeof(float) * kPix * (C + D * D);
+  HSL2_REQUIRE(lds <= 160 * 1024, HSL2_ERR_UNSUPPORTED, "altcorr fp8: LDS budget exceeded");
+  hipStream_t s = as_stream(stream);
+#define HSL2_ALT_FP8(To)                                                                              \
+  hipLaunchKernelGGL(altcorr_fwd_fp8_kernel<To>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), lds, \
+                     s, fmap1, scale1, fmap2, scale2, coords, ii, jj, static_cast<To*>(corr),              \
+                     static_cast<int>(N1), static_cast<int>(N2), static_cast<int>(C), HW,                  \
+                     static_cast<int>(W), static_cast<int>(H2), static_cast<int>(W2), static_cast<int>(M),  \
+                     radius, tiles)
+  switch (out_dtype) {
+    case HSL2_F32: HSL2_ALT_FP8(float); break;
+    case HSL2_BF16: HSL2_ALT_FP8(__hip_bfloat16); break;
+    default: set_error("altcorr fp8: output dtype must be f32 or bf16"); return HSL2_ERR_ARGUMENT;
+  }
+#undef HSL2_ALT_FP8
   HSL2_LAUNCH_CHECK();
   return HSL2_OK;
 }

@@ -337,6 +337,62 @@ std::vector<torch::Tensor> altcorr_backward(torch::Tensor fmap1, torch::Tensor f
   return {o1.to(fmap1.scalar_type()), o2.to(fmap2.scalar_type())};
 }
 
+// ---- fp8 feature maps for the on-the-fly correlation (additions of this build) -------------
+// fmap [B, N, C, H, W] (f32 or bf16) -> [q uint8 [B, N, H, W, C] (e4m3fn bits, channel-last),
+// scales f32 [C]]
+std::vector<torch::Tensor> fp8_quantize(torch::Tensor fmap) {
+  CHECK_CONTIGUOUS(fmap);
+  require_device(fmap, "fmap");
+  require(fmap.dim() == 5, "fp8_quantize: fmap must be [B, N, C, H, W]");
+  require(fmap.scalar_type() == torch::kFloat32 || fmap.scalar_type() == torch::kBFloat16,
+          "fp8_quantize: fmap must be float32 or bfloat16");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(fmap.device());
+  const int64_t B = fmap.size(0), N = fmap.size(1), C = fmap.size(2), H = fmap.size(3), W = fmap.size(4);
+  torch::Tensor cl = torch::empty_like(fmap);
+  raise_status(hsl2_channels_last(fmap.data_ptr(), cl.data_ptr(), dtype_code(fmap), B * N, C, H * W, current_stream()),
+               "fp8_quantize");
+  torch::Tensor q = torch::empty({B, N, H, W, C}, fmap.options().dtype(torch::kUInt8));
+  torch::Tensor sc = torch::empty({C}, fmap.options().dtype(torch::kFloat32));
+  raise_status(hsl2_fp8_quantize(cl.data_ptr(), dtype_code(fmap), B * N * H * W, C, q.data_ptr<uint8_t>(),
+                                 sc.data_ptr<float>(), current_stream()),
+               "fp8_quantize");
+  return {q, sc};
+}
+
+// q1 [B, N1, H, W, C], q2 [B, N2, H2, W2, C] uint8 (fp8_quantize output), scales [C] f32,
+// coords [B, M, 2, H, W] f32 -> corr [B, M, 2r+1, 2r+1, H, W] (bf16, or f32 when out_float)
+std::vector<torch::Tensor> altcorr_forward_fp8(torch::Tensor q1, torch::Tensor s1, torch::Tensor q2,
+                                               torch::Tensor s2, torch::Tensor coords, torch::Tensor ii,
+                                               torch::Tensor jj, int radius, bool out_float) {
+  for (auto* t : {&q1, &s1, &q2, &s2, &coords, &ii, &jj}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, "altcorr_forward_fp8 tensors");
+  }
+  require(q1.dim() == 5 && q2.dim() == 5 && q1.scalar_type() == torch::kUInt8 && q2.scalar_type() == torch::kUInt8,
+          "altcorr_forward_fp8: q1, q2 must be uint8 [B, N, H, W, C]");
+  const int64_t B = q1.size(0), N1 = q1.size(1), H = q1.size(2), W = q1.size(3), C = q1.size(4);
+  const int64_t N2 = q2.size(1), H2 = q2.size(2), W2 = q2.size(3);
+  require(q2.size(0) == B && q2.size(4) == C, "altcorr_forward_fp8: inconsistent fmaps");
+  require(s1.numel() == C && s2.numel() == C && s1.scalar_type() == torch::kFloat32 &&
+              s2.scalar_type() == torch::kFloat32,
+          "altcorr_forward_fp8: scales must be float32 [C]");
+  require(coords.dim() == 5 && coords.size(0) == B && coords.size(2) == 2 && coords.size(3) == H &&
+              coords.size(4) == W && coords.scalar_type() == torch::kFloat32,
+          "altcorr_forward_fp8: coords must be float32 [B, M, 2, H, W]");
+  const int64_t M = coords.size(1);
+  require(ii.numel() == M && jj.numel() == M && ii.scalar_type() == torch::kInt64 && jj.scalar_type() == torch::kInt64,
+          "altcorr_forward_fp8: ii, jj must be int64 with one entry per edge");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(q1.device());
+  auto odt = out_float ? torch::kFloat32 : torch::kBFloat16;
+  torch::Tensor corr = torch::empty({B, M, 2 * radius + 1, 2 * radius + 1, H, W}, coords.options().dtype(odt));
+  raise_status(hsl2_altcorr_forward_fp8(q1.data_ptr<uint8_t>(), s1.data_ptr<float>(), q2.data_ptr<uint8_t>(),
+                                        s2.data_ptr<float>(), coords.data_ptr<float>(), ii.data_ptr<int64_t>(),
+                                        jj.data_ptr<int64_t>(), corr.data_ptr(), out_float ? HSL2_F32 : HSL2_BF16,
+                                        B, N1, N2, C, H, W, H2, W2, M, radius, current_stream()),
+               "altcorr_forward_fp8");
+  return {corr};
+}
+
 // ---- correlation build (hislam2/modules/corr.py:24-38, 63-71) ---------------------------
 std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, int num_levels,
                                       py::object out_dtype, bool tiled) {
@@ -581,6 +637,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
         py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none(),
         py::arg("tiled") = false);
+  m.def("fp8_quantize", &fp8_quantize, "per-channel-scaled e4m3fn quantisation: [q (channel-last uint8), scales]");
+  m.def("altcorr_forward_fp8", &altcorr_forward_fp8, "altcorr lookup on fp8 feature maps", py::arg("q1"),
+        py::arg("s1"), py::arg("q2"), py::arg("s2"), py::arg("coords"), py::arg("ii"), py::arg("jj"),
+        py::arg("radius"), py::arg("out_float") = false);
   m.def("projective_transform", &projective_transform,
         "torch-formulation projective transform: [coords, valid, Ji, Jj, Jz]", py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("ii"), py::arg("jj"), py::arg("jacobian") = false,

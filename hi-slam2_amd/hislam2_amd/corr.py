@@ -150,12 +150,28 @@ class CorrLayer(torch.autograd.Function):
 
 
 class AltCorrBlock:
-    """corr.py:89-117."""
+    """corr.py:89-117.
 
-    def __init__(self, fmaps, num_levels=4, radius=3):
+    ``dtype=torch.float8_e4m3fn`` (an addition of this build for high-resolution frames) keeps
+    every pyramid level as e4m3fn codes with per-channel FP32 scales (``fp8_quantize``, the
+    levels pooled in FP32 first) and runs the lookup on them (``altcorr_forward_fp8``); the
+    output is bf16.  That path has no autograd."""
+
+    def __init__(self, fmaps, num_levels=4, radius=3, dtype=None):
         self.num_levels = num_levels
         self.radius = radius
+        self.fp8 = dtype == torch.float8_e4m3fn
         B, N, C, H, W = fmaps.shape
+        if self.fp8:
+            if torch.is_grad_enabled() and fmaps.requires_grad:
+                raise ValueError('AltCorrBlock: the fp8 path has no autograd')
+            f = fmaps.float().reshape(B * N, C, H, W)
+            self.levels = []
+            for i in range(self.num_levels):
+                lvl = f.reshape(B, N, C, H // 2 ** i, W // 2 ** i).contiguous()
+                self.levels.append(_db.fp8_quantize(lvl))
+                f = F.avg_pool2d(f, 2, stride=2)
+            return
         fmaps = fmaps.view(B * N, C, H, W)
         self.pyramid = []
         for i in range(self.num_levels):
@@ -167,8 +183,13 @@ class AltCorrBlock:
         coords = coords.permute(0, 1, 4, 2, 3).contiguous()
         corr_list = []
         for i in range(self.num_levels):
-            corr = CorrLayer.apply(self.pyramid[0], self.pyramid[i].contiguous(),
-                                   coords / 2 ** i, ii, jj, self.radius)
+            if self.fp8:
+                (q1, s1), (q2, s2) = self.levels[0], self.levels[i]
+                corr, = _db.altcorr_forward_fp8(q1, s1, q2, s2, (coords / 2 ** i).contiguous(),
+                                                ii.contiguous(), jj.contiguous(), self.radius)
+            else:
+                corr = CorrLayer.apply(self.pyramid[0], self.pyramid[i].contiguous(),
+                                       coords / 2 ** i, ii, jj, self.radius)
             corr_list.append(corr.flatten(2, 3))
         corr = torch.stack(corr_list, dim=2).flatten(2, 3)
         return corr

@@ -129,6 +129,21 @@ int hsl2_altcorr_backward(const void* fmap1, const void* fmap2, const float* coo
                           int64_t B, int64_t N1, int64_t N2, int64_t C, int64_t H, int64_t W,
                           int64_t H2, int64_t W2, int64_t M, int radius, void* stream);
 
+/* FP8 (OCP e4m3fn) feature maps with per-channel FP32 scales, for the on-the-fly correlation
+ * of high-resolution frames (no counterpart in the reference, which runs altcorr in the fmap
+ * dtype).  hsl2_fp8_quantize: x is `vectors` channel-last vectors of `channels` values (F32 or
+ * BF16); scales[c] = max_v |x[v][c]| / 448 (1 for an all-zero channel) and q[v][c] =
+ * e4m3(x[v][c] / scales[c]), round to nearest even.  hsl2_altcorr_forward_fp8: the lookup of
+ * hsl2_altcorr_forward on such maps (channel-last, C a multiple of 16), dequantised on the fly
+ * (s1_c s2_c folded into fmap1); out_dtype F32 or BF16. */
+int hsl2_fp8_quantize(const void* x, int dtype, int64_t vectors, int64_t channels, uint8_t* q,
+                      float* scales, void* stream);
+int hsl2_altcorr_forward_fp8(const uint8_t* fmap1, const float* scale1, const uint8_t* fmap2,
+                             const float* scale2, const float* coords, const int64_t* ii,
+                             const int64_t* jj, void* corr, int out_dtype, int64_t B, int64_t N1,
+                             int64_t N2, int64_t C, int64_t H, int64_t W, int64_t H2, int64_t W2,
+                             int64_t M, int radius, void* stream);
+
 /* ---- dense bundle adjustment ------------------------------------------------- */
 
 typedef struct hsl2_ba_plan hsl2_ba_plan;

@@ -32,6 +32,19 @@ def ba_fixture():
                 out_poses=poses, out_disps=disps, out_dx=dx, out_dz=dz)
 
 
+def ba_hires64_fixture():
+    """First GN iteration of the 96x128 workload (64 keyframes, 800 edges): the inputs are
+    regenerated from synth.ba_hires64() (seeded), so only input checksums, dx and dz at 4096
+    seeded pixel positions are stored."""
+    p = synth.ba_hires64()
+    _, _, dx, dz = oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta,
+                          p.ii, p.jj, p.t0, p.t1, 1, p.lm, p.ep)
+    idx = np.random.default_rng(0).choice(dz.size, size=4096, replace=False)
+    return dict(checksums=np.array([p.poses.astype(np.float64).sum(), p.disps.astype(np.float64).sum(),
+                                    p.targets.astype(np.float64).sum(), p.weights.astype(np.float64).sum()]),
+                out_dx=dx, dz_index=idx, out_dz=dz.reshape(-1)[idx], dz_absmax=np.abs(dz).max())
+
+
 def corr_fixture():
     fm = synth.fmaps(12, 2, c=32, h=8, w=16)
     coords = synth.lookup_coords(13, 1, h=8, w=16)
@@ -43,6 +56,7 @@ def corr_fixture():
 def main():
     np.savez_compressed(os.path.join(HERE, 'ba_small_24x32.npz'), **ba_fixture())
     np.savez_compressed(os.path.join(HERE, 'corrblock_8x16.npz'), **corr_fixture())
+    np.savez_compressed(os.path.join(HERE, 'ba_hires64_first_iteration.npz'), **ba_hires64_fixture())
 
 
 if __name__ == '__main__':

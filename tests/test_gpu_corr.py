@@ -186,3 +186,98 @@ def test_altcorr_block_equals_corrblock(db):
     assert alt.shape == ref.shape == (1, 3, 196, 16, 32)
     e = rel_err(alt.double().cpu().numpy(), ref.double().cpu().numpy())
     assert e <= 1e-5, e
+
+
+def test_corrblock_backward_to_fmaps_bf16(db):
+    """The differentiation workload's correlation part: a 4-level radius-3 CorrBlock over
+    1x128x48x64 bf16 feature maps, backpropagated to fmap1 and fmap2 (corr_index_backward per
+    level, the pooling adjoint, the GEMM adjoints).  Reference: the float64 oracle chain on the
+    same bf16-rounded features.  Tolerance 2e-2 relative (max norm): the volume gradient is held
+    in the volume's dtype (bf16), as in the reference."""
+    from hislam2_amd import corr as hc
+    from hislam2_amd import synth
+    fm = torch.from_numpy(synth.fmaps(5, 2)).to(torch.bfloat16)
+    coords = synth.lookup_coords(6, 1)
+    G = np.random.default_rng(7).standard_normal((1, 1, 196, 48, 64))
+    f1 = fm[:, :1].cuda().requires_grad_()
+    f2 = fm[:, 1:].cuda().requires_grad_()
+    blk = hc.CorrBlock(f1, f2, dtype=torch.bfloat16)
+    out = blk(torch.from_numpy(coords).cuda())
+    g1, g2 = torch.autograd.grad((out.float() * torch.from_numpy(G).float().cuda()).sum(), [f1, f2])
+    # float64 oracle chain
+    a = fm[:, :1].double().numpy()
+    b = fm[:, 1:].double().numpy()
+    h, w, r = 48, 64, 3
+    c = np.moveaxis(coords.astype(np.float64), -1, 2).reshape(1, 2, h, w)
+    Gl = G.reshape(1, 4, 7, 7, h, w)
+    g0 = np.zeros((h * w, h, w))
+    for lvl in range(4):
+        shp = (1, h, w, h >> lvl, w >> lvl)
+        gv = oc.corr_index_backward(shp, c / 2 ** lvl, Gl[:, lvl], r)[0]       # [h, w, h_l, w_l]
+        up = np.repeat(np.repeat(gv, 2 ** lvl, axis=-2), 2 ** lvl, axis=-1) / 4 ** lvl
+        g0 += up.reshape(h * w, h, w)
+    g0 = g0.reshape(h * w, h * w)
+    A = a.reshape(128, h * w)
+    Bm = b.reshape(128, h * w)
+    r1 = (Bm @ g0.T / 16.0).reshape(a.shape)
+    r2 = (A @ g0 / 16.0).reshape(b.shape)
+    e = rel_err(g1.double().cpu().numpy(), r1); assert e <= 2e-2, e
+    e = rel_err(g2.double().cpu().numpy(), r2); assert e <= 2e-2, e
+
+
+def test_fp8_quantize_codes_and_rounding(db):
+    """e4m3fn round trip: every finite code is reproduced when its channel's max is 448 (scale
+    1), and random data quantises bit-exactly like torch's float8_e4m3fn cast of x / scale."""
+    codes = torch.arange(256, dtype=torch.uint8)
+    vals = codes.view(torch.float8_e4m3fn).float()
+    finite = torch.isfinite(vals) & (codes != 0x80)            # -0 quantises to +0
+    v = vals[finite]
+    x = v[None, None, None, None, :].repeat(1, 1, 16, 1, 1).contiguous()     # [1, 1, C=16, 1, n]
+    q, s = db.fp8_quantize(x.cuda())
+    assert torch.equal(s.cpu(), torch.ones(16))
+    assert torch.equal(q.cpu()[0, 0, 0, :, 0], codes[finite])
+    rng = np.random.default_rng(60)
+    x = torch.as_tensor(rng.standard_normal((2, 3, 32, 6, 10)) * rng.uniform(0.1, 5, (1, 1, 32, 1, 1)),
+                        dtype=torch.float32)
+    q, s = db.fp8_quantize(x.cuda())
+    sref = x.abs().amax(dim=(0, 1, 3, 4)) / 448.0
+    assert torch.allclose(s.cpu(), sref, rtol=1e-7, atol=0)
+    ref = (x.permute(0, 1, 3, 4, 2) / s.cpu()).to(torch.float8_e4m3fn).view(torch.uint8)
+    assert torch.equal(q.cpu(), ref)
+
+
+def test_altcorr_fp8_matches_oracle_on_dequantized_maps(db):
+    rng = np.random.default_rng(61)
+    B, N, C, H, W, r = 1, 3, 64, 8, 12, 4
+    f1 = torch.as_tensor(rng.standard_normal((B, N, C, H, W)), dtype=torch.float32)
+    f2 = torch.as_tensor(rng.standard_normal((B, N, C, H // 2, W // 2)), dtype=torch.float32)
+    ii, jj = torch.tensor([0, 1, 2, 0]), torch.tensor([1, 2, 0, 0])
+    coords = torch.as_tensor(rng.uniform(-3, 9, (B, 4, 2, H, W)), dtype=torch.float32)
+    q1, s1 = db.fp8_quantize(f1.cuda())
+    q2, s2 = db.fp8_quantize(f2.cuda())
+    out, = db.altcorr_forward_fp8(q1, s1, q2, s2, coords.cuda(), ii.cuda(), jj.cuda(), r, True)
+    deq = lambda q, s: (q.cpu().view(torch.float8_e4m3fn).double() * s.cpu().double()).permute(0, 1, 4, 2, 3)
+    ref = oc.altcorr_forward(deq(q1, s1).numpy(), deq(q2, s2).numpy(), coords.double().numpy(),
+                             ii.numpy(), jj.numpy(), r)
+    e = rel_err(out.double().cpu().numpy(), ref)
+    assert e <= 1e-5, e
+    out_b, = db.altcorr_forward_fp8(q1, s1, q2, s2, coords.cuda(), ii.cuda(), jj.cuda(), r)
+    assert out_b.dtype == torch.bfloat16 and rel_err(out_b.double().cpu().numpy(), ref) <= 1e-2
+
+
+def test_altcorr_block_hires_r4_bf16_and_fp8(db):
+    """The on-the-fly correlation workload: 96x128 feature maps, 4 levels, radius 4, bf16 and
+    fp8-e4m3 with per-channel scales, two edges against the float64 oracle (on the bf16-rounded
+    maps).  bf16: 1e-2 relative (max norm); fp8: 8e-2 (3 mantissa bits)."""
+    from hislam2_amd import corr as hc
+    from hislam2_amd import synth
+    fm = torch.from_numpy(synth.fmaps(62, 3, h=96, w=128)).to(torch.bfloat16)
+    ii, jj = torch.tensor([0, 2]), torch.tensor([1, 0])
+    coords = torch.from_numpy(synth.lookup_coords(63, 2, h=96, w=128))
+    ref = oc.altcorr_block(fm.double().numpy(), coords.double().numpy(), ii.numpy(), jj.numpy(), r=4)
+    for dtype, tol in ((torch.bfloat16, 1e-2), (torch.float8_e4m3fn, 8e-2)):
+        out = hc.AltCorrBlock(fm.cuda(), radius=4, dtype=dtype)(coords.cuda(), ii.cuda(), jj.cuda())
+        assert out.shape == (1, 2, 4 * 81, 96, 128)
+        e = rel_err(out.double().cpu().numpy(), ref)
+        print('altcorr hires r=4', dtype, 'max-norm relative error', e)
+        assert e <= tol, (dtype, e)

@@ -121,7 +121,7 @@ def test_two_step_gradients_match_oracle_autograd_and_differences():
             d['eta'].double().requires_grad_()]
     Lr = _loss_two_steps(tb.BA, d, *r_in, d['poses'].double(), d['disps'].double(), d['intr'].double(), ii, jj)
     gref = torch.autograd.grad(Lr, r_in)
-    assert abs(float(L) - float(Lr)) <= 1e-4 * abs(float(Lr))
+    assert abs(float(L.detach()) - float(Lr.detach())) <= 1e-4 * abs(float(Lr.detach()))
     for name, g, gr in zip(['targets', 'weights', 'eta'], grads, gref):
         e = _rel(g, gr)
         assert e <= 2e-3, (name, e)
