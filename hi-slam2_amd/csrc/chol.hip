@@ -6,9 +6,10 @@
 // right-looking tiled algorithm over 64x64 tiles:
 //
 //   for each tile column k:
-//     chol_panel   one block per tile row i >= k: every block factors the diagonal tile in
-//                  registers and inverts the factor (redundantly, so no inter-block wait is
-//                  needed); block 0 stores L_kk^-1 in a separate diagonal buffer (A_kk is
+//     chol_panel   one block per tile row i >= k: in every block one wave factors the
+//                  diagonal tile in registers (readlane broadcasts, no barrier per column)
+//                  while the other waves stage A_ik, then all waves invert the factor
+//                  (redundantly in every block, so no inter-block wait is needed); block 0 stores L_kk^-1 in a separate diagonal buffer (A_kk is
 //                  still being read by the other blocks) and y_k = L_kk^-1 b_k, the others
 //                  store L_ik = A_ik (L_kk^-1)^T (a 64x64x64 product, no substitution)
 //     chol_update  one block per lower trailing tile (i, j), k < j <= i: A_ij -= L_ik L_jk^T,
@@ -115,88 +116,59 @@ __device__ __forceinline__ void inverse_level(R (*Lw)[kLd], R (*Xi)[kLd], R* tmp
   __syncthreads();
 }
 
-// Factor the diagonal tile (k, k) in registers, then invert the factor.  On return Xi
-// (LDS) holds L_kk^-1 (lower triangular, zeros above) and Lw holds L_kk (lower part).
-// A non-positive pivot sets info[0] = 1 when `report` and is replaced by 1 so that the
-// arithmetic stays finite.
-//
-// Factorisation: right-looking, ONE barrier per column.  The owners of column c publish
-// it unscaled; every thread then takes the pivot from the published column and applies
-// a[r][s] -= a[r][c] a[s][c] / pivot, while the owners scale their column by 1/sqrt(pivot).
-// Inverse: 4x4 diagonal blocks inverted in registers, then recursive doubling
-// inv([[L11, 0], [L21, L22]]) = [[X11, 0], [-X22 L21 X11, X22]] over block sizes 4..32,
-// i.e. two small products and two barriers per level instead of 64 dependent steps.
+// Broadcast lane `lane`'s value to the whole wave (v_readlane into SGPRs; `lane` must be
+// wave-uniform -- here always a compile-time constant after unrolling).
+template <typename R> __device__ __forceinline__ R lane_bcast(R v, int lane);
+template <> __device__ __forceinline__ double lane_bcast<double>(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <> __device__ __forceinline__ float lane_bcast<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Factor the diagonal tile (k, k) with ONE wave: lane r holds row r in registers; column c of
+// the right-looking factorisation is broadcast from the lanes that hold it (v_readlane), so
+// the 64 dependent column steps need no barrier and no LDS round trip.  Writes L_kk (zeros
+// above the diagonal) into Lw.  A non-positive pivot sets info[0] = 1 when `report` and is
+// replaced by 1 so that the arithmetic stays finite.
 template <typename R>
-__device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R (*Lw)[kLd],
-                                  R (*Xi)[kLd], bool report, int* __restrict__ info) {
-  __shared__ R colc[2][kTile];
+__device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, R (*Lw)[kLd], bool report,
+                            int* __restrict__ info) {
+  const int r = threadIdx.x;                     // 0..63
+  const int64_t base = (int64_t)k * kTile;
+  R a[kTile];
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) a[c] = A[(base + r) + (base + c) * lda];
+  bool bad = false;
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) {
+    R piv = lane_bcast<R>(a[c], c);
+    if (!(piv > R(0))) {
+      bad = true;
+      piv = R(1);
+    }
+    const R is = fast_rsqrt(piv);
+    const R t = a[c] * (is * is);                // a[r][c] / pivot
+#pragma unroll
+    for (int s = c + 1; s < kTile; ++s) a[s] -= t * lane_bcast<R>(a[c], s);
+    a[c] = (r == c) ? piv * is : a[c] * is;      // L[r][c] (above the diagonal: unused)
+  }
+  if (bad && report && r == 0) info[0] = 1;
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) Lw[r][c] = c <= r ? a[c] : R(0);
+}
+
+// Invert the factor in Lw (all 256 threads; Xi must be zero): 4x4 diagonal blocks in
+// registers, then recursive doubling inv([[L11, 0], [L21, L22]]) = [[X11, 0], [-X22 L21 X11, X22]]
+// over block sizes 4..32, i.e. two small products and two barriers per level.
+template <typename R>
+__device__ void invert_factor(R (*Lw)[kLd], R (*Xi)[kLd]) {
   __shared__ R tmp[32 * 32];
   const int t = threadIdx.x;
   const int tr = t >> 4, ts = t & 15;
-  const int64_t base = (int64_t)k * kTile;
-  R a[4][4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int r = 4 * tr + p, c = 4 * ts + q;
-      a[p][q] = r >= c ? A[(base + r) + (base + c) * lda] : R(0);
-      Xi[r][c] = R(0);
-    }
-  for (int c = 0; c < kTile; ++c) {
-    const int buf = c & 1;
-    const bool owner = ts == (c >> 2);
-    if (owner) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (q != (c & 3)) continue;
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-          if (4 * tr + p >= c) colc[buf][4 * tr + p] = a[p][q];
-      }
-    }
-    __syncthreads();
-    R piv = colc[buf][c];
-    if (!(piv > R(0))) {
-      if (report && t == 0) info[0] = 1;
-      piv = R(1);
-    }
-    if (owner) {
-      const R invd = fast_rsqrt(piv);
-      const R d = piv * invd;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (q != (c & 3)) continue;
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          const int r = 4 * tr + p;
-          if (r > c) {
-            a[p][q] *= invd;
-            Lw[r][c] = a[p][q];
-          } else if (r == c) {
-            a[p][q] = d;
-            Lw[c][c] = d;
-          }
-        }
-      }
-    }
-    const R ip = fast_rcp(piv);
-    R cr[4], cs[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      cr[p] = colc[buf][4 * tr + p] * ip;
-      cs[p] = colc[buf][4 * ts + p];
-    }
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = 4 * tr + p, sc = 4 * ts + q;
-        if (sc > c && r >= sc) a[p][q] -= cr[p] * cs[q];
-      }
-  }
-  __syncthreads();            // Lw complete
-  // 4x4 diagonal blocks of the inverse, in registers
   if (tr == ts) {
     R l[4][4], x[4][4];
 #pragma unroll
@@ -221,7 +193,6 @@ __device__ void factor_and_invert(const R* __restrict__ A, int64_t lda, int k, R
       for (int q = 0; q <= p; ++q) Xi[4 * tr + p][4 * tr + q] = x[p][q];
   }
   __syncthreads();
-  // recursive doubling: blocks of size 2m from blocks of size m
   inverse_level<R, 2>(Lw, Xi, tmp);
   inverse_level<R, 3>(Lw, Xi, tmp);
   inverse_level<R, 4>(Lw, Xi, tmp);
@@ -232,13 +203,36 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
            int* __restrict__ info) {
-  __shared__ R Lw[kTile][kLd];     // L_kk, then A_ik
+  __shared__ R Lw[kTile][kLd];     // L_kk
   __shared__ R Xi[kTile][kLd];     // L_kk^-1
+  __shared__ R Ab[kTile][kLd];     // A_ik
   __shared__ R part[4][kTile];
   const int t = threadIdx.x;
   const int i = k + blockIdx.x;
-  factor_and_invert<R>(A, lda, k, Lw, Xi, blockIdx.x == 0, info);
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
+  // wave 0 factors the diagonal tile; meanwhile waves 1..3 clear Xi and stage A_ik
+  if (t < kWave) {
+    factor_wave<R>(A, lda, k, Lw, blockIdx.x == 0, info);
+  } else {
+    const int u = t - kWave;                     // 0..191
+    constexpr int kIt = (kTile * kTile + 191) / 192;
+    R v[kIt];
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int idx = u + 192 * it, c = idx >> 6, r = idx & 63;
+      v[it] = (blockIdx.x > 0 && idx < kTile * kTile) ? A[(bi + r) + (bk + c) * lda] : R(0);
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int idx = u + 192 * it, c = idx >> 6, r = idx & 63;
+      if (idx < kTile * kTile) {
+        Xi[r][c] = R(0);
+        Ab[r][c] = v[it];
+      }
+    }
+  }
+  __syncthreads();
+  invert_factor<R>(Lw, Xi);
   if (blockIdx.x == 0) {
     // the diagonal factor is kept as its inverse (column-major) for the back-substitution
     R* Lk = Ld + (int64_t)k * kTile * kTile;
@@ -256,9 +250,7 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
     if (t < kTile) b[bk + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
     return;
   }
-  // L_ik = A_ik (L_kk^-1)^T, with A_ik staged in Lw (L_kk is no longer needed)
-  tile_to_lds<R>(A + bi + bk * lda, lda, Lw);
-  __syncthreads();
+  // L_ik = A_ik (L_kk^-1)^T
   const int tr = t >> 4, ts = t & 15;
   R acc[4][4];
 #pragma unroll
@@ -270,7 +262,7 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
     R av[4], lv[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-      av[p] = Lw[4 * tr + p][m];
+      av[p] = Ab[4 * tr + p][m];
       lv[p] = Xi[4 * ts + p][m];
     }
 #pragma unroll
