@@ -122,8 +122,16 @@ def main():
 
     def time_path(dtype, precision):
         f = fm32.to(dtype)
-        blk = hcorr.CorrBlock(f[:, p.ii], f[:, p.jj], dtype=dtype)   # volume build (untimed)
+        f1, f2 = f[:, p.ii], f[:, p.jj]
         del f
+        torch.cuda.synchronize()
+        b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b0.record()
+        blk = hcorr.CorrBlock(f1, f2, dtype=dtype)       # volume build: timed on its own
+        b1.record()
+        torch.cuda.synchronize()
+        build_ms = b0.elapsed_time(b1)
+        del f1, f2
         poses, disps = poses0.clone(), disps0.clone()
         stream = torch.cuda.current_stream()
         ev = []
@@ -159,7 +167,7 @@ def main():
         t_ba = time.perf_counter() - t1
         del blk
         torch.cuda.empty_cache()
-        return dict(t_total=t_total, t_ba=t_ba, lookup_ms=float(np.mean(lookup_ms)))
+        return dict(t_total=t_total, t_ba=t_ba, lookup_ms=float(np.mean(lookup_ms)), build_ms=build_ms)
 
     log('timing split path (bf16 volume, FP64 solve)')
     split = time_path(torch.bfloat16, 0)
@@ -170,6 +178,9 @@ def main():
 
     K = args.steps
     value = iters * K / split['t_total']
+    # corr build, per edge: 2*HW*C*2 B of bf16 feature maps + HW^2 * (1 + 1/4 + 1/16 + 1/64) * 2 B
+    HWp = h * w
+    build_bytes = E * (2 * HWp * fm32.shape[2] * 2 + HWp * HWp * (1 + 0.25 + 0.0625 + 0.015625) * 2)
     lookup_s = split['lookup_ms'] / 1e3
     achieved = LOOKUP_BYTES_BF16 * lookups / lookup_s / 1e9
     traffic = lookup_traffic()
@@ -197,6 +208,8 @@ def main():
                    'edges': E, 'keyframes': int(p.poses.shape[0]), 'iterations_per_call': iters},
         'ba_only_it_s': round(iters * K / split['t_ba'], 3),
         'lookup_ms': round(split['lookup_ms'], 4),
+        'corr_build_ms': round(split['build_ms'], 3),
+        'corr_build_GBps_algorithmic': round(build_bytes / (split['build_ms'] / 1e3) / 1e9, 1),
         'lookup_GBps_algorithmic': round(achieved, 1),
         'reproj_rms_start_px': round(rms0, 6),
         'reproj_rms_split_px': round(res['split']['rms'], 6),

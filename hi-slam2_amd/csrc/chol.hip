@@ -7,11 +7,12 @@
 //
 //   for each tile column k:
 //     chol_panel   one block per tile row i >= k: in every block one wave factors the
-//                  diagonal tile in registers (readlane broadcasts, no barrier per column)
-//                  while the other waves stage A_ik, then all waves invert the factor
-//                  (redundantly in every block, so no inter-block wait is needed); block 0 stores L_kk^-1 in a separate diagonal buffer (A_kk is
-//                  still being read by the other blocks) and y_k = L_kk^-1 b_k, the others
-//                  store L_ik = A_ik (L_kk^-1)^T (a 64x64x64 product, no substitution)
+//                  diagonal tile in registers (readlane broadcasts, no barrier per column;
+//                  redundantly in every block, so no inter-block wait is needed) and publishes
+//                  each finished column of L to LDS; a second wave consumes the columns as
+//                  they arrive: block 0 forms L_kk^-1 (kept in a separate diagonal buffer, A_kk
+//                  is still being read by the other blocks) and y_k = L_kk^-1 b_k, the others
+//                  solve L_ik L_kk^T = A_ik row by row
 //     chol_update  one block per lower trailing tile (i, j), k < j <= i: A_ij -= L_ik L_jk^T,
 //                  plus one block per tile row i > k: b_i -= L_ik y_k
 //   for each tile row k from the bottom:
@@ -21,6 +22,8 @@
 // A pivot that is not positive (or NaN) sets info = 1, as Eigen's LLT reports a
 // NumericalIssue; the caller then treats the solution as zero.  Templated on the element
 // type: double for the split-precision path, float for the plain FP32 path.
+#include <utility>
+
 #include "common.hpp"
 
 namespace hsl2 {
@@ -76,46 +79,6 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
   return y * (1.5f - 0.5f * x * y * y);
 }
 
-// One level of the recursive-doubling inverse (block size m = 2^LG -> 2m).
-template <typename R, int LG>
-__device__ __forceinline__ void inverse_level(R (*Lw)[kLd], R (*Xi)[kLd], R* tmp) {
-  constexpr int m = 1 << LG;
-  constexpr int total = 32 * m;                  // (64 / 2m) blocks of m x m
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int e0 = 0; e0 < total; e0 += 256) {
-    const int e = e0 + t;
-    if (total >= 256 || e < total) {
-      const int b = e >> (2 * LG), i = (e >> LG) & (m - 1), j = e & (m - 1);
-      const int o = 2 * m * b;
-      R v0 = 0, v1 = 0;                          // (L21 X11)[i][j]; X11 lower, so kk >= j
-#pragma unroll
-      for (int kk = 0; kk < m; kk += 2) {
-        v0 += kk >= j ? Lw[o + m + i][o + kk] * Xi[o + kk][o + j] : R(0);
-        v1 += kk + 1 >= j ? Lw[o + m + i][o + kk + 1] * Xi[o + kk + 1][o + j] : R(0);
-      }
-      tmp[e] = v0 + v1;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int e0 = 0; e0 < total; e0 += 256) {
-    const int e = e0 + t;
-    if (total >= 256 || e < total) {
-      const int b = e >> (2 * LG), i = (e >> LG) & (m - 1), j = e & (m - 1);
-      const int o = 2 * m * b;
-      R v0 = 0, v1 = 0;                          // -(X22 T)[i][j]; X22 lower, so kk <= i
-#pragma unroll
-      for (int kk = 0; kk < m; kk += 2) {
-        v0 += kk <= i ? Xi[o + m + i][o + m + kk] * tmp[(b << (2 * LG)) + (kk << LG) + j] : R(0);
-        v1 += kk + 1 <= i ? Xi[o + m + i][o + m + kk + 1] * tmp[(b << (2 * LG)) + ((kk + 1) << LG) + j] : R(0);
-      }
-      Xi[o + m + i][o + j] = -(v0 + v1);
-    }
-  }
-  __syncthreads();
-}
-
 // Broadcast lane `lane`'s value to the whole wave (v_readlane into SGPRs; `lane` must be
 // wave-uniform -- here always a compile-time constant after unrolling).
 template <typename R> __device__ __forceinline__ R lane_bcast(R v, int lane);
@@ -129,13 +92,15 @@ template <> __device__ __forceinline__ float lane_bcast<float>(float v, int lane
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Factor the diagonal tile (k, k) with ONE wave: lane r holds row r in registers; column c of
-// the right-looking factorisation is broadcast from the lanes that hold it (v_readlane), so
-// the 64 dependent column steps need no barrier and no LDS round trip.  Writes L_kk (zeros
-// above the diagonal) into Lw.  A non-positive pivot sets info[0] = 1 when `report` and is
-// replaced by 1 so that the arithmetic stays finite.
+// Producer (wave 0 of every panel block): factor the diagonal tile with ONE wave.  Lane r
+// holds row r in registers; column c of the right-looking factorisation is broadcast from the
+// lanes that hold it (v_readlane), so the column steps need no data exchange through LDS.
+// After step c, column c of L is published to Lc[c][*] and the block barrier hands it to the
+// consumer wave, which takes exactly one barrier per column too (64 in all, on both waves)
+// and works one column behind, so the producer rarely waits.  A non-positive pivot sets
+// info[0] = 1 when `report` and is replaced by 1 so that the arithmetic stays finite.
 template <typename R>
-__device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, R (*Lw)[kLd], bool report,
+__device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, R (*Lc)[kLd], bool report,
                             int* __restrict__ info) {
   const int r = threadIdx.x;                     // 0..63
   const int64_t base = (int64_t)k * kTile;
@@ -154,126 +119,95 @@ __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, R (*Lw)
     const R t = a[c] * (is * is);                // a[r][c] / pivot
 #pragma unroll
     for (int s = c + 1; s < kTile; ++s) a[s] -= t * lane_bcast<R>(a[c], s);
-    a[c] = (r == c) ? piv * is : a[c] * is;      // L[r][c] (above the diagonal: unused)
+    Lc[c][r] = (r == c) ? piv * is : (r > c ? a[c] * is : R(0));   // column c of L
+    __syncthreads();
   }
   if (bad && report && r == 0) info[0] = 1;
-#pragma unroll
-  for (int c = 0; c < kTile; ++c) Lw[r][c] = c <= r ? a[c] : R(0);
 }
 
-// Invert the factor in Lw (all 256 threads; Xi must be zero): 4x4 diagonal blocks in
-// registers, then recursive doubling inv([[L11, 0], [L21, L22]]) = [[X11, 0], [-X22 L21 X11, X22]]
-// over block sizes 4..32, i.e. two small products and two barriers per level.
-template <typename R>
-__device__ void invert_factor(R (*Lw)[kLd], R (*Xi)[kLd]) {
-  __shared__ R tmp[32 * 32];
-  const int t = threadIdx.x;
-  const int tr = t >> 4, ts = t & 15;
-  if (tr == ts) {
-    R l[4][4], x[4][4];
+// x[j] -= xm * Lc[m][j] for j in (m, kTile): the LDS reads are fenced into groups of 8 so that
+// the scheduler does not hoist a whole column of loads (register pressure, spills).
+template <typename R, int M>
+__device__ __forceinline__ void column_update(R (&x)[kTile], R xm, const R* __restrict__ col) {
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+  for (int j0 = M + 1; j0 < kTile; j0 += 8) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) l[p][q] = q <= p ? Lw[4 * tr + p][4 * tr + q] : R(0);
-    R dinv[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dinv[i] = fast_rcp(l[i][i]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        R v = i == j ? R(1) : R(0);
-#pragma unroll
-        for (int m = 0; m < i; ++m) v -= l[i][m] * x[m][j];
-        x[i][j] = i >= j ? v * dinv[i] : R(0);
-      }
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-      for (int q = 0; q <= p; ++q) Xi[4 * tr + p][4 * tr + q] = x[p][q];
+    for (int j = j0; j < j0 + 8 && j < kTile; ++j) x[j] -= xm * col[j];
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __syncthreads();
-  inverse_level<R, 2>(Lw, Xi, tmp);
-  inverse_level<R, 3>(Lw, Xi, tmp);
-  inverse_level<R, 4>(Lw, Xi, tmp);
-  inverse_level<R, 5>(Lw, Xi, tmp);
 }
 
+template <typename R, int M>
+__device__ __forceinline__ void consume_column(R (&x)[kTile], R (*Lc)[kLd]) {
+  __syncthreads();                               // column M of L is published
+  x[M] *= fast_rcp(Lc[M][M]);
+  column_update<R, M>(x, x[M], Lc[M]);
+}
+
+template <typename R, int... Ms>
+__device__ __forceinline__ void consume_all(R (&x)[kTile], R (*Lc)[kLd], std::integer_sequence<int, Ms...>) {
+  (consume_column<R, Ms>(x, Lc), ...);
+}
+
+// Consumer, off-diagonal block: L_ik = A_ik L_kk^-T, i.e. x L_kk^T = a for each of the 64 rows
+// (lane q = row q), right-looking as the columns of L arrive: x_m = acc_m / L[m][m], then
+// acc_j -= x_m L[j][m] for j > m.
 template <typename R>
-__global__ void __launch_bounds__(256)
+__device__ void solve_rows(R* __restrict__ A, int64_t lda, int64_t bi, int64_t bk, R (*Lc)[kLd]) {
+  const int q = threadIdx.x - kWave;
+  R x[kTile];
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) x[c] = A[(bi + q) + (bk + c) * lda];
+  consume_all<R>(x, Lc, std::make_integer_sequence<int, kTile>{});
+#pragma unroll
+  for (int c = 0; c < kTile; ++c) A[(bi + q) + (bk + c) * lda] = x[c];
+}
+
+// Consumer, diagonal block: X = L_kk^-1 by columns (lane j = column j), right-looking as the
+// columns of L arrive; X goes to Xs (row-major) for the caller.
+template <typename R>
+__device__ void invert_columns(R (*Lc)[kLd], R (*Xs)[kLd]) {
+  const int j = threadIdx.x - kWave;
+  R x[kTile];
+#pragma unroll
+  for (int m = 0; m < kTile; ++m) x[m] = m == j ? R(1) : R(0);
+  consume_all<R>(x, Lc, std::make_integer_sequence<int, kTile>{});
+#pragma unroll
+  for (int m = 0; m < kTile; ++m) Xs[m][j] = x[m];
+}
+
+// One block per tile row i >= k.  Wave 0 factors the diagonal tile and publishes L column by
+// column; wave 1 consumes the columns as they arrive: block 0 inverts L_kk (kept in Ld for the
+// back-substitution, and y_k = L_kk^-1 b_k), every other block solves its tile L_ik.
+template <typename R>
+__global__ void __launch_bounds__(128)
 chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
            int* __restrict__ info) {
-  __shared__ R Lw[kTile][kLd];     // L_kk
-  __shared__ R Xi[kTile][kLd];     // L_kk^-1
-  __shared__ R Ab[kTile][kLd];     // A_ik
-  __shared__ R part[4][kTile];
+  __shared__ R Lc[kTile][kLd];     // Lc[c][r] = L_kk[r][c]
+  __shared__ R Xs[kTile][kLd];     // L_kk^-1 (block 0)
   const int t = threadIdx.x;
   const int i = k + blockIdx.x;
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
-  // wave 0 factors the diagonal tile; meanwhile waves 1..3 clear Xi and stage A_ik
+  // both waves take exactly kTile barriers below (one per column), wave-uniform
   if (t < kWave) {
-    factor_wave<R>(A, lda, k, Lw, blockIdx.x == 0, info);
-  } else {
-    const int u = t - kWave;                     // 0..191
-    constexpr int kIt = (kTile * kTile + 191) / 192;
-    R v[kIt];
-#pragma unroll
-    for (int it = 0; it < kIt; ++it) {
-      const int idx = u + 192 * it, c = idx >> 6, r = idx & 63;
-      v[it] = (blockIdx.x > 0 && idx < kTile * kTile) ? A[(bi + r) + (bk + c) * lda] : R(0);
-    }
-#pragma unroll
-    for (int it = 0; it < kIt; ++it) {
-      const int idx = u + 192 * it, c = idx >> 6, r = idx & 63;
-      if (idx < kTile * kTile) {
-        Xi[r][c] = R(0);
-        Ab[r][c] = v[it];
-      }
-    }
-  }
-  __syncthreads();
-  invert_factor<R>(Lw, Xi);
-  if (blockIdx.x == 0) {
-    // the diagonal factor is kept as its inverse (column-major) for the back-substitution
-    R* Lk = Ld + (int64_t)k * kTile * kTile;
-#pragma unroll
-    for (int it = 0; it < 16; ++it) {
-      const int idx = t + 256 * it, c = idx >> 6, r = idx & 63;
-      Lk[idx] = Xi[r][c];
-    }
-    // y_k = L_kk^-1 b_k
-    const int r = t & 63, cg = t >> 6;
-    R acc = 0;
-    for (int c = cg; c < kTile; c += 4) acc += Xi[r][c] * b[bk + c];
-    part[cg][r] = acc;
-    __syncthreads();
-    if (t < kTile) b[bk + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    factor_wave<R>(A, lda, k, Lc, blockIdx.x == 0, info);
     return;
   }
-  // L_ik = A_ik (L_kk^-1)^T
-  const int tr = t >> 4, ts = t & 15;
-  R acc[4][4];
-#pragma unroll
-  for (int p = 0; p < 4; ++p)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[p][q] = 0;
-#pragma unroll 8
-  for (int m = 0; m < kTile; ++m) {
-    R av[4], lv[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      av[p] = Ab[4 * tr + p][m];
-      lv[p] = Xi[4 * ts + p][m];
-    }
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[p][q] += av[p] * lv[q];
+  if (blockIdx.x > 0) {
+    solve_rows<R>(A, lda, bi, bk, Lc);
+    return;
   }
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int p = 0; p < 4; ++p) A[(bi + 4 * tr + p) + (bk + 4 * ts + q) * lda] = acc[p][q];
+  invert_columns<R>(Lc, Xs);
+  // wave 1 alone from here (its own LDS writes, in order): L_kk^-1 column-major into Ld, and
+  // y_k = L_kk^-1 b_k
+  const int u = t - kWave;
+  R* Lk = Ld + (int64_t)k * kTile * kTile;
+#pragma unroll 8
+  for (int c = 0; c < kTile; ++c) Lk[c * kTile + u] = Xs[u][c];
+  R acc = 0;
+#pragma unroll 8
+  for (int c = 0; c < kTile; ++c) acc += Xs[u][c] * b[bk + c];
+  b[bk + u] = acc;
 }
 
 template <typename R>
@@ -390,7 +324,7 @@ int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_
   HSL2_REQUIRE(n_pad % kTile == 0, HSL2_ERR_SHAPE, "cholesky: size must be a multiple of 64");
   const int T = static_cast<int>(n_pad / kTile);
   for (int k = 0; k < T; ++k) {
-    hipLaunchKernelGGL(chol_panel<R>, dim3(T - k), dim3(256), 0, s, A, n_pad, Ld, b, k, info);
+    hipLaunchKernelGGL(chol_panel<R>, dim3(T - k), dim3(128), 0, s, A, n_pad, Ld, b, k, info);
     HSL2_LAUNCH_CHECK();
     const int m = T - k - 1;
     if (m > 0) {

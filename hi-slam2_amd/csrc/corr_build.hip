@@ -6,12 +6,14 @@
 // over q's (h, w) grid.
 //
 // One 256-thread block computes a 32 (p) x 8 image rows x WT columns (q) tile with
-// v_mfma_f32_32x32x{8,8,2} (bf16 / f16 / f32 operands, f32 accumulation): the
-// operands are staged through LDS in 32-channel chunks in [pixel][channel] order
-// (padded rows), each wave owns four 32x32 output blocks.  The finished f32 tile is
-// staged in LDS and written as level 0 plus every pooled level it fully contains
-// (8 rows cover 2x2, 4x4 and 8x8 pooling), so the volume is written once and never
-// read back.
+// v_mfma_f32_32x32x{8,8,2} (bf16 / f16 / f32 operands, f32 accumulation).  The feature maps
+// arrive channel-last ([pixel][channel]), so every lane fetches its MFMA operand fragment
+// (4 consecutive channels of one pixel: 8 bytes; 1 channel for f32) straight from memory:
+// no LDS staging and no barrier in the channel loop.  Each wave owns four 32x32 output
+// blocks.  Blocks are ordered so that each XCD works through whole edges (the operands of
+// one edge stay in that XCD's L2).  The finished f32 tile is staged in LDS and written as
+// level 0 plus every pooled level it fully contains (8 rows cover 2x2, 4x4 and 8x8 pooling),
+// so the volume is written once and never read back.
 #include "common.hpp"
 
 namespace hsl2 {
@@ -20,7 +22,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTP = 32;       // p rows per block
 constexpr int kRows = 8;      // image-2 rows per block
-constexpr int kKC = 32;       // channels per LDS chunk
 
 typedef float acc16 __attribute__((ext_vector_type(16)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -28,25 +29,27 @@ typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mfma;
 template <> struct Mfma<__hip_bfloat16> {
-  static constexpr int K = 8;
-  static __device__ __forceinline__ acc16 run(const __hip_bfloat16* a, const __hip_bfloat16* b, acc16 c) {
-    const bf16x4 av = *reinterpret_cast<const bf16x4*>(a);
-    const bf16x4 bv = *reinterpret_cast<const bf16x4*>(b);
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, c, 0, 0, 0);
+  static constexpr int K = 8;                      // channels per instruction
+  typedef bf16x4 Frag;
+  static __device__ __forceinline__ Frag load(const __hip_bfloat16* p) { return *reinterpret_cast<const bf16x4*>(p); }
+  static __device__ __forceinline__ acc16 run(Frag a, Frag b, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
   }
 };
 template <> struct Mfma<__half> {
   static constexpr int K = 8;
-  static __device__ __forceinline__ acc16 run(const __half* a, const __half* b, acc16 c) {
-    const f16x4 av = *reinterpret_cast<const f16x4*>(a);
-    const f16x4 bv = *reinterpret_cast<const f16x4*>(b);
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
+  typedef f16x4 Frag;
+  static __device__ __forceinline__ Frag load(const __half* p) { return *reinterpret_cast<const f16x4*>(p); }
+  static __device__ __forceinline__ acc16 run(Frag a, Frag b, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, c, 0, 0, 0);
   }
 };
 template <> struct Mfma<float> {
   static constexpr int K = 2;
-  static __device__ __forceinline__ acc16 run(const float* a, const float* b, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(*a, *b, c, 0, 0, 0);
+  typedef float Frag;
+  static __device__ __forceinline__ Frag load(const float* p) { return *p; }
+  static __device__ __forceinline__ acc16 run(Frag a, Frag b, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
   }
 };
 
@@ -56,89 +59,56 @@ __device__ __forceinline__ int plane_offset(int y, int x, int w) {
   return (((y >> 3) * (w >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
 }
 
-// grid: (p tiles, q tiles = (h/8) * (w/WT), batch)
+// 1-D grid over (p tile, q tile, edge), p tile fastest; see the block order note above.
 template <typename Ti, typename To>
 __global__ void __launch_bounds__(kThreads)
 corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __restrict__ l0,
                   To* __restrict__ l1, To* __restrict__ l2, To* __restrict__ l3, int C, int h,
-                  int w, int WT, int levels, int tiled) {
-  constexpr int KP = kKC + 4;                      // padded LDS row (elements)
+                  int w, int WT, int levels, int tiled, int ptiles, int qtiles, int64_t nblocks) {
   extern __shared__ __align__(16) unsigned char smem[];
-  Ti* s1 = reinterpret_cast<Ti*>(smem);            // [kTP][KP]
-  Ti* s2 = s1 + kTP * KP;                          // [kRows*WT][KP]
-  float* sout = reinterpret_cast<float*>(smem);    // [kTP][kRows*WT] after the K loop
+  float* sout = reinterpret_cast<float*>(smem);    // [kTP][kRows*WT]
+
+  // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a contiguous range of
+  // logical blocks (whole edges), so an edge's feature maps are fetched into one L2 only.
+  const int64_t b = blockIdx.x;
+  const int64_t per = nblocks / 8;
+  const int64_t lb = b < per * 8 ? (b % 8) * per + b / 8 : b;
+  const int pt = static_cast<int>(lb % ptiles);
+  const int qt = static_cast<int>((lb / ptiles) % qtiles);
+  const int n = static_cast<int>(lb / ((int64_t)ptiles * qtiles));
 
   const int HW = h * w;
-  const int n = blockIdx.z;
-  const int p0 = blockIdx.x * kTP;
+  const int p0 = pt * kTP;
   const int qtiles_x = w / WT;
-  const int y0 = (blockIdx.y / qtiles_x) * kRows;
-  const int x0 = (blockIdx.y % qtiles_x) * WT;
+  const int y0 = (qt / qtiles_x) * kRows;
+  const int x0 = (qt % qtiles_x) * WT;
   const int NQ = kRows * WT;                       // columns of this tile
-  const Ti* a = f1 + (int64_t)n * C * HW;
-  const Ti* b = f2 + (int64_t)n * C * HW;
   const int t = threadIdx.x;
   const int wave = t / kWave, lane = t % kWave;
-  const int nblk = NQ / 32;                        // 32-wide column blocks
-  const int per_wave = nblk / 4;                   // NQ is a multiple of 128
+  const int per_wave = NQ / 32 / 4;                // NQ is a multiple of 128
+  constexpr int K = Mfma<Ti>::K;
+  // lane i of the 32-row MFMA operand; lanes 32-63 carry the second half of the K block
+  const int i = lane % 32, kg = lane / 32;
+  const Ti* ap = f1 + ((int64_t)n * HW + p0 + i) * C + kg * (K / 2);
+  const Ti* bp[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = (wave * per_wave + (k < per_wave ? k : 0)) * 32 + i;
+    const int qpix = (y0 + q / WT) * w + x0 + q % WT;
+    bp[k] = f2 + ((int64_t)n * HW + qpix) * C + kg * (K / 2);
+  }
 
   acc16 acc[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
-
-  constexpr int EPL = 16 / sizeof(Ti);           // elements per 16-byte load
-  struct alignas(16) Vec { Ti v[EPL]; };
-  const int qpr = WT / EPL;                        // 16-byte loads per image-2 row of the tile
-  const int nb_loads = kKC * NQ / EPL;             // B-tile loads per chunk (a multiple of 256)
-  for (int c0 = 0; c0 < C; c0 += kKC) {
-    // stage [pixel][channel] chunks with 16-byte loads along pixels; all of a thread's
-    // loads are issued before the transposing LDS stores
-    if (t < kKC * kTP / EPL) {
-      const int c = t / (kTP / EPL), pv = t % (kTP / EPL);
-      const Vec va = *reinterpret_cast<const Vec*>(a + (int64_t)(c0 + c) * HW + p0 + pv * EPL);
+#pragma unroll 4
+  for (int kk = 0; kk < C; kk += K) {
+    const typename Mfma<Ti>::Frag a = Mfma<Ti>::load(ap + kk);
 #pragma unroll
-      for (int e = 0; e < EPL; ++e) s1[(pv * EPL + e) * KP + c] = va.v[e];
-    }
-    for (int base = 0; base < nb_loads; base += 8 * kThreads) {
-      Vec vb[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int idx = base + u * kThreads + t;
-        if (idx < nb_loads) {
-          const int c = idx / (NQ / EPL), qv = idx % (NQ / EPL);
-          const int qy = y0 + qv / qpr, qx = x0 + (qv % qpr) * EPL;
-          vb[u] = *reinterpret_cast<const Vec*>(b + (int64_t)(c0 + c) * HW + qy * w + qx);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int idx = base + u * kThreads + t;
-        if (idx < nb_loads) {
-          const int c = idx / (NQ / EPL), qv = idx % (NQ / EPL);
-#pragma unroll
-          for (int e = 0; e < EPL; ++e) s2[(qv * EPL + e) * KP + c] = vb[u].v[e];
-        }
-      }
-    }
-    __syncthreads();
-    const int i = lane % 32;
-    const int kg = lane / 32;
-#pragma unroll
-    for (int kk = 0; kk < kKC; kk += Mfma<Ti>::K) {
-      // lanes 0-31 carry the first K/2 channels of the MFMA's K block, lanes 32-63 the rest
-      const int ko = kk + kg * (Mfma<Ti>::K / 2);
-      const Ti* ap = s1 + i * KP + ko;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (k < per_wave) {
-          const int q = (wave * per_wave + k) * 32 + i;
-          acc[k] = Mfma<Ti>::run(ap, s2 + q * KP + ko, acc[k]);
-        }
-      }
-    }
-    __syncthreads();
+    for (int k = 0; k < 4; ++k)
+      if (k < per_wave) acc[k] = Mfma<Ti>::run(a, Mfma<Ti>::load(bp[k] + kk), acc[k]);
   }
 
   // scatter the accumulators into the LDS tile [p][q] (scaled by 1/16)
@@ -204,19 +174,19 @@ int launch(const void* f1, const void* f2, void* const* levels, int64_t batch, i
   const int WT = w % 64 == 0 ? 64 : static_cast<int>(w);
   HSL2_REQUIRE((kRows * WT) % 128 == 0, HSL2_ERR_UNSUPPORTED,
                "corr build: width must be a multiple of 64 or 16..63 with 8*w % 128 == 0");
-  constexpr int KP = kKC + 4;
-  const size_t ops = sizeof(Ti) * (kTP + kRows * WT) * KP;
-  const size_t outb = sizeof(float) * kTP * kRows * WT;
-  const size_t lds = ops > outb ? ops : outb;
+  const size_t lds = sizeof(float) * kTP * kRows * WT;
   HSL2_REQUIRE(lds <= 160 * 1024, HSL2_ERR_UNSUPPORTED, "corr build: LDS budget exceeded");
-  dim3 grid(static_cast<unsigned>(h * w / kTP), static_cast<unsigned>((h / kRows) * (w / WT)),
-            static_cast<unsigned>(batch));
-  if (batch == 0) return HSL2_OK;
+  const int ptiles = static_cast<int>(h * w / kTP);
+  const int qtiles = static_cast<int>((h / kRows) * (w / WT));
+  const int64_t nblocks = (int64_t)ptiles * qtiles * batch;
+  if (nblocks == 0) return HSL2_OK;
+  HSL2_REQUIRE(nblocks < (1ll << 31), HSL2_ERR_SHAPE, "corr build: grid too large");
   To* l[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < L; ++i) l[i] = static_cast<To*>(levels[i]);
-  hipLaunchKernelGGL((corr_build_kernel<Ti, To>), grid, dim3(kThreads), lds, s,
+  hipLaunchKernelGGL((corr_build_kernel<Ti, To>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
                      static_cast<const Ti*>(f1), static_cast<const Ti*>(f2), l[0], l[1], l[2], l[3],
-                     static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), WT, L, tiled);
+                     static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), WT, L, tiled, ptiles,
+                     qtiles, nblocks);
   HSL2_LAUNCH_CHECK();
   return HSL2_OK;
 }
@@ -234,11 +204,10 @@ extern "C" int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtyp
   for (int i = 0; i < num_levels; ++i)
     HSL2_REQUIRE(levels[i], HSL2_ERR_ARGUMENT, "corr build: null level pointer");
   HSL2_REQUIRE(batch >= 0 && channels > 0 && h > 0 && w > 0, HSL2_ERR_SHAPE, "corr build: bad sizes");
-  HSL2_REQUIRE(channels % kKC == 0, HSL2_ERR_UNSUPPORTED, "corr build: channels must be a multiple of 32");
+  HSL2_REQUIRE(channels % 8 == 0, HSL2_ERR_UNSUPPORTED, "corr build: channels must be a multiple of 8");
   HSL2_REQUIRE(h % kRows == 0 && w % 8 == 0, HSL2_ERR_UNSUPPORTED,
                "corr build: h and w must be multiples of 8");
   HSL2_REQUIRE(h * w < (1ll << 31) && (h * w) % kTP == 0, HSL2_ERR_SHAPE, "corr build: bad plane size");
-  HSL2_REQUIRE(batch < 65536, HSL2_ERR_SHAPE, "corr build: batch too large for one launch");
   hipStream_t s = as_stream(stream);
   if (out_dtype == HSL2_BF16) {
     if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);
