@@ -20,36 +20,47 @@ namespace hsl2 {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTP = 32;       // p rows per block
+constexpr int kTP = 32;       // p rows per MFMA operand
+constexpr int kPH = 1;        // 32-row operands per block (2 shares every fmap2 fragment twice but
+                              // halves occupancy; measured no faster at 48x64, 128 channels)
 constexpr int kRows = 8;      // image-2 rows per block
 
 typedef float acc16 __attribute__((ext_vector_type(16)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
+// Operand fragments are fetched 16 bytes per lane.  The sum over channels is order-free, so
+// within a group of channels the lanes of the two MFMA k-halves take contiguous 16-byte
+// pieces: for bf16 / f16 a group is 16 channels, lane half kg loads [8 kg, 8 kg + 8) and feeds
+// its first 4 values to one MFMA and the other 4 to the next; for f32 a group is 8 channels,
+// lane half kg loads [4 kg, 4 kg + 4) and feeds one value to each of four MFMAs.  A and B use
+// the same assignment, so every channel is multiplied with its own partner exactly once.
 template <typename T> struct Mfma;
 template <> struct Mfma<__hip_bfloat16> {
-  static constexpr int K = 8;                      // channels per instruction
-  typedef bf16x4 Frag;
-  static __device__ __forceinline__ Frag load(const __hip_bfloat16* p) { return *reinterpret_cast<const bf16x4*>(p); }
-  static __device__ __forceinline__ acc16 run(Frag a, Frag b, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+  static constexpr int G = 16;                     // channels per 16-byte group
+  static constexpr int S = 2;                      // MFMAs per group
+  struct alignas(16) Frag { bf16x4 v[2]; };
+  static __device__ __forceinline__ Frag load(const __hip_bfloat16* p) { return *reinterpret_cast<const Frag*>(p); }
+  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int s, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a.v[s], b.v[s], c, 0, 0, 0);
   }
 };
 template <> struct Mfma<__half> {
-  static constexpr int K = 8;
-  typedef f16x4 Frag;
-  static __device__ __forceinline__ Frag load(const __half* p) { return *reinterpret_cast<const f16x4*>(p); }
-  static __device__ __forceinline__ acc16 run(Frag a, Frag b, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, c, 0, 0, 0);
+  static constexpr int G = 16;
+  static constexpr int S = 2;
+  struct alignas(16) Frag { f16x4 v[2]; };
+  static __device__ __forceinline__ Frag load(const __half* p) { return *reinterpret_cast<const Frag*>(p); }
+  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int s, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x8f16(a.v[s], b.v[s], c, 0, 0, 0);
   }
 };
 template <> struct Mfma<float> {
-  static constexpr int K = 2;
-  typedef float Frag;
-  static __device__ __forceinline__ Frag load(const float* p) { return *p; }
-  static __device__ __forceinline__ acc16 run(Frag a, Frag b, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  static constexpr int G = 8;
+  static constexpr int S = 4;
+  struct alignas(16) Frag { float v[4]; };
+  static __device__ __forceinline__ Frag load(const float* p) { return *reinterpret_cast<const Frag*>(p); }
+  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int s, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[s], b.v[s], c, 0, 0, 0);
   }
 };
 
@@ -59,14 +70,156 @@ __device__ __forceinline__ int plane_offset(int y, int x, int w) {
   return (((y >> 3) * (w >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
 }
 
+// Chunk width for vector stores of a row segment of `width` elements.
+__host__ __device__ constexpr int chunk_of(int width) {
+  return width % 8 == 0 ? 8 : (width % 4 == 0 ? 4 : (width % 2 == 0 ? 2 : 1));
+}
+
+template <typename To, int CW>
+struct alignas(CW * sizeof(To)) OutV {
+  To v[CW];
+};
+
+// Pooled level (hierarchical: each level averages 2x2 of the level above it, as repeated
+// avg_pool2d does).  src holds the level above for the tile ([kTP][RS][WS] floats, pitch PS per
+// p); the values are stored to `out` and returned in v, so that the caller can place them in
+// LDS for the next level once every thread has finished reading src.
+template <int RS, int WS>
+struct Pool {
+  static constexpr int RD = RS / 2, WD = WS / 2, CW = chunk_of(WD);
+  static constexpr int NCH = kTP * RD * WD / CW;     // chunks of the tile
+  static constexpr int NIT = (NCH + kThreads - 1) / kThreads;
+  static constexpr int PD = RD * WD + 4;             // pitch per p of this level in LDS
+};
+
+template <typename To, int RS, int WS, int PS>
+__device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* __restrict__ out,
+                                             int64_t plane0, int64_t plane_stride, int gy0, int gx0,
+                                             int wl, bool tl, float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
+  using PL = Pool<RS, WS>;
+  constexpr int WD = PL::WD, CW = PL::CW;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < PL::NIT; ++it) {
+    const int c = it * kThreads + t;
+    if (PL::NCH % kThreads == 0 || c < PL::NCH) {
+      const int p = c / (PL::RD * WD / CW), r = c % (PL::RD * WD / CW);
+      const int y = r / (WD / CW), x = (r % (WD / CW)) * CW;
+      const float* a = src + p * PS + (2 * y) * WS + 2 * x;
+      OutV<To, CW> o;
+#pragma unroll
+      for (int e = 0; e < CW; ++e) {
+        v[it][e] = 0.25f * ((a[2 * e] + a[2 * e + 1]) + (a[WS + 2 * e] + a[WS + 2 * e + 1]));
+        o.v[e] = from_f<To>(v[it][e]);
+      }
+      const int gy = gy0 + y, gx = gx0 + x;
+      const int off = tl ? plane_offset(gy, gx, wl) : gy * wl + gx;
+      *reinterpret_cast<OutV<To, CW>*>(out + (plane0 + p) * plane_stride + off) = o;
+    }
+  }
+}
+
+template <int RS, int WS>
+__device__ __forceinline__ void pool_stash(float* __restrict__ dst,
+                                           const float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
+  using PL = Pool<RS, WS>;
+  constexpr int WD = PL::WD, CW = PL::CW;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < PL::NIT; ++it) {
+    const int c = it * kThreads + t;
+    if (PL::NCH % kThreads == 0 || c < PL::NCH) {
+      const int p = c / (PL::RD * WD / CW), r = c % (PL::RD * WD / CW);
+      const int y = r / (WD / CW), x = (r % (WD / CW)) * CW;
+#pragma unroll
+      for (int e = 0; e < CW; ++e) dst[p * PL::PD + y * WD + x + e] = v[it][e];
+    }
+  }
+}
+
+// The f32 tile of 32 p rows staged in LDS -> level 0 and the pooled levels of those rows.
+// pl0 = index of the first p row's plane.  Ends with every thread done reading LDS.
+template <typename To, int WT>
+__device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__ sout, To* __restrict__ l0,
+                          To* __restrict__ l1, To* __restrict__ l2, To* __restrict__ l3, int64_t pl0,
+                          int h, int w, int y0, int x0, int levels, int tiled) {
+  constexpr int NQ = kRows * WT;
+  constexpr int SQ = NQ + 4;
+  const int HW = h * w;
+  const int t = threadIdx.x;
+  // level 0: row segments of 8 elements (one row of an 8x8 block in the tiled layout), one
+  // vector store each; consecutive threads read consecutive segments of the LDS tile
+  const bool tile0 = tiled && (h % 8 == 0) && (w % 8 == 0);
+  constexpr int NCH0 = kTP * NQ / 8;
+#pragma unroll
+  for (int c0 = 0; c0 < NCH0; c0 += kThreads) {
+    const int c = c0 + t;
+    const int p = c / (NQ / 8), r = c % (NQ / 8);
+    const int y = r / (WT / 8), x = (r % (WT / 8)) * 8;
+    const float* src = sout_in + p * SQ + y * WT + x;
+    OutV<To, 8> o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o.v[e] = from_f<To>(src[e]);
+    const int qy = y0 + y, qx = x0 + x;
+    const int off = tile0 ? plane_offset(qy, qx, w) : qy * w + qx;
+    *reinterpret_cast<OutV<To, 8>*>(l0 + (pl0 + p) * HW + off) = o;
+  }
+  if (levels < 2) {
+    __syncthreads();
+    return;
+  }
+  // pooled levels, each from the level above; every level is kept in LDS for the next one,
+  // inside the (then finished) f32 tile region: level 1 at its start, level 2 behind it
+  using P1 = Pool<kRows, WT>;
+  using P2 = Pool<kRows / 2, WT / 2>;
+  using P3 = Pool<kRows / 4, WT / 4>;
+  float* s1 = sout;
+  float* s2 = s1 + kTP * P1::PD;
+  {
+    float v[P1::NIT][P1::CW];
+    const int hl = h >> 1, wl = w >> 1;
+    pool_compute<To, kRows, WT, SQ>(sout_in, l1, pl0, (int64_t)hl * wl, y0 >> 1, x0 >> 1, wl,
+                                    tiled && hl % 8 == 0 && wl % 8 == 0, v);
+    __syncthreads();                               // every read of the f32 tile is done
+    if (levels < 3) return;
+    pool_stash<kRows, WT>(s1, v);
+  }
+  __syncthreads();
+  {
+    float v[P2::NIT][P2::CW];
+    const int hl = h >> 2, wl = w >> 2;
+    pool_compute<To, kRows / 2, WT / 2, P1::PD>(s1, l2, pl0, (int64_t)hl * wl, y0 >> 2, x0 >> 2, wl,
+                                                tiled && hl % 8 == 0 && wl % 8 == 0, v);
+    if (levels < 4) {
+      __syncthreads();
+      return;
+    }
+    pool_stash<kRows / 2, WT / 2>(s2, v);           // disjoint from s1: no barrier needed first
+  }
+  __syncthreads();
+  {
+    float v[P3::NIT][P3::CW];
+    const int hl = h >> 3, wl = w >> 3;
+    pool_compute<To, kRows / 4, WT / 4, P2::PD>(s2, l3, pl0, (int64_t)hl * wl, y0 >> 3, x0 >> 3, wl,
+                                                tiled && hl % 8 == 0 && wl % 8 == 0, v);
+  }
+  __syncthreads();
+}
+
 // 1-D grid over (p tile, q tile, edge), p tile fastest; see the block order note above.
-template <typename Ti, typename To>
+// A block covers kPH x 32 p rows (kPH MFMA operands sharing every fmap2 fragment) x 8 image
+// rows x WT columns; WT is a compile-time constant so that every index split is a shift or a
+// constant division.
+template <typename Ti, typename To, int WT>
 __global__ void __launch_bounds__(kThreads)
 corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __restrict__ l0,
                   To* __restrict__ l1, To* __restrict__ l2, To* __restrict__ l3, int C, int h,
-                  int w, int WT, int levels, int tiled, int ptiles, int qtiles, int64_t nblocks) {
+                  int w, int levels, int tiled, int ptiles, int qtiles, int64_t nblocks) {
+  constexpr int NQ = kRows * WT;                   // columns of the tile
+  constexpr int SQ = NQ + 4;                       // padded LDS row of the f32 tile
+  constexpr int PW = NQ / 32 / 4;                  // 32-wide MFMA blocks per wave
   extern __shared__ __align__(16) unsigned char smem[];
-  float* sout = reinterpret_cast<float*>(smem);    // [kTP][kRows*WT]
+  float* sout = reinterpret_cast<float*>(smem);    // [kTP][SQ]
 
   // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a contiguous range of
   // logical blocks (whole edges), so an edge's feature maps are fetched into one L2 only.
@@ -78,117 +231,100 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   const int n = static_cast<int>(lb / ((int64_t)ptiles * qtiles));
 
   const int HW = h * w;
-  const int p0 = pt * kTP;
+  const int p0 = pt * kPH * kTP;
   const int qtiles_x = w / WT;
   const int y0 = (qt / qtiles_x) * kRows;
   const int x0 = (qt % qtiles_x) * WT;
-  const int NQ = kRows * WT;                       // columns of this tile
   const int t = threadIdx.x;
   const int wave = t / kWave, lane = t % kWave;
-  const int per_wave = NQ / 32 / 4;                // NQ is a multiple of 128
-  constexpr int K = Mfma<Ti>::K;
-  // lane i of the 32-row MFMA operand; lanes 32-63 carry the second half of the K block
+  constexpr int G = Mfma<Ti>::G;
+  // lane i of the 32-row MFMA operand; lanes 32-63 carry the other half of each channel group
   const int i = lane % 32, kg = lane / 32;
-  const Ti* ap = f1 + ((int64_t)n * HW + p0 + i) * C + kg * (K / 2);
-  const Ti* bp[4];
+  const Ti* ap0 = f1 + ((int64_t)n * HW + p0 + i) * C + kg * (G / 2);
+  const Ti* ap1 = ap0 + (int64_t)(kPH - 1) * kTP * C;
+  const Ti* bp[PW];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int q = (wave * per_wave + (k < per_wave ? k : 0)) * 32 + i;
+  for (int k = 0; k < PW; ++k) {
+    const int q = (wave * PW + k) * 32 + i;
     const int qpix = (y0 + q / WT) * w + x0 + q % WT;
-    bp[k] = f2 + ((int64_t)n * HW + qpix) * C + kg * (K / 2);
+    bp[k] = f2 + ((int64_t)n * HW + qpix) * C + kg * (G / 2);
   }
 
-  acc16 acc[4];
+  acc16 acc0[PW], acc1[PW];
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < PW; ++k)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
-#pragma unroll 4
-  for (int kk = 0; kk < C; kk += K) {
-    const typename Mfma<Ti>::Frag a = Mfma<Ti>::load(ap + kk);
+    for (int v = 0; v < 16; ++v) {
+      acc0[k][v] = 0.f;
+      acc1[k][v] = 0.f;
+    }
+#pragma unroll 2
+  for (int kk = 0; kk < C; kk += G) {
+    const typename Mfma<Ti>::Frag a0 = Mfma<Ti>::load(ap0 + kk);
+    typename Mfma<Ti>::Frag a1;
+    if constexpr (kPH == 2) a1 = Mfma<Ti>::load(ap1 + kk);
+    typename Mfma<Ti>::Frag bf[PW];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (k < per_wave) acc[k] = Mfma<Ti>::run(a, Mfma<Ti>::load(bp[k] + kk), acc[k]);
+    for (int k = 0; k < PW; ++k) bf[k] = Mfma<Ti>::load(bp[k] + kk);
+#pragma unroll
+    for (int st = 0; st < Mfma<Ti>::S; ++st)
+#pragma unroll
+      for (int k = 0; k < PW; ++k) {
+        acc0[k] = Mfma<Ti>::run(a0, bf[k], st, acc0[k]);
+        if constexpr (kPH == 2) acc1[k] = Mfma<Ti>::run(a1, bf[k], st, acc1[k]);
+      }
   }
 
-  // scatter the accumulators into the LDS tile [p][q] (scaled by 1/16)
+  // each group of 32 p rows through the same LDS tile; MFMA output layout: value v of lane l
+  // is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (k < per_wave) {
-      const int q = (wave * per_wave + k) * 32 + lane % 32;
+  for (int half = 0; half < kPH; ++half) {
+#pragma unroll
+    for (int k = 0; k < PW; ++k) {
+      const int q = (wave * PW + k) * 32 + lane % 32;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        sout[p * NQ + q] = acc[k][v] * 0.0625f;
+        sout[p * SQ + q] = (half ? acc1[k][v] : acc0[k][v]) * 0.0625f;
       }
     }
-  }
-  __syncthreads();
-
-  // level 0: groups of 8 elements that are consecutive in memory (a row segment of the
-  // tile, or one row of an 8x8 block in the tiled layout), one vector store each
-  const bool tile0 = tiled && (h % 8 == 0) && (w % 8 == 0);
-  struct alignas(8 * sizeof(To)) Out8 { To v[8]; };
-  for (int g = t; g < kTP * NQ / 8; g += kThreads) {
-    const int p = g / (NQ / 8), q8 = g % (NQ / 8);
-    int qy, qx;
-    if (tile0) {            // consecutive groups walk the 8x8 blocks of the 8-row band in memory order
-      qy = y0 + (q8 & 7);
-      qx = x0 + (q8 >> 3) * 8;
-    } else {
-      qy = y0 + (q8 * 8) / WT;
-      qx = x0 + (q8 * 8) % WT;
-    }
-    const float* src = sout + p * NQ + (qy - y0) * WT + (qx - x0);
-    Out8 o;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) o.v[e] = from_f<To>(src[e]);
-    const int off = tile0 ? plane_offset(qy, qx, w) : qy * w + qx;
-    *reinterpret_cast<Out8*>(l0 + ((int64_t)n * HW + p0 + p) * HW + off) = o;
-  }
-  // pooled levels: level L averages 2^L x 2^L blocks of level 0
-  To* outs[3] = {l1, l2, l3};
-  for (int L = 1; L < levels; ++L) {
-    const int s = 1 << L;
-    const int hl = h >> L, wl = w >> L;
-    const int ry = kRows >> L, rx = WT >> L;       // pooled rows / cols inside the tile
-    const int cnt = ry * rx;
-    const float inv = 1.f / static_cast<float>(s * s);
-    for (int idx = t; idx < kTP * cnt; idx += kThreads) {
-      const int p = idx / cnt, rem = idx % cnt;
-      const int yy = rem / rx, xx = rem % rx;
-      float sum = 0.f;
-      for (int dy = 0; dy < s; ++dy)
-        for (int dx = 0; dx < s; ++dx) sum += sout[p * NQ + (yy * s + dy) * WT + xx * s + dx];
-      const int gy = (y0 >> L) + yy, gx = (x0 >> L) + xx;
-      const bool tl = tiled && (hl % 8 == 0) && (wl % 8 == 0);
-      outs[L - 1][((int64_t)n * HW + p0 + p) * hl * wl + (tl ? plane_offset(gy, gx, wl) : gy * wl + gx)] =
-          from_f<To>(sum * inv);
-    }
+    __syncthreads();
+    emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + half * kTP, h, w, y0, x0,
+                      levels, tiled);
   }
 }
 
-template <typename Ti, typename To>
-int launch(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
-           int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
-  const int WT = w % 64 == 0 ? 64 : static_cast<int>(w);
-  HSL2_REQUIRE((kRows * WT) % 128 == 0, HSL2_ERR_UNSUPPORTED,
-               "corr build: width must be a multiple of 64 or 16..63 with 8*w % 128 == 0");
-  const size_t lds = sizeof(float) * kTP * kRows * WT;
+template <typename Ti, typename To, int WT>
+int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
+              int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
+  constexpr int NQ = kRows * WT, SQ = NQ + 4;
+  const size_t lds = sizeof(float) * kTP * SQ;
   HSL2_REQUIRE(lds <= 160 * 1024, HSL2_ERR_UNSUPPORTED, "corr build: LDS budget exceeded");
-  const int ptiles = static_cast<int>(h * w / kTP);
+  const int ptiles = static_cast<int>(h * w / (kPH * kTP));
   const int qtiles = static_cast<int>((h / kRows) * (w / WT));
   const int64_t nblocks = (int64_t)ptiles * qtiles * batch;
   if (nblocks == 0) return HSL2_OK;
   HSL2_REQUIRE(nblocks < (1ll << 31), HSL2_ERR_SHAPE, "corr build: grid too large");
   To* l[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < L; ++i) l[i] = static_cast<To*>(levels[i]);
-  hipLaunchKernelGGL((corr_build_kernel<Ti, To>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
+  hipLaunchKernelGGL((corr_build_kernel<Ti, To, WT>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
                      static_cast<const Ti*>(f1), static_cast<const Ti*>(f2), l[0], l[1], l[2], l[3],
-                     static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), WT, L, tiled, ptiles,
-                     qtiles, nblocks);
+                     static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), L, tiled, ptiles, qtiles,
+                     nblocks);
   HSL2_LAUNCH_CHECK();
   return HSL2_OK;
+}
+
+// tile widths: 64 for widths that are multiples of 64, else the width itself (16, 32 or 48)
+template <typename Ti, typename To>
+int launch(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
+           int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
+  if (w % 64 == 0) return launch_wt<Ti, To, 64>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 48) return launch_wt<Ti, To, 48>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 32) return launch_wt<Ti, To, 32>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 16) return launch_wt<Ti, To, 16>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  set_error("corr build: width must be a multiple of 64, or 16, 32 or 48");
+  return HSL2_ERR_UNSUPPORTED;
 }
 
 }  // namespace
@@ -204,10 +340,10 @@ extern "C" int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtyp
   for (int i = 0; i < num_levels; ++i)
     HSL2_REQUIRE(levels[i], HSL2_ERR_ARGUMENT, "corr build: null level pointer");
   HSL2_REQUIRE(batch >= 0 && channels > 0 && h > 0 && w > 0, HSL2_ERR_SHAPE, "corr build: bad sizes");
-  HSL2_REQUIRE(channels % 8 == 0, HSL2_ERR_UNSUPPORTED, "corr build: channels must be a multiple of 8");
+  HSL2_REQUIRE(channels % 16 == 0, HSL2_ERR_UNSUPPORTED, "corr build: channels must be a multiple of 16");
   HSL2_REQUIRE(h % kRows == 0 && w % 8 == 0, HSL2_ERR_UNSUPPORTED,
                "corr build: h and w must be multiples of 8");
-  HSL2_REQUIRE(h * w < (1ll << 31) && (h * w) % kTP == 0, HSL2_ERR_SHAPE, "corr build: bad plane size");
+  HSL2_REQUIRE(h * w < (1ll << 31) && (h * w) % (kPH * kTP) == 0, HSL2_ERR_SHAPE, "corr build: bad plane size");
   hipStream_t s = as_stream(stream);
   if (out_dtype == HSL2_BF16) {
     if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);

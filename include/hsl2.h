@@ -91,7 +91,7 @@ int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const fl
  * average-pool levels.  fmap1/fmap2 are CHANNEL-LAST [batch, h * w, C] (`in_dtype`
  * F32/BF16/F16; hsl2_channels_last converts the reference's [batch, C, h, w]).  levels[i]
  * points at [batch, h, w, h >> i, w >> i] of `out_dtype` (F32/BF16).  Requires h, w multiples
- * of 8 and C a multiple of 8.  tiled != 0 stores every level plane whose sides are multiples of 8 as
+ * of 8 and C a multiple of 16.  tiled != 0 stores every level plane whose sides are multiples of 8 as
  * row-major 8x8 tiles (read by hsl2_corr_pyramid_lookup with tiled != 0); tiled == 0 gives the
  * reference's row-major planes. */
 int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype, void* const* levels,
