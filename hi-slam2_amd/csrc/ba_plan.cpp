@@ -15,6 +15,28 @@ namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// (key, code) pushes, bucketed stably by key into a CSR table (ptr [nkeys + 1], src).
+struct Pushes {
+  std::vector<int32_t> key, code;
+  void reserve(int64_t n) {
+    key.reserve(static_cast<size_t>(n));
+    code.reserve(static_cast<size_t>(n));
+  }
+  size_t size() const { return key.size(); }
+  void push(int64_t k, int32_t c) {
+    key.push_back(static_cast<int32_t>(k));
+    code.push_back(c);
+  }
+  void to_csr(int64_t nkeys, std::vector<int32_t>& ptr, std::vector<int32_t>& src) const {
+    ptr.assign(static_cast<size_t>(nkeys + 1), 0);
+    for (int32_t k : key) ptr[k + 1]++;
+    for (int64_t k = 0; k < nkeys; ++k) ptr[k + 1] += ptr[k];
+    src.resize(key.size());
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t i = 0; i < key.size(); ++i) src[fill[key[i]]++] = code[i];
+  }
+};
+
 }  // namespace
 
 extern "C" int hsl2_ba_plan_create(const int64_t* ii, const int64_t* jj, int64_t num_edges,
@@ -117,59 +139,76 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
 
   // Schur slots: expanded edges n (ascending) with pose slot jj_exp - t0 in [0, P),
   // grouped by depth frame (the graph of src/droid_kernels.cu:1706-1716).
+  // Block (P x P) and rhs (P) contribution lists are collected as (key, code) pushes and
+  // bucketed stably into CSR tables at the end, so each list keeps its push order.
   std::vector<int32_t> slot_ptr(F + 1, 0), slot_row, slot_pose;
   std::vector<int32_t> tasks;
-  std::vector<std::vector<int32_t>> blk(static_cast<size_t>(P * P));
-  std::vector<std::vector<int32_t>> rhs(static_cast<size_t>(P));
+  Pushes blk, rhs;
+  blk.reserve(4 * E);
+  rhs.reserve(2 * E);
   // H-block and v contributions (src/droid_kernels.cu:1882-1889)
   for (int64_t e = 0; e < E; ++e) {
     const int64_t a = ii[e] - t0, b = jj[e] - t0;
     const int32_t ie = static_cast<int32_t>(e) << 3;
-    if (a >= 0) blk[a * P + a].push_back(ie | hsl2::kBlkHii);
+    if (a >= 0) blk.push(a * P + a, ie | hsl2::kBlkHii);
     if (a >= 0 && b >= 0) {
-      blk[a * P + b].push_back(ie | hsl2::kBlkHij);
-      blk[b * P + a].push_back(ie | hsl2::kBlkHji);
+      blk.push(a * P + b, ie | hsl2::kBlkHij);
+      blk.push(b * P + a, ie | hsl2::kBlkHji);
     }
-    if (b >= 0) blk[b * P + b].push_back(ie | hsl2::kBlkHjj);
-    if (a >= 0) rhs[a].push_back((static_cast<int32_t>(e) << 2) | hsl2::kRhsVi);
-    if (b >= 0) rhs[b].push_back((static_cast<int32_t>(e) << 2) | hsl2::kRhsVj);
+    if (b >= 0) blk.push(b * P + b, ie | hsl2::kBlkHjj);
+    if (a >= 0) rhs.push(a, (static_cast<int32_t>(e) << 2) | hsl2::kRhsVi);
+    if (b >= 0) rhs.push(b, (static_cast<int32_t>(e) << 2) | hsl2::kRhsVj);
   }
   if (!p->motion_only) {
-    std::vector<std::vector<int32_t>> per_frame(F);
+    // slots bucketed by depth frame, ascending n inside a frame
     for (int64_t n = 0; n < P + E; ++n) {
       const int64_t a = jj_exp[n] - t0;
-      if (a >= 0 && a < P && p->kk_exp[n] >= 0) per_frame[p->kk_exp[n]].push_back(static_cast<int32_t>(n));
+      if (a >= 0 && a < P && p->kk_exp[n] >= 0) slot_ptr[p->kk_exp[n] + 1]++;
     }
-    for (int64_t f = 0; f < F; ++f) {
-      slot_ptr[f + 1] = slot_ptr[f] + static_cast<int32_t>(per_frame[f].size());
-      for (int32_t n : per_frame[f]) {
-        slot_row.push_back(n);
-        slot_pose.push_back(static_cast<int32_t>(jj_exp[n] - t0));
-      }
-    }
-    p->NS = static_cast<int64_t>(slot_row.size());
-    int64_t npair = 0;
-    for (int64_t f = 0; f < F; ++f) {
-      for (int32_t s = slot_ptr[f]; s < slot_ptr[f + 1]; ++s) {
-        for (int32_t s2 = s; s2 < slot_ptr[f + 1]; ++s2) {
-          tasks.insert(tasks.end(), {hsl2::kTaskPair, s, s2, static_cast<int32_t>(f)});
-          const int64_t a = slot_pose[s], b = slot_pose[s2];
-          blk[a * P + b].push_back(static_cast<int32_t>(npair << 3) | hsl2::kBlkS);
-          if (s2 != s) blk[b * P + a].push_back(static_cast<int32_t>(npair << 3) | hsl2::kBlkSt);
-          ++npair;
+    for (int64_t f = 0; f < F; ++f) slot_ptr[f + 1] += slot_ptr[f];
+    p->NS = slot_ptr[F];
+    slot_row.resize(p->NS);
+    slot_pose.resize(p->NS);
+    {
+      std::vector<int32_t> fill(slot_ptr.begin(), slot_ptr.end() - 1);
+      for (int64_t n = 0; n < P + E; ++n) {
+        const int64_t a = jj_exp[n] - t0;
+        if (a >= 0 && a < P && p->kk_exp[n] >= 0) {
+          const int32_t s = fill[p->kk_exp[n]]++;
+          slot_row[s] = static_cast<int32_t>(n);
+          slot_pose[s] = static_cast<int32_t>(a);
         }
       }
     }
-    p->NPAIR = npair;
+    int64_t npair = 0;
+    for (int64_t f = 0; f < F; ++f) {
+      const int64_t m = slot_ptr[f + 1] - slot_ptr[f];
+      npair += m * (m + 1) / 2;
+    }
     if (npair >= (1ll << 27)) {
       delete p;
       set_error("ba plan: too many Schur pairs");
       return HSL2_ERR_SHAPE;
     }
+    p->NPAIR = npair;
+    tasks.reserve(4 * (npair + p->NS));
+    blk.reserve(blk.size() + 2 * npair);
+    npair = 0;
+    for (int64_t f = 0; f < F; ++f) {
+      for (int32_t s = slot_ptr[f]; s < slot_ptr[f + 1]; ++s) {
+        for (int32_t s2 = s; s2 < slot_ptr[f + 1]; ++s2) {
+          tasks.insert(tasks.end(), {hsl2::kTaskPair, s, s2, static_cast<int32_t>(f)});
+          const int64_t a = slot_pose[s], b = slot_pose[s2];
+          blk.push(a * P + b, static_cast<int32_t>(npair << 3) | hsl2::kBlkS);
+          if (s2 != s) blk.push(b * P + a, static_cast<int32_t>(npair << 3) | hsl2::kBlkSt);
+          ++npair;
+        }
+      }
+    }
     for (int64_t f = 0; f < F; ++f)
       for (int32_t s = slot_ptr[f]; s < slot_ptr[f + 1]; ++s) {
         tasks.insert(tasks.end(), {hsl2::kTaskEv, s, s, static_cast<int32_t>(f)});
-        rhs[slot_pose[s]].push_back((s << 2) | hsl2::kRhsEv);
+        rhs.push(slot_pose[s], (s << 2) | hsl2::kRhsEv);
       }
   }
   p->NT = static_cast<int64_t>(tasks.size() / 4);
@@ -200,18 +239,12 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
   p->o_slot_row = put(slot_row);
   p->o_slot_pose = put(slot_pose);
   p->o_task = put(tasks);
-  std::vector<int32_t> bptr(P * P + 1, 0), bsrc;
-  for (int64_t k = 0; k < P * P; ++k) {
-    bptr[k + 1] = bptr[k] + static_cast<int32_t>(blk[k].size());
-    bsrc.insert(bsrc.end(), blk[k].begin(), blk[k].end());
-  }
+  std::vector<int32_t> bptr, bsrc;
+  blk.to_csr(P * P, bptr, bsrc);
   p->o_blk_ptr = put(bptr);
   p->o_blk_src = put(bsrc);
-  std::vector<int32_t> rptr(P + 1, 0), rsrc;
-  for (int64_t k = 0; k < P; ++k) {
-    rptr[k + 1] = rptr[k] + static_cast<int32_t>(rhs[k].size());
-    rsrc.insert(rsrc.end(), rhs[k].begin(), rhs[k].end());
-  }
+  std::vector<int32_t> rptr, rsrc;
+  rhs.to_csr(P, rptr, rsrc);
   p->o_rhs_ptr = put(rptr);
   p->o_rhs_src = put(rsrc);
 

@@ -6,6 +6,7 @@
 // entirely on the device.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 

@@ -240,6 +240,161 @@ corr_lookup_win_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   }
 }
 
+// 16-bit volumes (bf16 / f16), radius 3, every level width a multiple of 8 and hw1 a multiple
+// of 4: 64 pixels of one level per 256-thread block, with far fewer (and wider) memory
+// requests than corr_lookup_win_kernel.  Each thread fetches two window rows; a row's 8 taps
+// x0 .. x0+7 lie in the two aligned 8-element chunks at floor(x0 / 8) * 8 and the next one,
+// fetched as two 16-byte loads (a chunk outside the plane reads as zeros; chunks never
+// straddle the plane edge because w2 % 8 == 0) and the taps extracted with a funnel shift.
+// The combine gives each thread four consecutive pixels of a set of output channels, so
+// every store writes 8 bytes.
+template <typename T> __device__ __forceinline__ float half_bits_to_f(uint32_t h);
+template <> __device__ __forceinline__ float half_bits_to_f<__hip_bfloat16>(uint32_t h) {
+  return __uint_as_float(h << 16);
+}
+template <> __device__ __forceinline__ float half_bits_to_f<__half>(uint32_t h) {
+  return __half2float(__ushort_as_half(static_cast<unsigned short>(h)));
+}
+template <typename T> __device__ __forceinline__ uint32_t f_to_half_bits(float v) {
+  const T h = from_f<T>(v);
+  return static_cast<uint32_t>(*reinterpret_cast<const unsigned short*>(&h));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
+                       int64_t cy_off, T* __restrict__ out, int hw1, int64_t out_batch_stride,
+                       int tiles_per_item) {
+  constexpr int RAD = 3, D = 8, R = 7;
+  constexpr int TS = D * D + 1;                 // padded per-pixel tap stride
+  __shared__ float taps[kPix * TS];
+  __shared__ int s_x0[kPix], s_y0[kPix];
+  __shared__ float s_dx[kPix], s_dy[kPix];
+
+  const int level = blockIdx.y;
+  const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
+  const int h2 = lv.h2[level], w2 = lv.w2[level];
+  const bool tiled = lv.tiled[level] != 0;
+  const float scale = lv.scale[level];
+  const int n = blockIdx.x / tiles_per_item;
+  const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
+  const int npix = min(kPix, hw1 - pix0);
+  const int t = threadIdx.x;
+
+  if (t < kPix) {
+    int x0 = 0, y0 = 0;
+    float fx = 0.f, fy = 0.f;
+    if (t < npix) {
+      const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
+      const float cx = cp_[0] * scale;
+      const float cy = cp_[cy_off] * scale;
+      const float flx = floorf(cx), fly = floorf(cy);
+      x0 = static_cast<int>(flx) - RAD;
+      y0 = static_cast<int>(fly) - RAD;
+      fx = cx - flx;
+      fy = cy - fly;
+    }
+    s_x0[t] = x0;
+    s_y0[t] = y0;
+    s_dx[t] = fx;
+    s_dy[t] = fy;
+  }
+  __syncthreads();
+
+  const int64_t plane = (int64_t)h2 * w2;
+  const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
+  // gather: item = (pixel, window row); all four loads of a thread are issued first
+  uint4 ca[2], cbv[2];
+  int sh[2];
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int item = t + kThreads * it;
+    const int p = item >> 3, ty = item & 7;
+    const int y = s_y0[p] + ty, x0 = s_x0[p];
+    const int xa = (x0 >> 3) << 3;              // floor to a multiple of 8 (also for x0 < 0)
+    sh[it] = x0 - xa;
+    const bool yok = p < npix && y >= 0 && y < h2;
+    const bool oka = yok && xa >= 0 && xa < w2;
+    const bool okb = yok && xa + 8 >= 0 && xa + 8 < w2;
+    const T* pl = vbase + (int64_t)(p < npix ? p : 0) * plane;
+    const int offa = tiled ? tiled_offset(y, xa, w2) : y * w2 + xa;
+    const int offb = tiled ? tiled_offset(y, xa + 8, w2) : y * w2 + xa + 8;
+    // unconditional 16-byte loads from a clamped address (offset 0 of the plane is always
+    // valid), zeroed afterwards: keeps them plain dwordx4 global loads
+    const uint4 la = *reinterpret_cast<const uint4*>(pl + (oka ? offa : 0));
+    const uint4 lb = *reinterpret_cast<const uint4*>(pl + (okb ? offb : 0));
+    ca[it] = make_uint4(oka ? la.x : 0u, oka ? la.y : 0u, oka ? la.z : 0u, oka ? la.w : 0u);
+    cbv[it] = make_uint4(okb ? lb.x : 0u, okb ? lb.y : 0u, okb ? lb.z : 0u, okb ? lb.w : 0u);
+  }
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int item = t + kThreads * it;
+    const int p = item >> 3, ty = item & 7;
+    // barrel shift the 16 halves left by sh: two word stages, then a half-word align
+    uint32_t w0 = ca[it].x, w1 = ca[it].y, w2 = ca[it].z, w3 = ca[it].w;
+    uint32_t w4 = cbv[it].x, w5 = cbv[it].y, w6 = cbv[it].z, w7 = cbv[it].w;
+    if (sh[it] & 4) {
+      w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7;
+    }
+    if (sh[it] & 2) {
+      w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5;
+    }
+    const uint32_t al = (sh[it] & 1) ? 2u : 0u;
+    const uint32_t d0 = __builtin_amdgcn_alignbyte(w1, w0, al);
+    const uint32_t d1 = __builtin_amdgcn_alignbyte(w2, w1, al);
+    const uint32_t d2 = __builtin_amdgcn_alignbyte(w3, w2, al);
+    const uint32_t d3 = __builtin_amdgcn_alignbyte(w4, w3, al);
+    float* dst = taps + p * TS + ty * D;
+    dst[0] = half_bits_to_f<T>(d0 & 0xffffu);
+    dst[1] = half_bits_to_f<T>(d0 >> 16);
+    dst[2] = half_bits_to_f<T>(d1 & 0xffffu);
+    dst[3] = half_bits_to_f<T>(d1 >> 16);
+    dst[4] = half_bits_to_f<T>(d2 & 0xffffu);
+    dst[5] = half_bits_to_f<T>(d2 >> 16);
+    dst[6] = half_bits_to_f<T>(d3 & 0xffffu);
+    dst[7] = half_bits_to_f<T>(d3 >> 16);
+  }
+  __syncthreads();
+
+  // combine: thread = (pixel quad q, channel group g); channel o = g, g + 16, g + 32, g + 48
+  const int q = t & 15, g = t >> 4;
+  T* obase = out + (int64_t)n * out_batch_stride + lv.out_offset[level] + pix0 + 4 * q;
+  float w00[4], w10[4], w01[4], w11[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const float fx = s_dx[4 * q + u], fy = s_dy[4 * q + u];
+    w00[u] = (1.f - fx) * (1.f - fy);
+    w10[u] = fx * (1.f - fy);
+    w01[u] = (1.f - fx) * fy;
+    w11[u] = fx * fy;
+  }
+#pragma unroll
+  for (int oo = 0; oo < 4; ++oo) {
+    const int o = g + 16 * oo;
+    if (o < R * R) {
+      const int a = o / R, b = o - (o / R) * R;
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float* tp = taps + (4 * q + u) * TS;
+        v[u] = w00[u] * tp[b * D + a] + w10[u] * tp[b * D + a + 1] + w01[u] * tp[(b + 1) * D + a] +
+               w11[u] * tp[(b + 1) * D + a + 1];
+      }
+      T* dst = obase + (int64_t)o * hw1;
+      if (4 * q + 3 < npix) {
+        uint2 pk;
+        pk.x = f_to_half_bits<T>(v[0]) | (f_to_half_bits<T>(v[1]) << 16);
+        pk.y = f_to_half_bits<T>(v[2]) | (f_to_half_bits<T>(v[3]) << 16);
+        *reinterpret_cast<uint2*>(dst) = pk;
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (4 * q + u < npix) dst[u] = from_f<T>(v[u]);
+      }
+    }
+  }
+}
+
 // Adjoint: writes the complete volume_grad planes of the block's 64 pixels.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -321,6 +476,21 @@ int launch_fwd(const LevelTable& lv, int levels, const float* coords, int64_t cb
                int64_t out_batch_stride, hipStream_t s) {
   if (batch * hw1 == 0) return HSL2_OK;
   const int tiles = static_cast<int>((hw1 + kPix - 1) / kPix);
+  if constexpr (sizeof(T) == 2) {
+    // 16-byte volume chunks and 8-byte output quads must be aligned
+    bool vec = r == 3 && hw1 % 4 == 0 && out_batch_stride % 4 == 0 && batch * tiles < (1ll << 31) &&
+               reinterpret_cast<uintptr_t>(corr) % 8 == 0;
+    for (int l = 0; l < levels; ++l)
+      vec = vec && lv.w2[l] % 8 == 0 && lv.out_offset[l] % 4 == 0 &&
+            reinterpret_cast<uintptr_t>(lv.vol[l]) % 16 == 0;
+    if (vec) {
+      dim3 grid(static_cast<unsigned>(batch * tiles), levels);
+      hipLaunchKernelGGL(corr_lookup_vec_kernel<T>, grid, dim3(256), 0, s, lv, coords, cb, cp, cy_off,
+                         static_cast<T*>(corr), static_cast<int>(hw1), out_batch_stride, tiles);
+      HSL2_LAUNCH_CHECK();
+      return HSL2_OK;
+    }
+  }
   if ((r == 3 || r == 4) && batch * tiles < (1ll << 31)) {
     dim3 grid(static_cast<unsigned>(batch * tiles), levels);
     if (r == 3)

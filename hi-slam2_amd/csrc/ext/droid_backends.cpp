@@ -13,6 +13,11 @@
 #include <ATen/hip/impl/HIPGuardImplMasqueradingAsCUDA.h>
 #include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
 
+#include <array>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +60,55 @@ int dtype_code(const torch::Tensor& t) {
 }
 
 // ---- ba (src/droid.cpp:141-168) ------------------------------------------------
+// The last few ba plans, keyed by the complete edge list and every planning argument, so
+// that repeated calls on one factor graph (each call of a global BA over the same graph)
+// skip the host planning.  A hit compares the whole key; entries are shared so an evicted
+// plan stays valid for a call still using it.
+class PlanCache {
+ public:
+  using Args = std::array<int64_t, 8>;   // num_poses, ht, wd, t0, t1, motion_only, precision, semantics
+  using Ptr = std::shared_ptr<const hsl2_ba_plan>;
+
+  Ptr get(const int64_t* ii, const int64_t* jj, int64_t E, const Args& a) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (auto it = entries_.begin(); it != entries_.end(); ++it) {
+        if (it->args == a && static_cast<int64_t>(it->ii.size()) == E &&
+            std::memcmp(it->ii.data(), ii, E * sizeof(int64_t)) == 0 &&
+            std::memcmp(it->jj.data(), jj, E * sizeof(int64_t)) == 0) {
+          entries_.splice(entries_.begin(), entries_, it);
+          return entries_.front().plan;
+        }
+      }
+    }
+    hsl2_ba_plan* raw = nullptr;
+    raise_status(hsl2_ba_plan_create_ex(ii, jj, E, a[0], a[1], a[2], static_cast<int>(a[3]),
+                                        static_cast<int>(a[4]), static_cast<int>(a[5]),
+                                        static_cast<int>(a[6]), static_cast<int>(a[7]), &raw),
+                 "ba");
+    Ptr plan(raw, hsl2_ba_plan_destroy);
+    std::lock_guard<std::mutex> lock(mu_);
+    entries_.push_front(Entry{a, std::vector<int64_t>(ii, ii + E), std::vector<int64_t>(jj, jj + E), plan});
+    if (entries_.size() > kCapacity) entries_.pop_back();
+    return plan;
+  }
+
+ private:
+  struct Entry {
+    Args args;
+    std::vector<int64_t> ii, jj;
+    Ptr plan;
+  };
+  static constexpr size_t kCapacity = 4;
+  std::mutex mu_;
+  std::list<Entry> entries_;
+};
+
+PlanCache& plan_cache() {
+  static PlanCache cache;
+  return cache;
+}
+
 std::vector<torch::Tensor> ba_impl(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
                                    torch::Tensor targets, torch::Tensor weights, torch::Tensor eta,
                                    torch::Tensor ii, torch::Tensor jj, int t0, int t1, int iterations,
@@ -83,11 +137,10 @@ std::vector<torch::Tensor> ba_impl(torch::Tensor poses, torch::Tensor disps, tor
 
   // one host copy of the edge list per call (the reference copies it 16+ times per iteration)
   torch::Tensor ii_h = ii.to(torch::kCPU), jj_h = jj.to(torch::kCPU);
-  hsl2_ba_plan* plan = nullptr;
-  raise_status(hsl2_ba_plan_create(ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E, poses.size(0),
-                                   ht, wd, t0, t1, motion_only ? 1 : 0, precision, &plan),
-               "ba");
-  std::unique_ptr<hsl2_ba_plan, void (*)(hsl2_ba_plan*)> guard_plan(plan, hsl2_ba_plan_destroy);
+  const PlanCache::Ptr plan_ref = plan_cache().get(
+      ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E,
+      {poses.size(0), ht, wd, t0, t1, motion_only ? 1 : 0, precision, HSL2_SEM_CUDA});
+  const hsl2_ba_plan* plan = plan_ref.get();
   const int64_t F = hsl2_ba_plan_num_frames(plan);
   int64_t eta_rows = 1;
   const float* eta_ptr = nullptr;

@@ -33,11 +33,16 @@ def rounded(x, dtype):
     return torch.as_tensor(x).to(dtype)
 
 
+# (h1, w1, h2, w2): the first and third take the 16-byte-chunk path of 16-bit volumes at
+# radius 3 (w2 % 8 == 0, h1 w1 % 4 == 0; the third with a partial last block of 36 pixels),
+# the second the per-tap path (w2 = 11, h1 w1 = 35)
+@pytest.mark.parametrize('shape', [(12, 16, 12, 16), (5, 7, 9, 11), (10, 10, 7, 24)])
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float64, torch.float16, torch.bfloat16])
 @pytest.mark.parametrize('r', [3, 4])
-def test_lookup_forward(db, dtype, r):
+def test_lookup_forward(db, dtype, r, shape):
     rng = np.random.default_rng(10 + r)
-    BN, h1, w1, h2, w2 = 2, 12, 16, 12, 16
+    BN = 2
+    h1, w1, h2, w2 = shape
     vol = rounded(rng.standard_normal((BN, h1, w1, h2, w2)), dtype)
     coords = np.stack([rng.uniform(-4, w2 + 4, (BN, h1, w1)), rng.uniform(-4, h2 + 4, (BN, h1, w1))], 1)
     coords = torch.as_tensor(coords, dtype=torch.float32)
