@@ -7,12 +7,13 @@
 //
 //   for each tile column k:
 //     chol_panel   one block per tile row i >= k: in every block one wave factors the
-//                  diagonal tile in registers (readlane broadcasts, no barrier per column;
-//                  redundantly in every block, so no inter-block wait is needed) and publishes
-//                  each finished column of L to LDS; a second wave consumes the columns as
-//                  they arrive: block 0 forms L_kk^-1 (kept in a separate diagonal buffer, A_kk
-//                  is still being read by the other blocks) and y_k = L_kk^-1 b_k, the others
-//                  solve L_ik L_kk^T = A_ik row by row
+//                  diagonal tile in registers (readlane broadcasts; redundantly in every
+//                  block, so no inter-block wait is needed) and publishes each finished column
+//                  of L to LDS behind a workgroup-scope counter; a second wave consumes the
+//                  columns four at a time as they arrive (no barrier per column): block 0 forms
+//                  L_kk^-1 (kept in a separate diagonal buffer, A_kk is still being read by the
+//                  other blocks) and y_k = L_kk^-1 b_k, the others solve L_ik L_kk^T = A_ik
+//                  row by row
 //     chol_update  one block per lower trailing tile (i, j), k < j <= i: A_ij -= L_ik L_jk^T,
 //                  plus one block per tile row i > k: b_i -= L_ik y_k
 //   for each tile row k from the bottom:
@@ -92,15 +93,50 @@ template <> __device__ __forceinline__ float lane_bcast<float>(float v, int lane
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Wait until the LDS word *flag (written by another wave of the block) reaches v.  The poll
+// loop is a single asm block so the caller's fully unrolled code stays straight-line for the
+// register allocator; the load is waited for inside the block.  Bounded: after 2^22 polls it
+// gives up and returns false (the caller then reports a failed factorisation).
+__device__ __forceinline__ bool wait_flag(const int* flag, int v) {
+  const unsigned addr = static_cast<unsigned>(reinterpret_cast<uintptr_t>(flag));
+  int val;
+  unsigned left;
+  asm volatile(
+      "s_mov_b32 %[left], 0x400000\n"
+      "1:\n"
+      "ds_read_b32 %[val], %[addr]\n"
+      "s_waitcnt lgkmcnt(0)\n"
+      "v_cmp_lt_i32 vcc, %[val], %[want]\n"
+      "s_cbranch_vccz 2f\n"
+      "s_sub_u32 %[left], %[left], 1\n"
+      "s_cmp_eq_u32 %[left], 0\n"
+      "s_cbranch_scc1 2f\n"
+      "s_sleep 1\n"
+      "s_branch 1b\n"
+      "2:\n"
+      : [val] "=&v"(val), [left] "=&s"(left)
+      : [addr] "v"(addr), [want] "v"(v)
+      : "vcc", "scc", "memory");
+  return left != 0;
+}
+
+// LDS shared by the two waves of a panel block.
+template <typename R> struct PanelShared {
+  R Lc[kTile][kLd];        // Lc[c][r] = L_kk[r][c]
+  R dinv[kTile];           // 1 / L_kk[c][c]
+  int published;           // columns of L_kk published so far
+};
+
 // Producer (wave 0 of every panel block): factor the diagonal tile with ONE wave.  Lane r
 // holds row r in registers; column c of the right-looking factorisation is broadcast from the
 // lanes that hold it (v_readlane), so the column steps need no data exchange through LDS.
-// After step c, column c of L is published to Lc[c][*] and the block barrier hands it to the
-// consumer wave, which takes exactly one barrier per column too (64 in all, on both waves)
-// and works one column behind, so the producer rarely waits.  A non-positive pivot sets
-// info[0] = 1 when `report` and is replaced by 1 so that the arithmetic stays finite.
+// After step c, column c of L and 1 / L[c][c] go to LDS and `published` is raised with a
+// workgroup-scope release store; the consumer wave follows through that counter, so neither
+// wave waits at a barrier per column.  (Every lane stores dinv[c]: the same value to the
+// same address, and no per-column branch.)  A non-positive pivot sets info[0] = 1 when
+// `report` and is replaced by 1 so that the arithmetic stays finite.
 template <typename R>
-__device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, R (*Lc)[kLd], bool report,
+__device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh, bool report,
                             int* __restrict__ info) {
   const int r = threadIdx.x;                     // 0..63
   const int64_t base = (int64_t)k * kTile;
@@ -119,85 +155,100 @@ __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, R (*Lc)
     const R t = a[c] * (is * is);                // a[r][c] / pivot
 #pragma unroll
     for (int s = c + 1; s < kTile; ++s) a[s] -= t * lane_bcast<R>(a[c], s);
-    Lc[c][r] = (r == c) ? piv * is : (r > c ? a[c] * is : R(0));   // column c of L
-    __syncthreads();
+    sh.Lc[c][r] = (r == c) ? piv * is : (r > c ? a[c] * is : R(0));   // column c of L
+    sh.dinv[c] = is;
+    __hip_atomic_store(&sh.published, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   if (bad && report && r == 0) info[0] = 1;
 }
 
-// x[j] -= xm * Lc[m][j] for j in (m, kTile): the LDS reads are fenced into groups of 8 so that
-// the scheduler does not hoist a whole column of loads (register pressure, spills).
+// Consumer step over columns M .. M+3 of L_kk once they are published: the 4x4 triangle
+// x[M+m] = (x[M+m] - sum_{q<m} x[M+q] L[M+m][M+q]) / L[M+m][M+m], then
+// x[j] -= sum_m x[M+m] L[j][M+m] for every j > M+3.  The LDS reads of the trailing update are
+// fenced into groups of 8 j so that the scheduler does not hoist them all (register pressure).
 template <typename R, int M>
-__device__ __forceinline__ void column_update(R (&x)[kTile], R xm, const R* __restrict__ col) {
+__device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh, unsigned& late) {
+  late |= wait_flag(&sh.published, M + 4) ? 0u : 1u;
 #pragma unroll
-  for (int j0 = M + 1; j0 < kTile; j0 += 8) {
+  for (int m = 0; m < 4; ++m) {
+    R v = x[M + m];
 #pragma unroll
-    for (int j = j0; j < j0 + 8 && j < kTile; ++j) x[j] -= xm * col[j];
+    for (int q = 0; q < m; ++q) v -= x[M + q] * sh.Lc[M + q][M + m];
+    x[M + m] = v * sh.dinv[M + m];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int j0 = M + 4; j0 < kTile; j0 += 8) {
+#pragma unroll
+    for (int j = j0; j < j0 + 8 && j < kTile; ++j)
+      x[j] -= x[M] * sh.Lc[M][j] + x[M + 1] * sh.Lc[M + 1][j] + x[M + 2] * sh.Lc[M + 2][j] +
+              x[M + 3] * sh.Lc[M + 3][j];
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-template <typename R, int M>
-__device__ __forceinline__ void consume_column(R (&x)[kTile], R (*Lc)[kLd]) {
-  __syncthreads();                               // column M of L is published
-  x[M] *= fast_rcp(Lc[M][M]);
-  column_update<R, M>(x, x[M], Lc[M]);
-}
-
+// All 64 columns; returns false if a wait gave up.
 template <typename R, int... Ms>
-__device__ __forceinline__ void consume_all(R (&x)[kTile], R (*Lc)[kLd], std::integer_sequence<int, Ms...>) {
-  (consume_column<R, Ms>(x, Lc), ...);
+__device__ __forceinline__ bool consume_all(R (&x)[kTile], PanelShared<R>& sh, std::integer_sequence<int, Ms...>) {
+  unsigned late = 0;
+  (consume_block4<R, 4 * Ms>(x, sh, late), ...);
+  return late == 0;
 }
 
 // Consumer, off-diagonal block: L_ik = A_ik L_kk^-T, i.e. x L_kk^T = a for each of the 64 rows
 // (lane q = row q), right-looking as the columns of L arrive: x_m = acc_m / L[m][m], then
 // acc_j -= x_m L[j][m] for j > m.
 template <typename R>
-__device__ void solve_rows(R* __restrict__ A, int64_t lda, int64_t bi, int64_t bk, R (*Lc)[kLd]) {
+__device__ bool solve_rows(R* __restrict__ A, int64_t lda, int64_t bi, int64_t bk, PanelShared<R>& sh) {
   const int q = threadIdx.x - kWave;
   R x[kTile];
 #pragma unroll
   for (int c = 0; c < kTile; ++c) x[c] = A[(bi + q) + (bk + c) * lda];
-  consume_all<R>(x, Lc, std::make_integer_sequence<int, kTile>{});
+  const bool ok = consume_all<R>(x, sh, std::make_integer_sequence<int, kTile / 4>{});
 #pragma unroll
   for (int c = 0; c < kTile; ++c) A[(bi + q) + (bk + c) * lda] = x[c];
+  return ok;
 }
 
 // Consumer, diagonal block: X = L_kk^-1 by columns (lane j = column j), right-looking as the
 // columns of L arrive; X goes to Xs (row-major) for the caller.
 template <typename R>
-__device__ void invert_columns(R (*Lc)[kLd], R (*Xs)[kLd]) {
+__device__ bool invert_columns(PanelShared<R>& sh, R (*Xs)[kLd]) {
   const int j = threadIdx.x - kWave;
   R x[kTile];
 #pragma unroll
   for (int m = 0; m < kTile; ++m) x[m] = m == j ? R(1) : R(0);
-  consume_all<R>(x, Lc, std::make_integer_sequence<int, kTile>{});
+  const bool ok = consume_all<R>(x, sh, std::make_integer_sequence<int, kTile / 4>{});
 #pragma unroll
   for (int m = 0; m < kTile; ++m) Xs[m][j] = x[m];
+  return ok;
 }
 
 // One block per tile row i >= k.  Wave 0 factors the diagonal tile and publishes L column by
 // column; wave 1 consumes the columns as they arrive: block 0 inverts L_kk (kept in Ld for the
-// back-substitution, and y_k = L_kk^-1 b_k), every other block solves its tile L_ik.
+// back-substitution, and y_k = L_kk^-1 b_k), every other block solves its tile L_ik.  The only
+// block barrier is the one after the counter is cleared.
 template <typename R>
 __global__ void __launch_bounds__(128)
 chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
            int* __restrict__ info) {
-  __shared__ R Lc[kTile][kLd];     // Lc[c][r] = L_kk[r][c]
+  __shared__ PanelShared<R> sh;
   __shared__ R Xs[kTile][kLd];     // L_kk^-1 (block 0)
   const int t = threadIdx.x;
   const int i = k + blockIdx.x;
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
-  // both waves take exactly kTile barriers below (one per column), wave-uniform
+  if (t == 0) sh.published = 0;
+  __syncthreads();
   if (t < kWave) {
-    factor_wave<R>(A, lda, k, Lc, blockIdx.x == 0, info);
+    factor_wave<R>(A, lda, k, sh, blockIdx.x == 0, info);
     return;
   }
   if (blockIdx.x > 0) {
-    solve_rows<R>(A, lda, bi, bk, Lc);
+    if (!solve_rows<R>(A, lda, bi, bk, sh) && t == kWave) info[0] = 1;
     return;
   }
-  invert_columns<R>(Lc, Xs);
+  const bool ok = invert_columns<R>(sh, Xs);
+  if (!ok && t == kWave) info[0] = 1;
   // wave 1 alone from here (its own LDS writes, in order): L_kk^-1 column-major into Ld, and
   // y_k = L_kk^-1 b_k
   const int u = t - kWave;
