@@ -68,9 +68,8 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
       HSL2_LAUNCH_CHECK();
     }
     if (schur) {
-      hipLaunchKernelGGL(ba_frame_reduce, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii,
-                         eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
-                         blob + p->o_frame_pose, HW, 0.f, Q, wv, Eall);
+      launch_frame_reduce(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
+                          blob + p->o_frame_pose, HW, static_cast<int>(F), 0.f, Q, wv, Eall, s);
       HSL2_LAUNCH_CHECK();
       const int nt = static_cast<int>(p->NT);
       if (nt > 0) {

@@ -168,11 +168,11 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   EdgeGeom g;
   load_edge_geom(poses, intr, intr_stride, ix, jx, g);
 
-  // Per-pixel products are formed in FP32; their sums are kept in FP64 (the FP64 VALU
-  // rate equals the FP32 rate on this part, so this costs registers, not throughput).
-  double acc[90];
+  // Per-pixel products and each thread's partial sums (HW / 256 pixels, 12 at 48x64) are
+  // formed in FP32; the block reduction of the partial sums runs in FP64.
+  float acc[90];
 #pragma unroll
-  for (int k = 0; k < 90; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 90; ++k) acc[k] = 0.f;
 
   const float* dsp = disps + (int64_t)ix * HW;
   const float* tg = targets + (int64_t)e * 2 * HW;
@@ -218,11 +218,11 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
       for (int n = 0; n < 12; ++n) {
         const float wun = ru.w * Ju[n], wvn = rv.w * Jv[n];
 #pragma unroll
-        for (int m = 0; m <= n; ++m) acc[tri(n, m)] += static_cast<double>(wun * Ju[m] + wvn * Jv[m]);
+        for (int m = 0; m <= n; ++m) acc[tri(n, m)] += wun * Ju[m] + wvn * Jv[m];
       }
       const float wru = ru.w * ru.r, wrv = rv.w * rv.r;
 #pragma unroll
-      for (int n = 0; n < 12; ++n) acc[78 + n] += static_cast<double>(wru * Ju[n] + wrv * Jv[n]);
+      for (int n = 0; n < 12; ++n) acc[78 + n] += wru * Ju[n] + wrv * Jv[n];
     }
   }
 
@@ -303,6 +303,14 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
 #pragma unroll
     for (int n = 0; n < 6; ++n) Eall[((int64_t)p * 6 + n) * HW + k] = static_cast<float>(ei[n]);
   }
+}
+
+inline void launch_frame_reduce(const float* Cii, const float* bz, const float* Eii, const float* eta,
+                                int eta_rows, const int* frame_eptr, const int* frame_edges,
+                                const int* frame_pose, int HW, int F, float c_eps, float* Q, float* w,
+                                float* Eall, hipStream_t s) {
+  hipLaunchKernelGGL(ba_frame_reduce, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii, eta, eta_rows,
+                     frame_eptr, frame_edges, frame_pose, HW, c_eps, Q, w, Eall);
 }
 
 // ---- 3. Schur Gram blocks and Ev, one wave per task --------------------------

@@ -52,10 +52,6 @@ __device__ __forceinline__ void tile_to_lds(const R* __restrict__ src, int64_t l
   }
 }
 
-// Thread t of a 256-thread block owns rows 4*(t>>4) .. +3 and columns 4*(t&15) .. +3 of a
-// 64x64 tile, in registers.  Register arrays are only indexed with compile-time indices;
-// a runtime column or row is selected with unrolled predicated loops.
-
 // 1/x and 1/sqrt(x) from the hardware estimates refined by Newton steps (two for double,
 // reaching full precision from the ~2^-26 estimate; one for float), instead of the IEEE
 // division / square-root sequences, which sit on the factorisation's critical path.
@@ -293,7 +289,10 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
   tile_to_lds<R>(A + bi + bk * lda, lda, Li);
   tile_to_lds<R>(A + bj + bk * lda, lda, Lj);
   __syncthreads();
-  const int tr = t >> 4, tc = t & 15;
+  // thread (tc, tr) owns rows tc + 16 p and columns tr + 16 q of the tile: within a wave the
+  // Li reads hit 16 consecutive padded rows (distinct banks) and the Lj reads 4 broadcast
+  // addresses; the global read-modify-write covers 16 consecutive rows per column
+  const int tc = t & 15, tr = t >> 4;
   R acc[4][4];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
@@ -304,8 +303,8 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
     R a[4], bb[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-      a[p] = Li[4 * tr + p][kk];
-      bb[p] = Lj[4 * tc + p][kk];
+      a[p] = Li[tc + 16 * p][kk];
+      bb[p] = Lj[tr + 16 * p][kk];
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -316,11 +315,11 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int p = 0; p < 4; ++p) old[p][q] = A[(bi + 4 * tr + p) + (bj + 4 * tc + q) * lda];
+    for (int p = 0; p < 4; ++p) old[p][q] = A[(bi + tc + 16 * p) + (bj + tr + 16 * q) * lda];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int p = 0; p < 4; ++p) A[(bi + 4 * tr + p) + (bj + 4 * tc + q) * lda] = old[p][q] - acc[p][q];
+    for (int p = 0; p < 4; ++p) A[(bi + tc + 16 * p) + (bj + tr + 16 * q) * lda] = old[p][q] - acc[p][q];
 }
 
 template <typename R>
@@ -328,18 +327,23 @@ __global__ void __launch_bounds__(256)
 chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __restrict__ y,
           R* __restrict__ x, int k) {
   __shared__ R Li[kTile][kLd];
+  __shared__ R Lt[kTile][kLd];
   __shared__ R ys[kTile];
   __shared__ R xs[kTile];
   __shared__ R part[4][kTile];
   const int t = threadIdx.x;
   const int64_t bk = (int64_t)k * kTile;
   const R* Lk = Ld + (int64_t)k * kTile * kTile;     // L_kk^-1, column-major
+  // block j + 1 also stages tile (k, j) of L (coalesced along its rows) for y_j -= L_kj^T x_k
+  const int j = blockIdx.x - 1;
+  const int64_t bj = (int64_t)j * kTile;
   tile_to_lds<R>(Lk, kTile, Li);
+  if (blockIdx.x > 0) tile_to_lds<R>(A + bk + bj * lda, lda, Lt);
   if (t < kTile) ys[t] = y[bk + t];
   __syncthreads();
+  const int c = t & 63, rg = t >> 6;
   // x_k = L_kk^-T y_k = (L_kk^-1)^T y_k
   {
-    const int c = t & 63, rg = t >> 6;
     R acc = 0;
     for (int r = rg; r < kTile; r += 4) acc += Li[r][c] * ys[r];
     part[rg][c] = acc;
@@ -351,13 +355,9 @@ chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __r
     if (t < kTile) x[bk + t] = xs[t];
     return;
   }
-  // y_j -= L_kj^T x_k for the block's tile column j < k
-  const int j = blockIdx.x - 1;
-  const int64_t bj = (int64_t)j * kTile;
-  const int c = t & 63, rg = t >> 6;
   R acc = 0;
 #pragma unroll
-  for (int r = rg; r < kTile; r += 4) acc += A[(bk + r) + (bj + c) * lda] * xs[r];
+  for (int r = rg; r < kTile; r += 4) acc += Lt[r][c] * xs[r];
   __syncthreads();
   part[rg][c] = acc;
   __syncthreads();

@@ -69,17 +69,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Reduce V per-thread values over a block of NT threads (NT a multiple of 64, <= 1024)
-// through LDS in chunks of 16 values; the block writes the V sums to out[0..V) (plain
-// stores).  `lds` must hold 16 * NT elements.  Every thread of the block must call this.
-template <int V, int NT, typename T>
-__device__ void block_reduce_store(const T (&vals)[V], T* lds, T* out) {
+// through LDS in chunks of 16 values, in the type T of `lds` and `out` (the per-thread values
+// are converted on the way in); the block writes the V sums to out[0..V) (plain stores).  `lds` must hold 16 * NT elements.  Every thread of the block must call this.
+template <int V, int NT, typename T, typename S>
+__device__ void block_reduce_store(const S (&vals)[V], T* lds, T* out) {
   constexpr int C = 16;
   const int t = threadIdx.x;
 #pragma unroll
   for (int c0 = 0; c0 < V; c0 += C) {
 #pragma unroll
     for (int c = 0; c < C; ++c)
-      if (c0 + c < V) lds[c * NT + t] = vals[c0 + c];
+      if (c0 + c < V) lds[c * NT + t] = static_cast<T>(vals[c0 + c]);
     __syncthreads();
     // NT threads: value c = t / (NT/16), segment = t % (NT/16)
     constexpr int SEG = NT / C;                 // threads per value

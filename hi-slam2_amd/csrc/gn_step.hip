@@ -553,9 +553,9 @@ extern "C" int hsl2_gn_step(const hsl2_ba_plan* p, float* poses, float* disps,
     HSL2_LAUNCH_CHECK();
   }
   if (schur) {
-    hipLaunchKernelGGL(ba_frame_reduce, dim3(blocks_for(HW), F), dim3(256), 0, s, v.Cii, v.bz, v.Eii,
-                       eta, static_cast<int>(F), v.blob + p->o_frame_eptr, v.blob + p->o_frame_edges,
-                       v.blob + p->o_frame_pose, HW, 1e-7f, v.Q, v.w, v.Eall);
+    launch_frame_reduce(v.Cii, v.bz, v.Eii, eta, static_cast<int>(F), v.blob + p->o_frame_eptr,
+                        v.blob + p->o_frame_edges, v.blob + p->o_frame_pose, HW, static_cast<int>(F), 1e-7f,
+                        v.Q, v.w, v.Eall, s);
     HSL2_LAUNCH_CHECK();
     if (int st = launch_gram(p, v, v.w, 0, s)) return st;
   }
