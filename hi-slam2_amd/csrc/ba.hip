@@ -8,9 +8,11 @@
 //                        (src/droid_kernels.cu:364-606).
 //   2. ba_frame_reduce   per depth frame: C, w, Q = 1/(C+eta) and the pose-i block
 //                        Ei of the frame's edges (accum_cuda, 1413-1463, 1901-1906).
-//   3. ba_schur_gram     one wave per slot pair of a depth frame:
-//                        S = sum_px E_a Q E_b^T, and Ev = sum_px Q w E
-//                        (EEt6x6 / Ev6x1, 1466-1560).
+//   3. ba_schur_gram_tiles  per depth frame, 16x16 tiles of the slots' Gram matrix
+//                        S = sum_px E_a Q E_b^T on the matrix cores, one wave per row tile
+//                        and four column tiles (FP32 tile sums per 256 pixels, FP64 totals);
+//                        ba_schur_ev, one wave per slot:
+//                        Ev = sum_px Q w E (EEt6x6 / Ev6x1, 1466-1560).
 //   4. ba_assemble       the damped reduced camera system A - S, b - bS in FP64
 //                        (FP32 in the plain mode), written densely in column-major
 //                        order, padded to a multiple of 64 with an identity block
@@ -71,18 +73,13 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
       launch_frame_reduce(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
                           blob + p->o_frame_pose, HW, static_cast<int>(F), 0.f, Q, wv, Eall, s);
       HSL2_LAUNCH_CHECK();
-      const int nt = static_cast<int>(p->NT);
-      if (nt > 0) {
-        const int tblocks = (nt + 3) / 4;
-        const int tb8 = (tblocks + 7) / 8;
-        hipLaunchKernelGGL(ba_schur_gram, dim3(tb8 * 8), dim3(256), 0, s, Eall, Q, wv,
-                           blob + p->o_task, blob + p->o_slot_row, 0, nt, HW, tb8, S, Ev);
-        HSL2_LAUNCH_CHECK();
-      }
+      launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, s);
+      HSL2_LAUNCH_CHECK();
     }
     const int64_t nn = n_pad * n_pad;
     hipLaunchKernelGGL(ba_assemble<R>, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, s,
-                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, 0, A, info);
+                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, 0,
+                       A, info);
     HSL2_LAUNCH_CHECK();
     hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3(static_cast<unsigned>((n_pad + 255) / 256)), dim3(256), 0, s,
                        vs, Ev, blob + p->o_rhs_ptr, blob + p->o_rhs_src, E, P,

@@ -10,6 +10,8 @@
 //   Exp drops V for |phi| <= 1e-4         lietorch Exp with the full left Jacobian V
 //   no clamp                              disps > 10 -> 0, clamp(min = 0.001) (ba.py:104-105)
 #pragma once
+#include <type_traits>
+
 #include "ba_plan.hpp"
 #include "common.hpp"
 
@@ -313,63 +315,214 @@ inline void launch_frame_reduce(const float* Cii, const float* bz, const float* 
                      frame_eptr, frame_edges, frame_pose, HW, c_eps, Q, w, Eall);
 }
 
-// ---- 3. Schur Gram blocks and Ev, one wave per task --------------------------
+// ---- 3. Ev rows, one wave per slot -------------------------------------------
+// Ev[s] = sum_px Q w E_s for the Ev tasks [task0, NT) of the plan's task list (task0 = NPAIR:
+// the pair blocks come from ba_schur_gram_tiles).
 __global__ void __launch_bounds__(256)
-ba_schur_gram(const float* __restrict__ Eall, const float* __restrict__ Q,
-              const float* __restrict__ w, const int* __restrict__ tasks,
-              const int* __restrict__ slot_row, int task0, int NT, int HW, int task_blocks8,
-              double* __restrict__ S, double* __restrict__ Ev) {
+ba_schur_ev(const float* __restrict__ Eall, const float* __restrict__ Q,
+            const float* __restrict__ w, const int* __restrict__ tasks,
+            const int* __restrict__ slot_row, int task0, int NT, int HW, int task_blocks8,
+            double* __restrict__ Ev) {
   // XCD-aware order: the 8 XCDs receive blocks round-robin, so give each XCD a
-  // contiguous run of tasks (tasks of one depth frame share their E rows in L2).
-  // Tasks [task0, NT) are processed (task0 > 0 runs only the trailing Ev tasks).
+  // contiguous run of tasks (tasks of one depth frame share Q and w in L2).
   const int b = blockIdx.x;
   const int tb = (b % 8) * task_blocks8 + b / 8;
   const int lane = threadIdx.x % kWave;
   const int task = task0 + tb * 4 + threadIdx.x / kWave;
   if (task >= NT) return;
-  const int kind = tasks[task * 4 + 0];
   const int sa = tasks[task * 4 + 1];
-  const int sb = tasks[task * 4 + 2];
   const int f = tasks[task * 4 + 3];
   const float* Qf = Q + (int64_t)f * HW;
   const float* Ea = Eall + (int64_t)slot_row[sa] * 6 * HW;
-  if (kind == kTaskPair) {
-    const float* Eb = Eall + (int64_t)slot_row[sb] * 6 * HW;
-    double acc[36];
+  const float* wf = w + (int64_t)f * HW;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = lane; k < HW; k += kWave) {
+    const double qw = static_cast<double>(Qf[k]) * wf[k];
 #pragma unroll
-    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
-    for (int k = lane; k < HW; k += kWave) {
-      const double q = Qf[k];
-      double ea[6], eb[6];
+    for (int n = 0; n < 6; ++n) acc[n] += qw * Ea[n * HW + k];
+  }
 #pragma unroll
-      for (int n = 0; n < 6; ++n) {
-        ea[n] = Ea[n * HW + k] * q;
-        eb[n] = Eb[n * HW + k];
+  for (int n = 0; n < 6; ++n) {
+    const double v = wave_sum(acc[n]);
+    if (lane == n) Ev[(int64_t)sa * 6 + n] = v;
+  }
+}
+
+// ---- 3b. Schur Gram blocks of the slot pairs on the matrix cores ----------------
+// For depth frame f with m slots, G = E_f diag(Q_f) E_f^T is the 6m x 6m Gram matrix of the
+// slots' E rows (row 6a + n = component n of slot a); the Schur pair (a, b >= a) is its 6 x 6
+// block at (6a, 6b).  One wave per item (f, ti, tj0) computes the 16 x 16 tiles (ti, tj),
+// tj0 <= tj < tj0 + kGramCols, straight from the E rows: per 16 pixels, lane l loads 4
+// consecutive pixels (float4) of row-tile row l % 16 (scaled by Q) and of each column tile,
+// and four v_mfma_f32_16x16x4f32 steps consume them (step s takes pixel 4 (l / 16) + s, the
+// same pixel on both operands); the loads of 64 pixels are issued together.  The FP32 tile sums are added to FP64 accumulators every 256
+// pixels.  Only entries with a <= b are written; a diagonal block (a == b) is written from its
+// entries r <= c, mirrored, so it is exactly symmetric.
+// MFMA operand layout (16x16x4, f32): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
+// accumulator element i of lane l is D[4 (l / 16) + i][l % 16].
+typedef float gram_f4 __attribute__((ext_vector_type(4)));
+
+// row[k .. k+3] with elements at or beyond `end` read as zero
+__device__ __forceinline__ float4 gram_load4(const float* __restrict__ row, int k, int end, bool vec) {
+  if (vec) return *reinterpret_cast<const float4*>(row + k);
+  return make_float4(k < end ? row[k] : 0.f, k + 1 < end ? row[k + 1] : 0.f, k + 2 < end ? row[k + 2] : 0.f,
+                     k + 3 < end ? row[k + 3] : 0.f);
+}
+
+__global__ void __launch_bounds__(256)
+ba_schur_gram_tiles(const float* __restrict__ Eall, const float* __restrict__ Q,
+                    const int* __restrict__ items, int n_items, int item_blocks8,
+                    const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
+                    const int* __restrict__ frame_pair0, int HW, double* __restrict__ S) {
+  // XCD-aware order: the 8 XCDs receive blocks round-robin; each gets a contiguous run of
+  // items (items of one frame read the same E rows through that XCD's L2)
+  const int blk = blockIdx.x;
+  const int item = ((blk % 8) * item_blocks8 + blk / 8) * 4 + threadIdx.x / kWave;
+  if (item >= n_items) return;
+  const int kb = 0, ke = HW;
+  const int lane = threadIdx.x % kWave;
+  const int f = items[3 * item], ti = items[3 * item + 1], tj0 = items[3 * item + 2];
+  const int s0 = slot_ptr[f], m = slot_ptr[f + 1] - s0, R = 6 * m;
+  const int nt = (R + 15) / 16;
+  const int tj1 = min(nt, tj0 + kGramCols);
+  // this lane's operand rows.  A row beyond the frame's 6m rows (or a column tile beyond tj1)
+  // reads row 0 of the frame's first slot instead: D row r depends only on A row r and D
+  // column c only on B column c, and those rows and columns of D are never written, so every
+  // load is unconditional and no operand needs masking.
+  const float* base = Eall + (int64_t)slot_row[s0] * 6 * HW;
+  const int ra = 16 * ti + lane % 16;
+  const float* pa = ra < R ? Eall + ((int64_t)slot_row[s0 + ra / 6] * 6 + ra % 6) * HW : base;
+  const float* pb[kGramCols];
+#pragma unroll
+  for (int u = 0; u < kGramCols; ++u) {
+    const int rb = 16 * (tj0 + u) + lane % 16;
+    pb[u] = (tj0 + u < tj1 && rb < R) ? Eall + ((int64_t)slot_row[s0 + rb / 6] * 6 + rb % 6) * HW : base;
+  }
+  const float* qf = Q + (int64_t)f * HW;
+  const bool vec = (HW % 4) == 0 && (kb % 4) == 0;
+  const int kq = 4 * (lane / 16);
+
+  gram_f4 acc[kGramCols];
+  double dacc[kGramCols][4];
+#pragma unroll
+  for (int u = 0; u < kGramCols; ++u) {
+    acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dacc[u][i] = 0.0;
+  }
+  // 64 pixels per step.  Whole 16-byte aligned steps are software-pipelined: the loads of
+  // step i + 1 are issued before the 64 MFMAs of step i.  A ragged tail, or rows that are not
+  // 16-byte aligned, take bounded scalar loads.
+  constexpr int kSub = 2;
+  struct Step { float4 a[kSub], q[kSub], b[kSub][kGramCols]; };
+  auto load = [&](Step& st, int k0, auto v4_tag) {
+    constexpr bool V4 = decltype(v4_tag)::value;
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      const int k = k0 + 16 * j + kq;
+      st.a[j] = gram_load4(pa, k, ke, V4);
+      st.q[j] = gram_load4(qf, k, ke, V4);
+#pragma unroll
+      for (int u = 0; u < kGramCols; ++u) st.b[j][u] = gram_load4(pb[u], k, ke, V4);
+    }
+  };
+  int since_fold = 0;
+  auto compute = [&](const Step& st) {
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      const float4 aj = make_float4(st.a[j].x * st.q[j].x, st.a[j].y * st.q[j].y, st.a[j].z * st.q[j].z,
+                                    st.a[j].w * st.q[j].w);
+#pragma unroll
+      for (int u = 0; u < kGramCols; ++u) {
+        const float4 b = st.b[j][u];
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.x, b.x, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.y, b.y, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.z, b.z, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.w, b.w, acc[u], 0, 0, 0);
       }
-#pragma unroll
-      for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int j = 0; j < 6; ++j) acc[i * 6 + j] += ea[i] * eb[j];
     }
+    if (++since_fold == 256 / (16 * kSub)) {    // every 256 pixels
+      since_fold = 0;
 #pragma unroll
-    for (int k = 0; k < 36; ++k) {
-      const double v = wave_sum(acc[k]);
-      if (lane == k) S[(int64_t)(task) * 36 + k] = v;
+      for (int u = 0; u < kGramCols; ++u) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<double>(acc[u][i]);
+        acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
+      }
     }
-  } else {
-    const float* wf = w + (int64_t)f * HW;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = lane; k < HW; k += kWave) {
-      const double qw = static_cast<double>(Qf[k]) * wf[k];
-#pragma unroll
-      for (int n = 0; n < 6; ++n) acc[n] += qw * Ea[n * HW + k];
-    }
-#pragma unroll
-    for (int n = 0; n < 6; ++n) {
-      const double v = wave_sum(acc[n]);
-      if (lane == n) Ev[(int64_t)sa * 6 + n] = v;
+  };
+  // two register sets used in turn (no copies between them, so no load has to have landed
+  // before the next step's loads are issued)
+  constexpr int kStep = 16 * kSub;
+  int k0 = kb;
+  if (vec && k0 + kStep <= ke) {
+    Step s0, s1;
+    load(s0, k0, std::true_type{});
+    while (true) {
+      const bool more1 = k0 + 2 * kStep <= ke;
+      if (more1) load(s1, k0 + kStep, std::true_type{});
+      __builtin_amdgcn_sched_barrier(0);        // keep the next step's loads ahead of the MFMAs
+      compute(s0);
+      k0 += kStep;
+      if (!more1) break;
+      const bool more0 = k0 + 2 * kStep <= ke;
+      if (more0) load(s0, k0 + kStep, std::true_type{});
+      __builtin_amdgcn_sched_barrier(0);
+      compute(s1);
+      k0 += kStep;
+      if (!more0) break;
     }
   }
+  for (; k0 < ke; k0 += 16 * kSub) {
+    Step st;
+    load(st, k0, std::false_type{});
+    compute(st);
+  }
+#pragma unroll
+  for (int u = 0; u < kGramCols; ++u)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<double>(acc[u][i]);
+
+  const int64_t pair0 = frame_pair0[f];
+#pragma unroll
+  for (int u = 0; u < kGramCols; ++u) {
+    const int tj = tj0 + u;
+    if (tj >= tj1) continue;
+    const int c = 16 * tj + lane % 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * ti + 4 * (lane / 16) + i;
+      if (r >= R || c >= R || r > c) continue;
+      const int a = r / 6, n = r - 6 * a, b = c / 6, n2 = c - 6 * b;
+      const int64_t pair = pair0 + (int64_t)a * m - (int64_t)a * (a - 1) / 2 + (b - a);
+      S[pair * 36 + n * 6 + n2] = dacc[u][i];
+      if (a == b) S[pair * 36 + n2 * 6 + n] = dacc[u][i];
+    }
+  }
+}
+
+// The Ev rows with the weight vector wvec (one wave per slot).
+inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
+                            const float* wvec, double* Ev, hipStream_t s) {
+  const int task0 = static_cast<int>(p->NPAIR);
+  const int nt = static_cast<int>(p->NT) - task0;
+  if (nt <= 0) return;
+  const int tb8 = ((nt + 3) / 4 + 7) / 8;
+  hipLaunchKernelGGL(ba_schur_ev, dim3(tb8 * 8), dim3(256), 0, s, Eall, Q, wvec, blob + p->o_task,
+                     blob + p->o_slot_row, task0, static_cast<int>(p->NT), static_cast<int>(p->HW), tb8, Ev);
+}
+
+// Schur pair blocks (matrix-core tiles) and the Ev rows, for the pair/Ev layout of the plan.
+inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
+                              const float* wvec, double* S, double* Ev, hipStream_t s) {
+  const int HW = static_cast<int>(p->HW);
+  if (p->NGI > 0) {
+    const int ni = static_cast<int>(p->NGI);
+    const int ib8 = ((ni + 3) / 4 + 7) / 8;
+    hipLaunchKernelGGL(ba_schur_gram_tiles, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_item, ni,
+                       ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, S);
+  }
+  launch_schur_ev(p, blob, Eall, Q, wvec, Ev, s);
 }
 
 // ---- 4. reduced camera system, dense column-major ----------------------------

@@ -494,17 +494,6 @@ Views views(const hsl2_ba_plan* p, char* ws) {
 
 unsigned blocks_for(int64_t n, int t = 256) { return static_cast<unsigned>((n + t - 1) / t); }
 
-int launch_gram(const hsl2_ba_plan* p, const Views& v, const float* wvec, int task0, hipStream_t s) {
-  const int nt = static_cast<int>(p->NT) - task0;
-  if (nt <= 0) return HSL2_OK;
-  const int tblocks = (nt + 3) / 4;
-  const int tb8 = (tblocks + 7) / 8;
-  hipLaunchKernelGGL(ba_schur_gram, dim3(tb8 * 8), dim3(256), 0, s, v.Eall, v.Q, wvec,
-                     v.blob + p->o_task, v.blob + p->o_slot_row, task0, static_cast<int>(p->NT),
-                     static_cast<int>(p->HW), tb8, v.S, v.Ev);
-  HSL2_LAUNCH_CHECK();
-  return HSL2_OK;
-}
 
 int check_common(const hsl2_ba_plan* plan, const void* workspace, size_t bytes) {
   HSL2_REQUIRE(plan, HSL2_ERR_ARGUMENT, "gn_step: null plan");
@@ -557,7 +546,8 @@ extern "C" int hsl2_gn_step(const hsl2_ba_plan* p, float* poses, float* disps,
                         v.blob + p->o_frame_edges, v.blob + p->o_frame_pose, HW, static_cast<int>(F), 1e-7f,
                         v.Q, v.w, v.Eall, s);
     HSL2_LAUNCH_CHECK();
-    if (int st = launch_gram(p, v, v.w, 0, s)) return st;
+    launch_schur_gram(p, v.blob, v.Eall, v.Q, v.w, v.S, v.Ev, s);
+    HSL2_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(ba_assemble<double>, dim3(blocks_for(n_pad * n_pad)), dim3(256), 0, s, v.Hs, v.S,
                      v.blob + p->o_blk_ptr, v.blob + p->o_blk_src, E, P, n_pad, lm, ep, schur ? 1 : 2, v.A,
@@ -629,8 +619,10 @@ extern "C" int hsl2_gn_step_backward(const hsl2_ba_plan* p, const float* poses, 
   const int64_t n_pad = p->n_pad;
   // lambda_x = S^-1 (dx_bar - E Q dz_bar): the Ev tasks of the Gram kernel with w := dz_bar,
   // the reduced system re-assembled from the forward's Hs and S, the same tiled Cholesky
-  if (schur)
-    if (int st = launch_gram(p, v, grad_dz, static_cast<int>(p->NPAIR), s)) return st;
+  if (schur) {
+    launch_schur_ev(p, v.blob, v.Eall, v.Q, grad_dz, v.Ev, s);
+    HSL2_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(ba_assemble<double>, dim3(blocks_for(n_pad * n_pad)), dim3(256), 0, s, v.Hs, v.S,
                      v.blob + p->o_blk_ptr, v.blob + p->o_blk_src, E, P, n_pad, lm, ep, schur ? 1 : 2, v.A,
                      v.info);
