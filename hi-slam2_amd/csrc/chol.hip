@@ -220,26 +220,24 @@ __device__ bool invert_columns(PanelShared<R>& sh, R (*Xs)[kLd]) {
   return ok;
 }
 
-// One block per tile row i >= k.  Wave 0 factors the diagonal tile and publishes L column by
-// column; wave 1 consumes the columns as they arrive: block 0 inverts L_kk (kept in Ld for the
+// Panel block for tile row i = k + blk (threads 0..127; the launch's other threads have
+// returned).  Wave 0 factors the diagonal tile and publishes L column by column; wave 1
+// consumes the columns as they arrive: block 0 inverts L_kk (kept in Ld for the
 // back-substitution, and y_k = L_kk^-1 b_k), every other block solves its tile L_ik.  The only
 // block barrier is the one after the counter is cleared.
 template <typename R>
-__global__ void __launch_bounds__(128)
-chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
-           int* __restrict__ info) {
-  __shared__ PanelShared<R> sh;
-  __shared__ R Xs[kTile][kLd];     // L_kk^-1 (block 0)
+__device__ void panel_block(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
+                            int blk, int* __restrict__ info, PanelShared<R>& sh, R (*Xs)[kLd]) {
   const int t = threadIdx.x;
-  const int i = k + blockIdx.x;
+  const int i = k + blk;
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
   if (t == 0) sh.published = 0;
   __syncthreads();
   if (t < kWave) {
-    factor_wave<R>(A, lda, k, sh, blockIdx.x == 0, info);
+    factor_wave<R>(A, lda, k, sh, blk == 0, info);
     return;
   }
-  if (blockIdx.x > 0) {
+  if (blk > 0) {
     if (!solve_rows<R>(A, lda, bi, bk, sh) && t == kWave) info[0] = 1;
     return;
   }
@@ -257,34 +255,11 @@ chol_panel(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b
   b[bk + u] = acc;
 }
 
+// Trailing-update work of step k (256 threads): tile (k+1+ii, k+1+jj) -= L_{k+1+ii,k} L_{k+1+jj,k}^T.
 template <typename R>
-__global__ void __launch_bounds__(256)
-chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
-  __shared__ R Li[kTile][kLd];
-  __shared__ R Lj[kTile][kLd];
+__device__ void update_tile(R* __restrict__ A, int64_t lda, int k, int ii, int jj, R (*Li)[kLd], R (*Lj)[kLd]) {
   const int t = threadIdx.x;
-  const int ntiles = m * (m + 1) / 2;
-  const int task = blockIdx.x;
   const int64_t bk = (int64_t)k * kTile;
-  if (task >= ntiles) {
-    // rhs: b_i -= L_ik y_k
-    const int i = k + 1 + (task - ntiles);
-    const int64_t bi = (int64_t)i * kTile;
-    __shared__ R part[4][kTile];
-    const int r = t & 63, cg = t >> 6;
-    R acc = 0;
-#pragma unroll
-    for (int c = cg; c < kTile; c += 4) acc += A[(bi + r) + (bk + c) * lda] * b[bk + c];
-    part[cg][r] = acc;
-    __syncthreads();
-    if (t < kTile) b[bi + t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
-    return;
-  }
-  // lower-triangular task index -> (ii, jj), jj <= ii
-  int ii = static_cast<int>((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
-  while ((ii + 1) * (ii + 2) / 2 <= task) ++ii;
-  while (ii * (ii + 1) / 2 > task) --ii;
-  const int jj = task - ii * (ii + 1) / 2;
   const int64_t bi = (int64_t)(k + 1 + ii) * kTile, bj = (int64_t)(k + 1 + jj) * kTile;
   tile_to_lds<R>(A + bi + bk * lda, lda, Li);
   tile_to_lds<R>(A + bj + bk * lda, lda, Lj);
@@ -320,6 +295,80 @@ chol_update(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
   for (int q = 0; q < 4; ++q)
 #pragma unroll
     for (int p = 0; p < 4; ++p) A[(bi + tc + 16 * p) + (bj + tr + 16 * q) * lda] = old[p][q] - acc[p][q];
+}
+
+// ... and its rhs part: b_{k+1+ii} -= L_{k+1+ii,k} y_k.
+template <typename R>
+__device__ void update_rhs(const R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int ii,
+                           R (*part)[kTile]) {
+  const int t = threadIdx.x;
+  const int64_t bk = (int64_t)k * kTile, bi = (int64_t)(k + 1 + ii) * kTile;
+  const int r = t & 63, cg = t >> 6;
+  R acc = 0;
+#pragma unroll
+  for (int c = cg; c < kTile; c += 4) acc += A[(bi + r) + (bk + c) * lda] * b[bk + c];
+  part[cg][r] = acc;
+  __syncthreads();
+  if (t < kTile) b[bi + t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
+}
+
+// LDS of a chol_step block: a panel block's shared state and L_kk^-1, or an update block's
+// two L tiles and rhs partial sums.
+template <typename R> union StepShared {
+  struct {
+    PanelShared<R> sh;
+    R Xs[kTile][kLd];
+  } panel;
+  struct {
+    R Li[kTile][kLd];
+    R Lj[kTile][kLd];
+    R part[4][kTile];
+  } update;
+};
+
+// Step k of the factorisation (256-thread blocks).  Blocks [0, T - k) are the panel blocks of
+// tile column k.  For k >= 1 the further blocks apply the rest of step k - 1's trailing update
+// (column k and rhs row k were updated by chol_update_next before this launch): with
+// m = T - k, tiles (ii, jj) of step k - 1 with 1 <= jj <= ii < m, then rhs rows 1 .. m - 1.
+// Neither part touches what the other reads or writes.
+template <typename R>
+__global__ void __launch_bounds__(256)
+chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k, int T,
+          int* __restrict__ info) {
+  __shared__ StepShared<R> smem;
+  const int npanel = T - k;
+  const int blk = blockIdx.x;
+  if (blk < npanel) {
+    if (threadIdx.x >= 2 * kWave) return;
+    panel_block<R>(A, lda, Ld, b, k, blk, info, smem.panel.sh, smem.panel.Xs);
+    return;
+  }
+  const int kp = k - 1, m1 = T - k - 1;           // sub-triangle size of the remaining tiles
+  const int task = blk - npanel;
+  const int ntiles = m1 * (m1 + 1) / 2;
+  if (task >= ntiles) {
+    update_rhs<R>(A, lda, b, kp, 1 + (task - ntiles), smem.update.part);
+    return;
+  }
+  // lower-triangular task index -> (ii, jj), jj <= ii, of the sub-triangle
+  int ii = static_cast<int>((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
+  while ((ii + 1) * (ii + 2) / 2 <= task) ++ii;
+  while (ii * (ii + 1) / 2 > task) --ii;
+  const int jj = task - ii * (ii + 1) / 2;
+  update_tile<R>(A, lda, kp, 1 + ii, 1 + jj, smem.update.Li, smem.update.Lj);
+}
+
+// The part of step k's trailing update that the next panel needs: tiles (ii, 0) of column
+// k + 1 for ii < m = T - k - 1 (one block each), and rhs row k + 1 (block m).
+template <typename R>
+__global__ void __launch_bounds__(256)
+chol_update_next(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
+  __shared__ R Li[kTile][kLd];
+  __shared__ R Lj[kTile][kLd];
+  __shared__ R part[4][kTile];
+  const int task = blockIdx.x;
+  if (task < m) update_tile<R>(A, lda, k, task, 0, Li, Lj);
+  else update_rhs<R>(A, lda, b, k, 0, part);
 }
 
 template <typename R>
@@ -375,11 +424,14 @@ int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_
   HSL2_REQUIRE(n_pad % kTile == 0, HSL2_ERR_SHAPE, "cholesky: size must be a multiple of 64");
   const int T = static_cast<int>(n_pad / kTile);
   for (int k = 0; k < T; ++k) {
-    hipLaunchKernelGGL(chol_panel<R>, dim3(T - k), dim3(128), 0, s, A, n_pad, Ld, b, k, info);
+    // panel k, with the rest of step k - 1's trailing update alongside
+    const int m1 = k > 0 ? T - k - 1 : 0;
+    const int rest = k > 0 ? m1 * (m1 + 1) / 2 + m1 : 0;
+    hipLaunchKernelGGL(chol_step<R>, dim3(T - k + rest), dim3(256), 0, s, A, n_pad, Ld, b, k, T, info);
     HSL2_LAUNCH_CHECK();
     const int m = T - k - 1;
     if (m > 0) {
-      hipLaunchKernelGGL(chol_update<R>, dim3(m * (m + 1) / 2 + m), dim3(256), 0, s, A, n_pad, b, k, m);
+      hipLaunchKernelGGL(chol_update_next<R>, dim3(m + 1), dim3(256), 0, s, A, n_pad, b, k, m);
       HSL2_LAUNCH_CHECK();
     }
   }
