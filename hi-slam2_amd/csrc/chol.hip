@@ -124,11 +124,10 @@ template <typename R> struct PanelShared {
 };
 
 // Producer (wave 0 of every panel block): factor the diagonal tile with ONE wave.  Lane r
-// holds row r in registers; column c of the right-looking factorisation is broadcast from the
-// lanes that hold it (v_readlane), so the column steps need no data exchange through LDS.
-// After step c, column c of L and 1 / L[c][c] go to LDS and `published` is raised with a
-// workgroup-scope release store; the consumer wave follows through that counter, so neither
-// wave waits at a barrier per column.  (Every lane stores dinv[c]: the same value to the
+// holds row r in registers.  At step c the pivot is broadcast from lane c (v_readlane); column
+// c of L and 1 / L[c][c] go to LDS and `published` is raised with a workgroup-scope release
+// store; the rank-1 update of the trailing rows then reads column c back from LDS.  The
+// consumer wave follows through the counter, so neither wave waits at a barrier per column.  (Every lane stores dinv[c]: the same value to the
 // same address, and no per-column branch.)  A non-positive pivot sets info[0] = 1 when
 // `report` and is replaced by 1 so that the arithmetic stays finite.
 template <typename R>
@@ -148,12 +147,14 @@ __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelSh
       piv = R(1);
     }
     const R is = fast_rsqrt(piv);
-    const R t = a[c] * (is * is);                // a[r][c] / pivot
-#pragma unroll
-    for (int s = c + 1; s < kTile; ++s) a[s] -= t * lane_bcast<R>(a[c], s);
-    sh.Lc[c][r] = (r == c) ? piv * is : (r > c ? a[c] * is : R(0));   // column c of L
+    const R l = (r == c) ? piv * is : (r > c ? a[c] * is : R(0));     // L[r][c]
+    sh.Lc[c][r] = l;                             // column c of L
     sh.dinv[c] = is;
     __hip_atomic_store(&sh.published, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // trailing update a[r][s] -= L[r][c] L[s][c], s > c, with L[s][c] read back from the column
+    // just stored (the same address for every lane: LDS broadcast reads, two values per read)
+#pragma unroll
+    for (int s = c + 1; s < kTile; ++s) a[s] -= l * sh.Lc[c][s];
   }
   if (bad && report && r == 0) info[0] = 1;
 }
