@@ -160,11 +160,14 @@ __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelSh
 }
 
 // Consumer step over columns M .. M+3 of L_kk once they are published: the 4x4 triangle
-// x[M+m] = (x[M+m] - sum_{q<m} x[M+q] L[M+m][M+q]) / L[M+m][M+m], then
+// x[M+m] = (x[M+m] - sum_{q<m} x[M+q] L[M+m][M+q]) / L[M+m][M+m] -- final values, handed to
+// `out` at once so their registers are free for the rest of the sweep -- then
 // x[j] -= sum_m x[M+m] L[j][M+m] for every j > M+3.  The LDS reads of the trailing update are
-// fenced into groups of 8 j so that the scheduler does not hoist them all (register pressure).
-template <typename R, int M>
-__device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh, unsigned& late) {
+// fenced into groups of kConsumeGroup j so that the scheduler does not hoist them all.
+constexpr int kConsumeGroup = 16;
+
+template <typename R, int M, typename Out>
+__device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh, unsigned& late, Out& out) {
   late |= wait_flag(&sh.published, M + 4) ? 0u : 1u;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -172,12 +175,13 @@ __device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh
 #pragma unroll
     for (int q = 0; q < m; ++q) v -= x[M + q] * sh.Lc[M + q][M + m];
     x[M + m] = v * sh.dinv[M + m];
+    out(M + m, x[M + m]);
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int j0 = M + 4; j0 < kTile; j0 += 8) {
+  for (int j0 = M + 4; j0 < kTile; j0 += kConsumeGroup) {
 #pragma unroll
-    for (int j = j0; j < j0 + 8 && j < kTile; ++j)
+    for (int j = j0; j < j0 + kConsumeGroup && j < kTile; ++j)
       x[j] -= x[M] * sh.Lc[M][j] + x[M + 1] * sh.Lc[M + 1][j] + x[M + 2] * sh.Lc[M + 2][j] +
               x[M + 3] * sh.Lc[M + 3][j];
     __builtin_amdgcn_sched_barrier(0);
@@ -185,10 +189,11 @@ __device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh
 }
 
 // All 64 columns; returns false if a wait gave up.
-template <typename R, int... Ms>
-__device__ __forceinline__ bool consume_all(R (&x)[kTile], PanelShared<R>& sh, std::integer_sequence<int, Ms...>) {
+template <typename R, typename Out, int... Ms>
+__device__ __forceinline__ bool consume_all(R (&x)[kTile], PanelShared<R>& sh, Out& out,
+                                            std::integer_sequence<int, Ms...>) {
   unsigned late = 0;
-  (consume_block4<R, 4 * Ms>(x, sh, late), ...);
+  (consume_block4<R, 4 * Ms>(x, sh, late, out), ...);
   return late == 0;
 }
 
@@ -201,10 +206,8 @@ __device__ bool solve_rows(R* __restrict__ A, int64_t lda, int64_t bi, int64_t b
   R x[kTile];
 #pragma unroll
   for (int c = 0; c < kTile; ++c) x[c] = A[(bi + q) + (bk + c) * lda];
-  const bool ok = consume_all<R>(x, sh, std::make_integer_sequence<int, kTile / 4>{});
-#pragma unroll
-  for (int c = 0; c < kTile; ++c) A[(bi + q) + (bk + c) * lda] = x[c];
-  return ok;
+  auto out = [&](int c, R v) { A[(bi + q) + (bk + c) * lda] = v; };
+  return consume_all<R>(x, sh, out, std::make_integer_sequence<int, kTile / 4>{});
 }
 
 // Consumer, diagonal block: X = L_kk^-1 by columns (lane j = column j), right-looking as the
@@ -215,10 +218,8 @@ __device__ bool invert_columns(PanelShared<R>& sh, R (*Xs)[kLd]) {
   R x[kTile];
 #pragma unroll
   for (int m = 0; m < kTile; ++m) x[m] = m == j ? R(1) : R(0);
-  const bool ok = consume_all<R>(x, sh, std::make_integer_sequence<int, kTile / 4>{});
-#pragma unroll
-  for (int m = 0; m < kTile; ++m) Xs[m][j] = x[m];
-  return ok;
+  auto out = [&](int m, R v) { Xs[m][j] = v; };
+  return consume_all<R>(x, sh, out, std::make_integer_sequence<int, kTile / 4>{});
 }
 
 // Panel block for tile row i = k + blk (threads 0..127; the launch's other threads have
