@@ -315,16 +315,19 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     sh[it] = x0 - xa;
     const bool yok = p < npix && y >= 0 && y < h2;
     const bool oka = yok && xa >= 0 && xa < w2;
-    const bool okb = yok && xa + 8 >= 0 && xa + 8 < w2;
+    // the second chunk only when the window reaches into it (x0 not a multiple of 8): an
+    // aligned window would otherwise pull in a neighbouring 8x8 tile's line for nothing
+    const bool okb = yok && x0 != xa && xa + 8 >= 0 && xa + 8 < w2;
     const T* pl = vbase + (int64_t)(p < npix ? p : 0) * plane;
     const int offa = tiled ? tiled_offset(y, xa, w2) : y * w2 + xa;
     const int offb = tiled ? tiled_offset(y, xa + 8, w2) : y * w2 + xa + 8;
-    // unconditional 16-byte loads from a clamped address (offset 0 of the plane is always
-    // valid), zeroed afterwards: keeps them plain dwordx4 global loads
-    const uint4 la = *reinterpret_cast<const uint4*>(pl + (oka ? offa : 0));
-    const uint4 lb = *reinterpret_cast<const uint4*>(pl + (okb ? offb : 0));
-    ca[it] = make_uint4(oka ? la.x : 0u, oka ? la.y : 0u, oka ? la.z : 0u, oka ? la.w : 0u);
-    cbv[it] = make_uint4(okb ? lb.x : 0u, okb ? lb.y : 0u, okb ? lb.z : 0u, okb ? lb.w : 0u);
+    // lanes whose chunk lies outside the plane issue no load (exec-masked), so no line is
+    // fetched for them
+    uint4 la = make_uint4(0u, 0u, 0u, 0u), lb = make_uint4(0u, 0u, 0u, 0u);
+    if (oka) la = *reinterpret_cast<const uint4*>(pl + offa);
+    if (okb) lb = *reinterpret_cast<const uint4*>(pl + offb);
+    ca[it] = la;
+    cbv[it] = lb;
   }
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
