@@ -220,11 +220,11 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
       for (int n = 0; n < 12; ++n) {
         const float wun = ru.w * Ju[n], wvn = rv.w * Jv[n];
 #pragma unroll
-        for (int m = 0; m <= n; ++m) acc[tri(n, m)] += wun * Ju[m] + wvn * Jv[m];
+        for (int m = 0; m <= n; ++m) acc[tri(n, m)] = fmaf(wvn, Jv[m], fmaf(wun, Ju[m], acc[tri(n, m)]));
       }
       const float wru = ru.w * ru.r, wrv = rv.w * rv.r;
 #pragma unroll
-      for (int n = 0; n < 12; ++n) acc[78 + n] += wru * Ju[n] + wrv * Jv[n];
+      for (int n = 0; n < 12; ++n) acc[78 + n] = fmaf(wrv, Jv[n], fmaf(wru, Ju[n], acc[78 + n]));
     }
   }
 
