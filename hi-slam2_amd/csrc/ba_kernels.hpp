@@ -179,8 +179,17 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   const float* dsp = disps + (int64_t)ix * HW;
   const float* tg = targets + (int64_t)e * 2 * HW;
   const float* wt = weights + (int64_t)e * 2 * HW;
+  // the next pixel's five loads are issued before this pixel's arithmetic
+  float nd = 0.f, ntu = 0.f, ntv = 0.f, nwu = 0.f, nwv = 0.f;
+  if (threadIdx.x < HW) {
+    const int k = threadIdx.x;
+    nd = dsp[k], ntu = tg[k], ntv = tg[HW + k], nwu = wt[k], nwv = wt[HW + k];
+  }
   for (int k = threadIdx.x; k < HW; k += kLinThreads) {
-    const float d = dsp[k], tu = tg[k], tv = tg[HW + k], wu = wt[k], wv = wt[HW + k];
+    const float d = nd, tu = ntu, tv = ntv, wu = nwu, wv = nwv;
+    const int kn = k + kLinThreads;
+    if (kn < HW) nd = dsp[kn], ntu = tg[kn], ntv = tg[HW + kn], nwu = wt[kn], nwv = wt[HW + kn];
+    __builtin_amdgcn_sched_barrier(0);
     const int row = k / wd;
     const int col = k - row * wd;
     RowJac ru, rv;
