@@ -130,6 +130,82 @@ template <typename R> struct PanelShared {
 // consumer wave follows through the counter, so neither wave waits at a barrier per column.  (Every lane stores dinv[c]: the same value to the
 // same address, and no per-column branch.)  A non-positive pivot sets info[0] = 1 when
 // `report` and is replaced by 1 so that the arithmetic stays finite.
+// Trailing update of producer step C in groups of kProdGroup columns s: the group's LDS
+// broadcast reads are issued, then (fenced) the previous group's FMAs, so one group's reads
+// are in flight while the other's FMAs run.
+constexpr int kProdGroup = 16;
+
+template <typename R, int C, int G>
+__device__ __forceinline__ void prod_read(const PanelShared<R>& sh, R (&v)[kProdGroup]) {
+#pragma unroll
+  for (int i = 0; i < kProdGroup; ++i) {
+    constexpr int s0 = C + 1 + G * kProdGroup;
+    if (s0 + i < kTile) v[i] = sh.Lc[C][s0 + i];
+  }
+}
+
+template <typename R, int C, int G>
+__device__ __forceinline__ void prod_fma(R (&a)[kTile], R l, const R (&v)[kProdGroup]) {
+#pragma unroll
+  for (int i = 0; i < kProdGroup; ++i) {
+    constexpr int s0 = C + 1 + G * kProdGroup;
+    if (s0 + i < kTile) a[s0 + i] -= l * v[i];
+  }
+}
+
+template <typename R, int C, typename Hook, int... Gs>
+__device__ __forceinline__ void prod_update(R (&a)[kTile], R l, const PanelShared<R>& sh, Hook& after_first,
+                                            std::integer_sequence<int, Gs...>) {
+  R v[2][kProdGroup];
+  prod_read<R, C, 0>(sh, v[0]);
+  // group G: issue the reads of G + 1, then the FMAs of G; the hook (the next column's pivot,
+  // which needs a[C + 1] from group 0) shares group 0's scheduling region
+  (([&] {
+     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) prod_read<R, C, Gs + 1>(sh, v[(Gs + 1) & 1]);
+     __builtin_amdgcn_sched_barrier(0);
+     prod_fma<R, C, Gs>(a, l, v[Gs & 1]);
+     if constexpr (Gs == 0) after_first();
+     __builtin_amdgcn_sched_barrier(0);
+   }()),
+   ...);
+}
+
+// Pivot of column C (non-positive or NaN -> 1, flagged) and its reciprocal square root.
+template <typename R, int C>
+__device__ __forceinline__ void prod_pivot(const R (&a)[kTile], R& piv, R& is, bool& bad) {
+  piv = lane_bcast<R>(a[C], C);
+  if (!(piv > R(0))) {
+    bad = true;
+    piv = R(1);
+  }
+  is = fast_rsqrt(piv);
+}
+
+template <typename R, int C>
+__device__ __forceinline__ void prod_column(R (&a)[kTile], PanelShared<R>& sh, bool& bad, R& piv, R& is) {
+  const int r = threadIdx.x;
+  const R l = (r == C) ? piv * is : (r > C ? a[C] * is : R(0));     // L[r][C]
+  sh.Lc[C][r] = l;                               // column C of L
+  sh.dinv[C] = is;
+  __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  // trailing update a[r][s] -= L[r][C] L[s][C], s > C, with L[s][C] read back from the column
+  // just stored (the same address for every lane: LDS broadcast reads, two values per read);
+  // the next column's pivot is formed as soon as a[C + 1] is final
+  if constexpr (C + 1 < kTile) {
+    auto next_pivot = [&] { prod_pivot<R, C + 1>(a, piv, is, bad); };
+    prod_update<R, C>(a, l, sh, next_pivot,
+                      std::make_integer_sequence<int, (kTile - 1 - C + kProdGroup - 1) / kProdGroup>{});
+  }
+}
+
+template <typename R, int... Cs>
+__device__ __forceinline__ void prod_all(R (&a)[kTile], PanelShared<R>& sh, bool& bad,
+                                         std::integer_sequence<int, Cs...>) {
+  R piv, is;
+  prod_pivot<R, 0>(a, piv, is, bad);
+  (prod_column<R, Cs>(a, sh, bad, piv, is), ...);
+}
+
 template <typename R>
 __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh, bool report,
                             int* __restrict__ info) {
@@ -139,23 +215,7 @@ __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelSh
 #pragma unroll
   for (int c = 0; c < kTile; ++c) a[c] = A[(base + r) + (base + c) * lda];
   bool bad = false;
-#pragma unroll
-  for (int c = 0; c < kTile; ++c) {
-    R piv = lane_bcast<R>(a[c], c);
-    if (!(piv > R(0))) {
-      bad = true;
-      piv = R(1);
-    }
-    const R is = fast_rsqrt(piv);
-    const R l = (r == c) ? piv * is : (r > c ? a[c] * is : R(0));     // L[r][c]
-    sh.Lc[c][r] = l;                             // column c of L
-    sh.dinv[c] = is;
-    __hip_atomic_store(&sh.published, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-    // trailing update a[r][s] -= L[r][c] L[s][c], s > c, with L[s][c] read back from the column
-    // just stored (the same address for every lane: LDS broadcast reads, two values per read)
-#pragma unroll
-    for (int s = c + 1; s < kTile; ++s) a[s] -= l * sh.Lc[c][s];
-  }
+  prod_all<R>(a, sh, bad, std::make_integer_sequence<int, kTile>{});
   if (bad && report && r == 0) info[0] = 1;
 }
 
