@@ -222,9 +222,43 @@ __device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelSh
 // Consumer step over columns M .. M+3 of L_kk once they are published: the 4x4 triangle
 // x[M+m] = (x[M+m] - sum_{q<m} x[M+q] L[M+m][M+q]) / L[M+m][M+m] -- final values, handed to
 // `out` at once so their registers are free for the rest of the sweep -- then
-// x[j] -= sum_m x[M+m] L[j][M+m] for every j > M+3.  The LDS reads of the trailing update are
-// fenced into groups of kConsumeGroup j so that the scheduler does not hoist them all.
-constexpr int kConsumeGroup = 16;
+// x[j] -= sum_m x[M+m] L[j][M+m] for every j > M+3, in groups of kConsumeGroup j whose LDS
+// reads are issued one group ahead of the FMAs.
+constexpr int kConsumeGroup = 8;
+
+template <typename R, int M, int G>
+__device__ __forceinline__ void cons_read(const PanelShared<R>& sh, R (&v)[4][kConsumeGroup]) {
+  constexpr int j0 = M + 4 + G * kConsumeGroup;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int i = 0; i < kConsumeGroup; ++i)
+      if (j0 + i < kTile) v[m][i] = sh.Lc[M + m][j0 + i];
+}
+
+template <typename R, int M, int G>
+__device__ __forceinline__ void cons_fma(R (&x)[kTile], const R (&v)[4][kConsumeGroup]) {
+  constexpr int j0 = M + 4 + G * kConsumeGroup;
+#pragma unroll
+  for (int i = 0; i < kConsumeGroup; ++i)
+    if (j0 + i < kTile)
+      x[j0 + i] -= x[M] * v[0][i] + x[M + 1] * v[1][i] + x[M + 2] * v[2][i] + x[M + 3] * v[3][i];
+}
+
+// the trailing update of a consumer step, group by group, the next group's reads in flight
+template <typename R, int M, int... Gs>
+__device__ __forceinline__ void cons_update(R (&x)[kTile], const PanelShared<R>& sh,
+                                            std::integer_sequence<int, Gs...>) {
+  R v[2][4][kConsumeGroup];
+  cons_read<R, M, 0>(sh, v[0]);
+  (([&] {
+     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) cons_read<R, M, Gs + 1>(sh, v[(Gs + 1) & 1]);
+     __builtin_amdgcn_sched_barrier(0);
+     cons_fma<R, M, Gs>(x, v[Gs & 1]);
+     __builtin_amdgcn_sched_barrier(0);
+   }()),
+   ...);
+}
 
 template <typename R, int M, typename Out>
 __device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh, unsigned& late, Out& out) {
@@ -238,14 +272,8 @@ __device__ __forceinline__ void consume_block4(R (&x)[kTile], PanelShared<R>& sh
     out(M + m, x[M + m]);
   }
   __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int j0 = M + 4; j0 < kTile; j0 += kConsumeGroup) {
-#pragma unroll
-    for (int j = j0; j < j0 + kConsumeGroup && j < kTile; ++j)
-      x[j] -= x[M] * sh.Lc[M][j] + x[M + 1] * sh.Lc[M + 1][j] + x[M + 2] * sh.Lc[M + 2][j] +
-              x[M + 3] * sh.Lc[M + 3][j];
-    __builtin_amdgcn_sched_barrier(0);
-  }
+  if constexpr (M + 4 < kTile)
+    cons_update<R, M>(x, sh, std::make_integer_sequence<int, (kTile - 4 - M + kConsumeGroup - 1) / kConsumeGroup>{});
 }
 
 // All 64 columns; returns false if a wait gave up.
