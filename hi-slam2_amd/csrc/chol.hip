@@ -153,11 +153,14 @@ __device__ __forceinline__ void prod_fma(R (&a)[kTile], R l, const R (&v)[kProdG
   }
 }
 
-template <typename R, int C, typename Hook, int... Gs>
-__device__ __forceinline__ void prod_update(R (&a)[kTile], R l, const PanelShared<R>& sh, Hook& after_first,
-                                            std::integer_sequence<int, Gs...>) {
+template <typename R, int C, typename Publish, typename Hook, int... Gs>
+__device__ __forceinline__ void prod_update(R (&a)[kTile], R l, const PanelShared<R>& sh, Publish& publish,
+                                            Hook& after_first, std::integer_sequence<int, Gs...>) {
   R v[2][kProdGroup];
   prod_read<R, C, 0>(sh, v[0]);
+  // the counter is raised after the first reads are issued: its release wait then also covers
+  // them, instead of the reads starting one LDS round trip later
+  publish();
   // group G: issue the reads of G + 1, then the FMAs of G; the hook (the next column's pivot,
   // which needs a[C + 1] from group 0) shares group 0's scheduling region
   (([&] {
@@ -187,14 +190,18 @@ __device__ __forceinline__ void prod_column(R (&a)[kTile], PanelShared<R>& sh, b
   const R l = (r == C) ? piv * is : (r > C ? a[C] * is : R(0));     // L[r][C]
   sh.Lc[C][r] = l;                               // column C of L
   sh.dinv[C] = is;
-  __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  auto publish = [&] {
+    __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
   // trailing update a[r][s] -= L[r][C] L[s][C], s > C, with L[s][C] read back from the column
   // just stored (the same address for every lane: LDS broadcast reads, two values per read);
   // the next column's pivot is formed as soon as a[C + 1] is final
   if constexpr (C + 1 < kTile) {
     auto next_pivot = [&] { prod_pivot<R, C + 1>(a, piv, is, bad); };
-    prod_update<R, C>(a, l, sh, next_pivot,
+    prod_update<R, C>(a, l, sh, publish, next_pivot,
                       std::make_integer_sequence<int, (kTile - 1 - C + kProdGroup - 1) / kProdGroup>{});
+  } else {
+    publish();
   }
 }
 
