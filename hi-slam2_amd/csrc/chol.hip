@@ -456,16 +456,57 @@ chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b,
 }
 
 // The part of step k's trailing update that the next panel needs: tiles (ii, 0) of column
-// k + 1 for ii < m = T - k - 1 (one block each), and rhs row k + 1 (block m).
+// k + 1 for ii < m = T - k - 1, each cut into four 32 x 32 quarters (blocks 4 ii .. 4 ii + 3,
+// so four CUs share a tile's latency), and rhs row k + 1 (block 4 m).  A quarter stages the 32
+// rows of L_{k+1+ii,k} and of L_{k+1,k} it needs; thread (tc, tr) owns rows tc + 16 p and
+// columns tr + 16 q of the quarter.
 template <typename R>
 __global__ void __launch_bounds__(256)
 chol_update_next(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m) {
-  __shared__ R Li[kTile][kLd];
-  __shared__ R Lj[kTile][kLd];
+  __shared__ R Li[kTile / 2][kLd];
+  __shared__ R Lj[kTile / 2][kLd];
   __shared__ R part[4][kTile];
   const int task = blockIdx.x;
-  if (task < m) update_tile<R>(A, lda, k, task, 0, Li, Lj);
-  else update_rhs<R>(A, lda, b, k, 0, part);
+  if (task >= 4 * m) {
+    update_rhs<R>(A, lda, b, k, 0, part);
+    return;
+  }
+  const int ii = task >> 2, rh = task & 1, ch = (task >> 1) & 1;
+  const int t = threadIdx.x;
+  const int64_t bk = (int64_t)k * kTile;
+  const int64_t bi = (int64_t)(k + 1 + ii) * kTile + 32 * rh, bj = (int64_t)(k + 1) * kTile + 32 * ch;
+  // stage 32 rows x 64 columns of each operand (rows along the lanes: coalesced)
+  {
+    R vi[8], vj[8];
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int idx = t + 256 * it, c = idx >> 5, r = idx & 31;
+      vi[it] = A[(bi + r) + (bk + c) * lda];
+      vj[it] = A[(bj + r) + (bk + c) * lda];
+    }
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int idx = t + 256 * it, c = idx >> 5, r = idx & 31;
+      Li[r][c] = vi[it];
+      Lj[r][c] = vj[it];
+    }
+  }
+  __syncthreads();
+  const int tc = t & 15, tr = t >> 4;             // rows tc, tc + 16; columns tr, tr + 16
+  R acc[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll 8
+  for (int kk = 0; kk < kTile; ++kk) {
+    const R a0 = Li[tc][kk], a1 = Li[tc + 16][kk];
+    const R b0 = Lj[tr][kk], b1 = Lj[tr + 16][kk];
+    acc[0][0] += a0 * b0;
+    acc[0][1] += a0 * b1;
+    acc[1][0] += a1 * b0;
+    acc[1][1] += a1 * b1;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) A[(bi + tc + 16 * p) + (bj + tr + 16 * q) * lda] -= acc[p][q];
 }
 
 template <typename R>
@@ -528,7 +569,7 @@ int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_
     HSL2_LAUNCH_CHECK();
     const int m = T - k - 1;
     if (m > 0) {
-      hipLaunchKernelGGL(chol_update_next<R>, dim3(m + 1), dim3(256), 0, s, A, n_pad, b, k, m);
+      hipLaunchKernelGGL(chol_update_next<R>, dim3(4 * m + 1), dim3(256), 0, s, A, n_pad, b, k, m);
       HSL2_LAUNCH_CHECK();
     }
   }
