@@ -300,7 +300,7 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
   p->w_b = take(static_cast<size_t>(p->n_pad) * real);
   p->w_x = take(static_cast<size_t>(p->n_pad) * real);
   p->w_Ld = take(static_cast<size_t>(p->n_pad) * 64 * real);
-  p->w_info = take(4 * sizeof(int32_t));
+  p->w_info = take((4 + p->n_pad / 64) * sizeof(int32_t));
   if (semantics == HSL2_SEM_TORCH) {
     p->w_lz = take(schur ? F * HW * f4 : 0);
     p->w_dpart = take(E * HW * f4);

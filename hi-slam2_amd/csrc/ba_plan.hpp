@@ -59,7 +59,7 @@ struct hsl2_ba_plan {
   size_t w_b = 0;        // f64 or f32 [n_pad]: rhs, then the forward-substitution result
   size_t w_x = 0;        // f64 or f32 [n_pad]: solution
   size_t w_Ld = 0;       // f64 or f32 [n_pad / 64][64][64]: diagonal Cholesky factors
-  size_t w_info = 0;     // int32 [2]: solver info, all-ok flag
+  size_t w_info = 0;     // int32 [4 + n_pad / 64]: solver info, all-ok flag; back-substitution counters
   // torch-semantics step only
   size_t w_lz = 0;       // f32 [F][HW]: depth part of the adjoint solution
   size_t w_dpart = 0;    // f32 [E][HW]: per-edge disparity-state gradients of the adjoint

@@ -16,9 +16,9 @@
 //                  row by row
 //     chol_update  one block per lower trailing tile (i, j), k < j <= i: A_ij -= L_ik L_jk^T,
 //                  plus one block per tile row i > k: b_i -= L_ik y_k
-//   for each tile row k from the bottom:
-//     chol_back    block 0 writes x_k = (L_kk^-1)^T y_k; blocks 1..k recompute x_k and
-//                  apply y_j -= L_kj^T x_k for j < k
+//   back-substitution, one chol_back_all launch: one block per tile row j applies
+//                  y_j -= L_kj^T x_k as each x_k (k > j) is published through an agent-scope
+//                  counter, then writes x_j = (L_jj^-1)^T y_j and publishes it
 //
 // A pivot that is not positive (or NaN) sets info = 1, as Eigen's LLT reports a
 // NumericalIssue; the caller then treats the solution as zero.  Templated on the element
@@ -30,6 +30,8 @@
 namespace hsl2 {
 
 constexpr int kTile = 64;
+// info[0]: factorisation status; info[kBackFlags + j]: x_j published (back-substitution)
+constexpr int kBackFlags = 4;
 constexpr int kLd = kTile + 1;      // padded LDS row
 
 namespace {
@@ -433,6 +435,8 @@ __global__ void __launch_bounds__(256)
 chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k, int T,
           int* __restrict__ info) {
   __shared__ StepShared<R> smem;
+  if (k == 0 && blockIdx.x == 0)                 // clear the back-substitution's tile counters
+    for (int i = threadIdx.x; i < T; i += blockDim.x) info[kBackFlags + i] = 0;
   const int npanel = T - k;
   const int blk = blockIdx.x;
   if (blk < npanel) {
@@ -509,46 +513,68 @@ chol_update_next(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m
     for (int p = 0; p < 2; ++p) A[(bi + tc + 16 * p) + (bj + tr + 16 * q) * lda] -= acc[p][q];
 }
 
+// Back-substitution x = L^-T y in one launch, one block per tile row j (block b takes
+// j = T - 1 - b, so the blocks without dependencies start first; all T blocks are resident
+// together at the sizes used here, T <= 1024 blocks of 256 threads).  Block j keeps y_j in
+// LDS and, for k = T-1 down to j+1, stages tile (k, j) of L, waits until x_k is published
+// (agent-scope acquire of flags[k]; bounded, a timeout sets info[0] = 1) and applies
+// y_j -= L_kj^T x_k; then x_j = (L_jj^-1)^T y_j is written and flags[j] raised with an
+// agent-scope release.  The tile loads of later steps overlap the waits for earlier ones.
+constexpr int kBackSpin = 1 << 22;
+
 template <typename R>
 __global__ void __launch_bounds__(256)
-chol_back(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, R* __restrict__ y,
-          R* __restrict__ x, int k) {
-  __shared__ R Li[kTile][kLd];
+chol_back_all(const R* __restrict__ A, int64_t lda, const R* __restrict__ Ld, const R* __restrict__ y,
+              R* __restrict__ x, int T, int* __restrict__ info) {
   __shared__ R Lt[kTile][kLd];
+  __shared__ R Xj[kTile][kLd];
   __shared__ R ys[kTile];
   __shared__ R xs[kTile];
   __shared__ R part[4][kTile];
-  const int t = threadIdx.x;
-  const int64_t bk = (int64_t)k * kTile;
-  const R* Lk = Ld + (int64_t)k * kTile * kTile;     // L_kk^-1, column-major
-  // block j + 1 also stages tile (k, j) of L (coalesced along its rows) for y_j -= L_kj^T x_k
-  const int j = blockIdx.x - 1;
+  int* flags = info + kBackFlags;
+  const int j = T - 1 - static_cast<int>(blockIdx.x);
+  const int t = threadIdx.x, c = t & 63, rg = t >> 6;
   const int64_t bj = (int64_t)j * kTile;
-  tile_to_lds<R>(Lk, kTile, Li);
-  if (blockIdx.x > 0) tile_to_lds<R>(A + bk + bj * lda, lda, Lt);
-  if (t < kTile) ys[t] = y[bk + t];
-  __syncthreads();
-  const int c = t & 63, rg = t >> 6;
-  // x_k = L_kk^-T y_k = (L_kk^-1)^T y_k
+  tile_to_lds<R>(Ld + (int64_t)j * kTile * kTile, kTile, Xj);   // L_jj^-1, Xj[r][c]
+  if (t < kTile) ys[t] = y[bj + t];
+  bool late = false;
+  for (int k = T - 1; k > j; --k) {
+    const int64_t bk = (int64_t)k * kTile;
+    tile_to_lds<R>(A + bk + bj * lda, lda, Lt);   // Lt[r][c] = L_kj[r][c]
+    if (t < kTile) {
+      int spins = 0;
+      while (__hip_atomic_load(&flags[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (++spins > kBackSpin) {
+          late = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      xs[t] = x[bk + t];
+    }
+    __syncthreads();
+    R acc = 0;
+#pragma unroll 8
+    for (int r = rg; r < kTile; r += 4) acc += Lt[r][c] * xs[r];
+    part[rg][c] = acc;
+    __syncthreads();
+    if (t < kTile) ys[t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    __syncthreads();
+  }
+  __syncthreads();                               // Xj and ys complete (also when no k > j)
   {
     R acc = 0;
-    for (int r = rg; r < kTile; r += 4) acc += Li[r][c] * ys[r];
+#pragma unroll 8
+    for (int r = rg; r < kTile; r += 4) acc += Xj[r][c] * ys[r];
     part[rg][c] = acc;
   }
   __syncthreads();
-  if (t < kTile) xs[t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
-  __syncthreads();
-  if (blockIdx.x == 0) {
-    if (t < kTile) x[bk + t] = xs[t];
-    return;
+  if (t < kTile) {
+    x[bj + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    if (late) info[0] = 1;
   }
-  R acc = 0;
-#pragma unroll
-  for (int r = rg; r < kTile; r += 4) acc += Lt[r][c] * xs[r];
   __syncthreads();
-  part[rg][c] = acc;
-  __syncthreads();
-  if (t < kTile) y[bj + t] -= part[0][t] + part[1][t] + part[2][t] + part[3][t];
+  if (t == 0) __hip_atomic_store(&flags[j], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -573,10 +599,8 @@ int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_
       HSL2_LAUNCH_CHECK();
     }
   }
-  for (int k = T - 1; k >= 0; --k) {
-    hipLaunchKernelGGL(chol_back<R>, dim3(k + 1), dim3(256), 0, s, A, n_pad, Ld, b, x, k);
-    HSL2_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL(chol_back_all<R>, dim3(T), dim3(256), 0, s, A, n_pad, Ld, b, x, T, info);
+  HSL2_LAUNCH_CHECK();
   return HSL2_OK;
 }
 
@@ -613,7 +637,8 @@ using namespace hsl2;
 extern "C" size_t hsl2_spd_solve_workspace_bytes(int dtype, int64_t n) {
   const size_t es = dtype == HSL2_F64 ? 8 : 4;
   const size_t np = static_cast<size_t>((n + 63) / 64) * 64;
-  return align256(np * np * es) + 3 * align256(np * es) + align256(np * 64 * es) + 256;
+  return align256(np * np * es) + 3 * align256(np * es) + align256(np * 64 * es) +
+         align256((4 + np / 64) * sizeof(int));
 }
 
 extern "C" int hsl2_spd_solve(int dtype, const void* A, const void* b, void* x, int64_t n,
