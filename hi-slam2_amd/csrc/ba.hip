@@ -81,7 +81,7 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
                        Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, 0,
                        A, info);
     HSL2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3(static_cast<unsigned>((n_pad + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3(static_cast<unsigned>((n_pad / 6 + 4) / 4)), dim3(256), 0, s,
                        vs, Ev, blob + p->o_rhs_ptr, blob + p->o_rhs_src, E, P,
                        static_cast<int>(n_pad), bvec);
     HSL2_LAUNCH_CHECK();

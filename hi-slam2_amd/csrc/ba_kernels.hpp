@@ -578,23 +578,29 @@ __global__ void __launch_bounds__(256)
 ba_assemble_rhs(const double* __restrict__ vs, const double* __restrict__ Ev,
                 const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
                 int n_pad, R* __restrict__ bvec) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n_pad) return;
-  if (idx >= 6 * P) {
-    bvec[idx] = R(0);
-    return;
+  // one wave per pose (six rows): lanes take the pose's contributions in turn, each lane
+  // sums its share of the six components, then the wave reduces them (fixed order)
+  const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (6 * a >= n_pad) return;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (a < P) {
+    for (int q = rhs_ptr[a] + lane; q < rhs_ptr[a + 1]; q += kWave) {
+      const int code = rhs_src[q];
+      const int kind = code & 3;
+      const int64_t id = code >> 2;
+      const double* src = kind == kRhsVi ? vs + id * 6 : (kind == kRhsVj ? vs + ((int64_t)E + id) * 6 : Ev + id * 6);
+      const double sg = kind == kRhsEv ? -1.0 : 1.0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) v[i] += sg * src[i];
+    }
   }
-  const int a = idx / 6, i = idx % 6;
-  double v = 0;
-  for (int q = rhs_ptr[a]; q < rhs_ptr[a + 1]; ++q) {
-    const int code = rhs_src[q];
-    const int kind = code & 3;
-    const int64_t id = code >> 2;
-    if (kind == kRhsVi) v += vs[id * 6 + i];
-    else if (kind == kRhsVj) v += vs[((int64_t)E + id) * 6 + i];
-    else v -= Ev[id * 6 + i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double tot = wave_sum(v[i]);
+    const int row = 6 * a + i;
+    if (lane == i && row < n_pad) bvec[row] = static_cast<R>(row < 6 * P ? tot : 0.0);
   }
-  bvec[idx] = static_cast<R>(v);
 }
 
 // ---- 6. back-substitution and disparity update -------------------------------

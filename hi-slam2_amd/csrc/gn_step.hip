@@ -553,8 +553,8 @@ extern "C" int hsl2_gn_step(const hsl2_ba_plan* p, float* poses, float* disps,
                      v.blob + p->o_blk_ptr, v.blob + p->o_blk_src, E, P, n_pad, lm, ep, schur ? 1 : 2, v.A,
                      v.info);
   HSL2_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ba_assemble_rhs<double>, dim3(blocks_for(n_pad)), dim3(256), 0, s, v.vs, v.Ev,
-                     v.blob + p->o_rhs_ptr, v.blob + p->o_rhs_src, E, P, static_cast<int>(n_pad), v.b);
+  hipLaunchKernelGGL(ba_assemble_rhs<double>, dim3(static_cast<unsigned>((n_pad / 6 + 4) / 4)), dim3(256), 0, s,
+                     v.vs, v.Ev, v.blob + p->o_rhs_ptr, v.blob + p->o_rhs_src, E, P, static_cast<int>(n_pad), v.b);
   HSL2_LAUNCH_CHECK();
   if (int st = cholesky_solve<double>(v.A, n_pad, v.Ld, v.b, v.x, v.info, s)) return st;
   if (schur) {
