@@ -555,13 +555,34 @@ ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
   const int b = static_cast<int>(col / 6), j = static_cast<int>(col % 6);
   const int64_t blk = (int64_t)a * P + b;
   double vh = 0, vs = 0;
-  for (int q = blk_ptr[blk]; q < blk_ptr[blk + 1]; ++q) {
-    const int code = blk_src[q];
+  // contributions four at a time (their code and value loads in flight together), summed in
+  // list order
+  auto src = [&](int code) {
     const int kind = code & 7;
     const int64_t id = code >> 3;
-    if (kind < 4) vh += Hs[((int64_t)kind * E + id) * 36 + i * 6 + j];
-    else if (kind == kBlkS) vs += S[id * 36 + i * 6 + j];
-    else vs += S[id * 36 + j * 6 + i];
+    if (kind < 4) return Hs + ((int64_t)kind * E + id) * 36 + i * 6 + j;
+    return S + id * 36 + (kind == kBlkS ? i * 6 + j : j * 6 + i);
+  };
+  const int q1 = blk_ptr[blk + 1];
+  int q = blk_ptr[blk];
+  for (; q + 4 <= q1; q += 4) {
+    int code[4];
+    double val[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) code[u] = blk_src[q + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) val[u] = *src(code[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if ((code[u] & 7) < 4) vh += val[u];
+      else vs += val[u];
+    }
+  }
+  for (; q < q1; ++q) {
+    const int code = blk_src[q];
+    const double val = *src(code);
+    if ((code & 7) < 4) vh += val;
+    else vs += val;
   }
   double v = vh - vs;
   // damp_on_H = 0: ba_cuda damps the reduced diagonal (src/droid_kernels.cu:1663-1664);
@@ -617,12 +638,33 @@ ba_backsub(const float* __restrict__ Eall, const float* __restrict__ Q,
   const bool solved = info[0] == 0;
   R acc = 0;
   if (solved) {
-    for (int s = slot_ptr[f]; s < slot_ptr[f + 1]; ++s) {
-      const int a = slot_pose[s];
-      if (t0_skip && a <= 0) continue;            // t0 skip, src/droid_kernels.cu:1572
+    // slots two at a time, their loads in flight together; summed in slot order
+    const int s1 = slot_ptr[f + 1];
+    auto add = [&](int a, const float (&e)[6]) {
+      if (t0_skip && a <= 0) return;              // t0 skip, src/droid_kernels.cu:1572
+#pragma unroll
+      for (int n = 0; n < 6; ++n) acc += static_cast<R>(e[n]) * x[6 * a + n];
+    };
+    int s = slot_ptr[f];
+    for (; s + 2 <= s1; s += 2) {
+      float e[2][6];
+      int a[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        a[u] = slot_pose[s + u];
+        const float* Er = Eall + (int64_t)slot_row[s + u] * 6 * HW + k;
+#pragma unroll
+        for (int n = 0; n < 6; ++n) e[u][n] = Er[(int64_t)n * HW];
+      }
+      add(a[0], e[0]);
+      add(a[1], e[1]);
+    }
+    if (s < s1) {
+      float e[6];
       const float* Er = Eall + (int64_t)slot_row[s] * 6 * HW + k;
 #pragma unroll
-      for (int n = 0; n < 6; ++n) acc += static_cast<R>(Er[(int64_t)n * HW]) * x[6 * a + n];
+      for (int n = 0; n < 6; ++n) e[n] = Er[(int64_t)n * HW];
+      add(slot_pose[s], e);
     }
   }
   const float qv = Q[(int64_t)f * HW + k];
