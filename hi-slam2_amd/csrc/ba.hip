@@ -8,9 +8,10 @@
 //                        (src/droid_kernels.cu:364-606).
 //   2. ba_frame_reduce   per depth frame: C, w, Q = 1/(C+eta) and the pose-i block
 //                        Ei of the frame's edges (accum_cuda, 1413-1463, 1901-1906).
-//   3. ba_schur_gram_tiles  per depth frame, 16x16 tiles of the slots' Gram matrix
-//                        S = sum_px E_a Q E_b^T on the matrix cores, one wave per row tile
-//                        and four column tiles (FP32 tile sums per 256 pixels, FP64 totals);
+//   3. ba_schur_gram_lds per depth frame, 16x16 tiles of the slots' Gram matrix
+//                        S = sum_px E_a Q E_b^T on the matrix cores, one block per group
+//                        pair of row tiles staged in LDS by 64-pixel chunks (FP32 tile
+//                        sums per 256 pixels, FP64 totals);
 //                        ba_schur_ev, one wave per slot:
 //                        Ev = sum_px Q w E (EEt6x6 / Ev6x1, 1466-1560).
 //   4. ba_assemble       the damped reduced camera system A - S, b - bS in FP64

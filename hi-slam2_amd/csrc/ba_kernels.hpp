@@ -326,7 +326,7 @@ inline void launch_frame_reduce(const float* Cii, const float* bz, const float* 
 
 // ---- 3. Ev rows, one wave per slot -------------------------------------------
 // Ev[s] = sum_px Q w E_s for the Ev tasks [task0, NT) of the plan's task list (task0 = NPAIR:
-// the pair blocks come from ba_schur_gram_tiles).
+// the pair blocks come from ba_schur_gram_lds).
 __global__ void __launch_bounds__(256)
 ba_schur_ev(const float* __restrict__ Eall, const float* __restrict__ Q,
             const float* __restrict__ w, const int* __restrict__ tasks,
@@ -360,13 +360,7 @@ ba_schur_ev(const float* __restrict__ Eall, const float* __restrict__ Q,
 // ---- 3b. Schur Gram blocks of the slot pairs on the matrix cores ----------------
 // For depth frame f with m slots, G = E_f diag(Q_f) E_f^T is the 6m x 6m Gram matrix of the
 // slots' E rows (row 6a + n = component n of slot a); the Schur pair (a, b >= a) is its 6 x 6
-// block at (6a, 6b).  One wave per item (f, ti, tj0) computes the 16 x 16 tiles (ti, tj),
-// tj0 <= tj < tj0 + kGramCols, straight from the E rows: per 16 pixels, lane l loads 4
-// consecutive pixels (float4) of row-tile row l % 16 (scaled by Q) and of each column tile,
-// and four v_mfma_f32_16x16x4f32 steps consume them (step s takes pixel 4 (l / 16) + s, the
-// same pixel on both operands); the loads of 64 pixels are issued together.  The FP32 tile sums are added to FP64 accumulators every 256
-// pixels.  Only entries with a <= b are written; a diagonal block (a == b) is written from its
-// entries r <= c, mirrored, so it is exactly symmetric.
+// block at (6a, 6b), computed in 16 x 16 tiles with v_mfma_f32_16x16x4f32.
 // MFMA operand layout (16x16x4, f32): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
 // accumulator element i of lane l is D[4 (l / 16) + i][l % 16].
 typedef float gram_f4 __attribute__((ext_vector_type(4)));
@@ -378,131 +372,168 @@ __device__ __forceinline__ float4 gram_load4(const float* __restrict__ row, int 
                      k + 3 < end ? row[k + 3] : 0.f);
 }
 
+// One block per item (f, ga, gb): the 16-row tiles of groups ga and gb (kGramGroup tiles each)
+// of frame f's Gram matrix, i.e. the tile pairs (ti <= tj) inside group ga when ga == gb and
+// all pairs (ti in ga, tj in gb) otherwise.  Each chunk of kGramKC pixels of the item's rows
+// (at most 2 x 64) and of Q is staged in LDS (pixel-major, rows padded so that fragment reads
+// and staging writes are free of bank conflicts) while the previous chunk is consumed; the
+// next chunk's loads are in flight meanwhile (one barrier per chunk).  Tile pair p goes to
+// wave p % 4; step s of a chunk feeds lane l pixel 4 s + l / 16 of its A row (scaled by Q) and
+// B row.  FP32 tile sums are folded into FP64 every 256 pixels.  Only entries with a <= b are
+// written; a diagonal block (a == b) is written from its entries r <= c, mirrored, so it is
+// exactly symmetric.
+constexpr int kGramKC = 64;
+constexpr int kGramRS = 2 * 16 * kGramGroup + 16;   // LDS row stride (floats), 16 mod 64
+
 __global__ void __launch_bounds__(256)
-ba_schur_gram_tiles(const float* __restrict__ Eall, const float* __restrict__ Q,
-                    const int* __restrict__ items, int n_items, int item_blocks8,
-                    const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
-                    const int* __restrict__ frame_pair0, int HW, double* __restrict__ S) {
-  // XCD-aware order: the 8 XCDs receive blocks round-robin; each gets a contiguous run of
-  // items (items of one frame read the same E rows through that XCD's L2)
+ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
+                  const int* __restrict__ items, int n_items, int item_blocks8,
+                  const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
+                  const int* __restrict__ frame_pair0, int HW, double* __restrict__ S) {
+  __shared__ float Ls[2][kGramKC][kGramRS];
+  __shared__ float qs[2][kGramKC];
   const int blk = blockIdx.x;
-  const int item = ((blk % 8) * item_blocks8 + blk / 8) * 4 + threadIdx.x / kWave;
+  const int item = (blk % 8) * item_blocks8 + blk / 8;  // XCD-aware: runs of consecutive items
   if (item >= n_items) return;
-  const int kb = 0, ke = HW;
-  const int lane = threadIdx.x % kWave;
-  const int f = items[3 * item], ti = items[3 * item + 1], tj0 = items[3 * item + 2];
+  const int t = threadIdx.x, lane = t % kWave;
+  const int wv = __builtin_amdgcn_readfirstlane(t / kWave);
+  const int f = items[3 * item], ga = items[3 * item + 1], gb = items[3 * item + 2];
   const int s0 = slot_ptr[f], m = slot_ptr[f + 1] - s0, R = 6 * m;
   const int nt = (R + 15) / 16;
-  const int tj1 = min(nt, tj0 + kGramCols);
-  // this lane's operand rows.  A row beyond the frame's 6m rows (or a column tile beyond tj1)
-  // reads row 0 of the frame's first slot instead: D row r depends only on A row r and D
-  // column c only on B column c, and those rows and columns of D are never written, so every
-  // load is unconditional and no operand needs masking.
-  const float* base = Eall + (int64_t)slot_row[s0] * 6 * HW;
-  const int ra = 16 * ti + lane % 16;
-  const float* pa = ra < R ? Eall + ((int64_t)slot_row[s0 + ra / 6] * 6 + ra % 6) * HW : base;
-  const float* pb[kGramCols];
+  const int a0 = ga * kGramGroup, a1 = min(nt, a0 + kGramGroup);
+  const int b0 = gb * kGramGroup, b1 = min(nt, b0 + kGramGroup);
+  const bool diag = ga == gb;
+  const int na = 16 * (a1 - a0), nr = na + (diag ? 0 : 16 * (b1 - b0));
+  // this wave's tile pairs: staged row offsets of the A and B tiles, and the tile indices
+  int rowA[4] = {0, 0, 0, 0}, rowB[4] = {0, 0, 0, 0}, tiA[4] = {0, 0, 0, 0}, tjB[4] = {0, 0, 0, 0};
+  int np = 0;
+  {
+    int p = 0;
+    for (int ti = a0; ti < a1; ++ti)
+      for (int tj = diag ? ti : b0; tj < (diag ? a1 : b1); ++tj, ++p)
+        if (p % 4 == wv && np < 4) {
+          rowA[np] = 16 * (ti - a0);
+          rowB[np] = diag ? 16 * (tj - a0) : na + 16 * (tj - b0);
+          tiA[np] = ti;
+          tjB[np] = tj;
+          ++np;
+        }
+    np = __builtin_amdgcn_readfirstlane(np);
 #pragma unroll
-  for (int u = 0; u < kGramCols; ++u) {
-    const int rb = 16 * (tj0 + u) + lane % 16;
-    pb[u] = (tj0 + u < tj1 && rb < R) ? Eall + ((int64_t)slot_row[s0 + rb / 6] * 6 + rb % 6) * HW : base;
+    for (int u = 0; u < 4; ++u) {
+      rowA[u] = __builtin_amdgcn_readfirstlane(rowA[u]);
+      rowB[u] = __builtin_amdgcn_readfirstlane(rowB[u]);
+    }
+  }
+  // staging role: thread t stages pixels [32 h, 32 h + 32) of staged row r; rows beyond the
+  // frame's 6m stage zeros
+  const int r = t & 127, h = t >> 7;
+  const float* src = nullptr;
+  if (r < nr) {
+    const int gr = r < na ? 16 * a0 + r : 16 * b0 + (r - na);
+    if (gr < R) src = Eall + ((int64_t)slot_row[s0 + gr / 6] * 6 + gr % 6) * HW;
   }
   const float* qf = Q + (int64_t)f * HW;
-  const bool vec = (HW % 4) == 0 && (kb % 4) == 0;
-  const int kq = 4 * (lane / 16);
-
-  gram_f4 acc[kGramCols];
-  double dacc[kGramCols][4];
+  const bool vec = (HW % 4) == 0;
+  float4 nx[8];
+  float4 nq = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto load = [&](int k0) {
+    const int kk = k0 + 32 * h;
 #pragma unroll
-  for (int u = 0; u < kGramCols; ++u) {
-    acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 8; ++j) nx[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    nq = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (vec && k0 + kGramKC <= HW) {               // whole chunk inside the row: 16-byte loads
+      if (src) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) nx[j] = *reinterpret_cast<const float4*>(src + kk + 4 * j);
+      }
+      if (t < kGramKC / 4) nq = *reinterpret_cast<const float4*>(qf + k0 + 4 * t);
+    } else {
+      if (src) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) nx[j] = gram_load4(src, kk + 4 * j, HW, false);
+      }
+      if (t < kGramKC / 4) nq = gram_load4(qf, k0 + 4 * t, HW, false);
+    }
+  };
+  auto store = [&](int buf) {
+    if (r < nr) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = 32 * h + 4 * j;
+        Ls[buf][k][r] = nx[j].x;
+        Ls[buf][k + 1][r] = nx[j].y;
+        Ls[buf][k + 2][r] = nx[j].z;
+        Ls[buf][k + 3][r] = nx[j].w;
+      }
+    }
+    if (t < kGramKC / 4) {
+      qs[buf][4 * t] = nq.x;
+      qs[buf][4 * t + 1] = nq.y;
+      qs[buf][4 * t + 2] = nq.z;
+      qs[buf][4 * t + 3] = nq.w;
+    }
+  };
+  double dacc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int i = 0; i < 4; ++i) dacc[u][i] = 0.0;
-  }
-  // 64 pixels per step.  Whole 16-byte aligned steps are software-pipelined: the loads of
-  // step i + 1 are issued before the 64 MFMAs of step i.  A ragged tail, or rows that are not
-  // 16-byte aligned, take bounded scalar loads.
-  constexpr int kSub = 2;
-  struct Step { float4 a[kSub], q[kSub], b[kSub][kGramCols]; };
-  auto load = [&](Step& st, int k0, auto v4_tag) {
-    constexpr bool V4 = decltype(v4_tag)::value;
+  const int nch = (HW + kGramKC - 1) / kGramKC;
+  // NP tile pairs per wave, the same for the whole block: waves with fewer pairs run
+  // copies of a staged tile whose sums are not written
+  auto run = [&](auto np_c) {
+    constexpr int NP = decltype(np_c)::value;
+    gram_f4 acc[NP];
 #pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      const int k = k0 + 16 * j + kq;
-      st.a[j] = gram_load4(pa, k, ke, V4);
-      st.q[j] = gram_load4(qf, k, ke, V4);
+    for (int u = 0; u < NP; ++u) acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      const int buf = c & 1;
+      if (c + 1 < nch) load((c + 1) * kGramKC);
 #pragma unroll
-      for (int u = 0; u < kGramCols; ++u) st.b[j][u] = gram_load4(pb[u], k, ke, V4);
+      for (int st = 0; st < kGramKC / 4; ++st) {
+        const int kr = 4 * st + lane / 16;
+        const float q = qs[buf][kr];
+#pragma unroll
+        for (int u = 0; u < NP; ++u) {
+          const float a = Ls[buf][kr][rowA[u] + lane % 16] * q;
+          const float b = Ls[buf][kr][rowB[u] + lane % 16];
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[u], 0, 0, 0);
+        }
+      }
+      if ((c & 3) == 3 || c + 1 == nch) {         // every 256 pixels
+#pragma unroll
+        for (int u = 0; u < NP; ++u) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<double>(acc[u][i]);
+          acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+      if (c + 1 < nch) store(buf ^ 1);
+      __syncthreads();
     }
   };
-  int since_fold = 0;
-  auto compute = [&](const Step& st) {
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      const float4 aj = make_float4(st.a[j].x * st.q[j].x, st.a[j].y * st.q[j].y, st.a[j].z * st.q[j].z,
-                                    st.a[j].w * st.q[j].w);
-#pragma unroll
-      for (int u = 0; u < kGramCols; ++u) {
-        const float4 b = st.b[j][u];
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.x, b.x, acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.y, b.y, acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.z, b.z, acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aj.w, b.w, acc[u], 0, 0, 0);
-      }
-    }
-    if (++since_fold == 256 / (16 * kSub)) {    // every 256 pixels
-      since_fold = 0;
-#pragma unroll
-      for (int u = 0; u < kGramCols; ++u) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<double>(acc[u][i]);
-        acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
-      }
-    }
-  };
-  // two register sets used in turn (no copies between them, so no load has to have landed
-  // before the next step's loads are issued)
-  constexpr int kStep = 16 * kSub;
-  int k0 = kb;
-  if (vec && k0 + kStep <= ke) {
-    Step s0, s1;
-    load(s0, k0, std::true_type{});
-    while (true) {
-      const bool more1 = k0 + 2 * kStep <= ke;
-      if (more1) load(s1, k0 + kStep, std::true_type{});
-      __builtin_amdgcn_sched_barrier(0);        // keep the next step's loads ahead of the MFMAs
-      compute(s0);
-      k0 += kStep;
-      if (!more1) break;
-      const bool more0 = k0 + 2 * kStep <= ke;
-      if (more0) load(s0, k0 + kStep, std::true_type{});
-      __builtin_amdgcn_sched_barrier(0);
-      compute(s1);
-      k0 += kStep;
-      if (!more0) break;
-    }
+  const int npairs = diag ? (a1 - a0) * (a1 - a0 + 1) / 2 : (a1 - a0) * (b1 - b0);
+  switch ((npairs + 3) / 4) {
+    case 1: run(std::integral_constant<int, 1>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 3: run(std::integral_constant<int, 3>{}); break;
+    default: run(std::integral_constant<int, 4>{}); break;
   }
-  for (; k0 < ke; k0 += 16 * kSub) {
-    Step st;
-    load(st, k0, std::false_type{});
-    compute(st);
-  }
-#pragma unroll
-  for (int u = 0; u < kGramCols; ++u)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<double>(acc[u][i]);
 
   const int64_t pair0 = frame_pair0[f];
 #pragma unroll
-  for (int u = 0; u < kGramCols; ++u) {
-    const int tj = tj0 + u;
-    if (tj >= tj1) continue;
-    const int c = 16 * tj + lane % 16;
+  for (int u = 0; u < 4; ++u) {
+    if (u >= np) continue;
+    const int cidx = 16 * tjB[u] + lane % 16;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = 16 * ti + 4 * (lane / 16) + i;
-      if (r >= R || c >= R || r > c) continue;
-      const int a = r / 6, n = r - 6 * a, b = c / 6, n2 = c - 6 * b;
+      const int ridx = 16 * tiA[u] + 4 * (lane / 16) + i;
+      if (ridx >= R || cidx >= R || ridx > cidx) continue;
+      const int a = ridx / 6, n = ridx - 6 * a, b = cidx / 6, n2 = cidx - 6 * b;
       const int64_t pair = pair0 + (int64_t)a * m - (int64_t)a * (a - 1) / 2 + (b - a);
       S[pair * 36 + n * 6 + n2] = dacc[u][i];
       if (a == b) S[pair * 36 + n2 * 6 + n] = dacc[u][i];
@@ -525,11 +556,11 @@ inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float*
 inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
                               const float* wvec, double* S, double* Ev, hipStream_t s) {
   const int HW = static_cast<int>(p->HW);
-  if (p->NGI > 0) {
-    const int ni = static_cast<int>(p->NGI);
-    const int ib8 = ((ni + 3) / 4 + 7) / 8;
-    hipLaunchKernelGGL(ba_schur_gram_tiles, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_item, ni,
-                       ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, S);
+  if (p->NGGI > 0) {
+    const int ni = static_cast<int>(p->NGGI);
+    const int ib8 = (ni + 7) / 8;
+    hipLaunchKernelGGL(ba_schur_gram_lds, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_group_item,
+                       ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, S);
   }
   launch_schur_ev(p, blob, Eall, Q, wvec, Ev, s);
 }

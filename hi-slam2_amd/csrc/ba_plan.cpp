@@ -247,7 +247,7 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
   rhs.to_csr(P, rptr, rsrc);
   p->o_rhs_ptr = put(rptr);
   p->o_rhs_src = put(rsrc);
-  // Gram tile items in frame order, and each frame's first pair
+  // Gram group items in frame order, and each frame's first pair
   {
     std::vector<int32_t> pair0(F, 0), items;
     int64_t acc = 0;
@@ -256,21 +256,18 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
       const int64_t m = slot_ptr[f + 1] - slot_ptr[f];
       acc += m * (m + 1) / 2;
     }
-    struct Item { int32_t f, ti, tj0, cols; };
-    std::vector<Item> list;
     if (!p->motion_only) {
       for (int64_t f = 0; f < F; ++f) {
         const int64_t m = slot_ptr[f + 1] - slot_ptr[f];
         const int32_t nt = static_cast<int32_t>((6 * m + 15) / 16);
-        for (int32_t ti = 0; ti < nt; ++ti)
-          for (int32_t tj0 = ti; tj0 < nt; tj0 += hsl2::kGramCols)
-            list.push_back({static_cast<int32_t>(f), ti, tj0, std::min(nt - tj0, hsl2::kGramCols)});
+        const int32_t ng = (nt + hsl2::kGramGroup - 1) / hsl2::kGramGroup;
+        for (int32_t ga = 0; ga < ng; ++ga)
+          for (int32_t gb = ga; gb < ng; ++gb) items.insert(items.end(), {static_cast<int32_t>(f), ga, gb});
       }
     }
-    for (const Item& it : list) items.insert(items.end(), {it.f, it.ti, it.tj0});
-    p->NGI = static_cast<int64_t>(list.size());
+    p->NGGI = static_cast<int64_t>(items.size() / 3);
     p->o_frame_pair0 = put(pair0);
-    p->o_gram_item = put(items);
+    p->o_gram_group_item = put(items);
   }
 
   // ---- workspace layout ----

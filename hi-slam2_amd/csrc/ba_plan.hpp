@@ -39,8 +39,8 @@ struct hsl2_ba_plan {
   int64_t o_blk_ptr = 0, o_blk_src = 0;        // [P*P+1], [*]
   int64_t o_rhs_ptr = 0, o_rhs_src = 0;        // [P+1], [*]
   int64_t o_frame_pair0 = 0;                   // [F] first Schur pair of each frame
-  int64_t o_gram_item = 0;                     // [NGI][3] Gram tile items (frame, row tile, first col tile)
-  int64_t NGI = 0;
+  int64_t o_gram_group_item = 0;               // [NGGI][3] Gram group items (frame, row group, column group)
+  int64_t NGGI = 0;
 
   // ---- workspace layout (byte offsets) ----
   size_t w_blob = 0;
@@ -75,7 +75,7 @@ enum { kBlkHii = 0, kBlkHij = 1, kBlkHji = 2, kBlkHjj = 3, kBlkS = 4, kBlkSt = 5
 enum { kRhsVi = 0, kRhsVj = 1, kRhsEv = 2 };
 // gram task kinds
 enum { kTaskPair = 0, kTaskEv = 1 };
-// Gram tiles: a frame's 6m x 6m Gram matrix is cut into 16 x 16 tiles; one item (one wave)
-// covers one row tile and up to kGramCols column tiles
-constexpr int kGramCols = 4;
+// Gram items: a frame's 6m x 6m Gram matrix is cut into 16 x 16 tiles, and the row tiles into
+// groups of kGramGroup; one item (one block) covers the tile pairs of one or two groups
+constexpr int kGramGroup = 4;
 }  // namespace hsl2

@@ -157,28 +157,31 @@ __device__ __forceinline__ void prod_fma(R (&a)[kTile], R l, const R (&v)[kProdG
 
 template <typename R, int C, typename Publish, typename Hook, int... Gs>
 __device__ __forceinline__ void prod_update(R (&a)[kTile], R l, const PanelShared<R>& sh, Publish& publish,
-                                            Hook& after_first, std::integer_sequence<int, Gs...>) {
+                                            Hook& while_reading, std::integer_sequence<int, Gs...>) {
   R v[2][kProdGroup];
   prod_read<R, C, 0>(sh, v[0]);
-  // the counter is raised after the first reads are issued: its release wait then also covers
-  // them, instead of the reads starting one LDS round trip later
+  // the hook (the next column's pivot, which needs no LDS value) runs while the first reads
+  // are in flight; the counter is raised after them, so its release wait also covers the
+  // reads instead of the reads starting one LDS round trip later
+  __builtin_amdgcn_sched_barrier(0);
+  while_reading();
+  __builtin_amdgcn_sched_barrier(0);
   publish();
-  // group G: issue the reads of G + 1, then the FMAs of G; the hook (the next column's pivot,
-  // which needs a[C + 1] from group 0) shares group 0's scheduling region
+  // group G: issue the reads of G + 1, then the FMAs of G
   (([&] {
      if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) prod_read<R, C, Gs + 1>(sh, v[(Gs + 1) & 1]);
      __builtin_amdgcn_sched_barrier(0);
      prod_fma<R, C, Gs>(a, l, v[Gs & 1]);
-     if constexpr (Gs == 0) after_first();
      __builtin_amdgcn_sched_barrier(0);
    }()),
    ...);
 }
 
-// Pivot of column C (non-positive or NaN -> 1, flagged) and its reciprocal square root.
+// Pivot of column C from lane C's value x (non-positive or NaN -> 1, flagged) and its
+// reciprocal square root.
 template <typename R, int C>
-__device__ __forceinline__ void prod_pivot(const R (&a)[kTile], R& piv, R& is, bool& bad) {
-  piv = lane_bcast<R>(a[C], C);
+__device__ __forceinline__ void prod_pivot(R x, R& piv, R& is, bool& bad) {
+  piv = lane_bcast<R>(x, C);
   if (!(piv > R(0))) {
     bad = true;
     piv = R(1);
@@ -196,10 +199,12 @@ __device__ __forceinline__ void prod_column(R (&a)[kTile], PanelShared<R>& sh, b
     __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
   // trailing update a[r][s] -= L[r][C] L[s][C], s > C, with L[s][C] read back from the column
-  // just stored (the same address for every lane: LDS broadcast reads, two values per read);
-  // the next column's pivot is formed as soon as a[C + 1] is final
+  // just stored (the same address for every lane: LDS broadcast reads, two values per read).
+  // The next pivot a[C+1][C+1] - L[C+1][C]^2 needs only lane C+1's own values, so it is formed
+  // while those reads are in flight (lane C+1's updated a[C + 1] is not used again: its L entry
+  // is piv * is)
   if constexpr (C + 1 < kTile) {
-    auto next_pivot = [&] { prod_pivot<R, C + 1>(a, piv, is, bad); };
+    auto next_pivot = [&] { prod_pivot<R, C + 1>(a[C + 1] - l * l, piv, is, bad); };
     prod_update<R, C>(a, l, sh, publish, next_pivot,
                       std::make_integer_sequence<int, (kTile - 1 - C + kProdGroup - 1) / kProdGroup>{});
   } else {
@@ -211,7 +216,7 @@ template <typename R, int... Cs>
 __device__ __forceinline__ void prod_all(R (&a)[kTile], PanelShared<R>& sh, bool& bad,
                                          std::integer_sequence<int, Cs...>) {
   R piv, is;
-  prod_pivot<R, 0>(a, piv, is, bad);
+  prod_pivot<R, 0>(a[0], piv, is, bad);
   (prod_column<R, Cs>(a, sh, bad, piv, is), ...);
 }
 
