@@ -97,4 +97,32 @@ __device__ void block_reduce_store(const S (&vals)[V], T* lds, T* out) {
   }
 }
 
+// ---- stored correlation-pyramid levels -------------------------------------------
+// Element (pixel pix = n * hw1 + p, y, x) of a level with h2 x w2 planes, by layout:
+//   kPlaneRows   row-major planes: pix * h2 w2 + y w2 + x (the reference's layout)
+//   kPlaneTiles  each plane as row-major 8x8 tiles, each tile row-major (h2, w2 multiples of 8)
+//   kPlaneGroups the 64 pixels of a group g = pix / 64 interleaved: group g holds
+//                [y][x / 8][pix % 64][x % 8] (w2 a multiple of 8, hw1 of 64), so that the
+//                neighbouring pixels of a lookup block read neighbouring 16-byte segments
+enum { kPlaneRows = 0, kPlaneTiles = 1, kPlaneGroups = 2 };
+constexpr int kPlaneGroup = 64;
+
+// Layout of level `level` (planes h2 x w2, hw1 pixels per item) in a tiled pyramid
+// (tiled != 0): level 0 in 8x8 tiles, the pooled levels pixel-interleaved; a level whose
+// sizes do not allow it stays row-major.
+__host__ __device__ inline int pyramid_layout(int level, int64_t hw1, int64_t h2, int64_t w2, int tiled) {
+  if (!tiled) return kPlaneRows;
+  if (level > 0 && w2 % 8 == 0 && hw1 % kPlaneGroup == 0) return kPlaneGroups;
+  return (h2 % 8 == 0 && w2 % 8 == 0) ? kPlaneTiles : kPlaneRows;
+}
+
+__host__ __device__ inline int64_t pyramid_offset(int layout, int64_t pix, int y, int x, int w2, int64_t plane) {
+  if (layout == kPlaneGroups)
+    return (pix - pix % kPlaneGroup) * plane +
+           ((int64_t)(y * (w2 >> 3) + (x >> 3)) * kPlaneGroup + pix % kPlaneGroup) * 8 + (x & 7);
+  if (layout == kPlaneTiles)
+    return pix * plane + ((((y >> 3) * (w2 >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7));
+  return pix * plane + (int64_t)y * w2 + x;
+}
+
 }  // namespace hsl2

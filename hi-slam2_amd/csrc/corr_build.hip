@@ -64,12 +64,6 @@ template <> struct Mfma<float> {
   }
 };
 
-// Offset of (y, x) in a plane of width w stored as row-major 8x8 tiles (the layout the
-// pyramid lookup reads when asked for tiled volumes).
-__device__ __forceinline__ int plane_offset(int y, int x, int w) {
-  return (((y >> 3) * (w >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
-}
-
 // Chunk width for vector stores of a row segment of `width` elements.
 __host__ __device__ constexpr int chunk_of(int width) {
   return width % 8 == 0 ? 8 : (width % 4 == 0 ? 4 : (width % 2 == 0 ? 2 : 1));
@@ -82,8 +76,11 @@ struct alignas(CW * sizeof(To)) OutV {
 
 // Pooled level (hierarchical: each level averages 2x2 of the level above it, as repeated
 // avg_pool2d does).  src holds the level above for the tile ([kTP][RS][WS] floats, pitch PS per
-// p); the values are stored to `out` and returned in v, so that the caller can place them in
-// LDS for the next level once every thread has finished reading src.
+// p); the values are stored to `out` (layout `layout`, pyramid_offset) and returned in v, so
+// that the caller can place them in LDS for the next level once every thread has finished
+// reading src.  Thread chunk c covers CW consecutive x of one (p, y): with p the slowest index,
+// or, for pixel-interleaved levels, the fastest (the 32 p of one (y, x) segment are then one
+// contiguous run in memory); pool_stash uses the same split.
 template <int RS, int WS>
 struct Pool {
   static constexpr int RD = RS / 2, WD = WS / 2, CW = chunk_of(WD);
@@ -92,19 +89,30 @@ struct Pool {
   static constexpr int PD = RD * WD + 4;             // pitch per p of this level in LDS
 };
 
+template <int RS, int WS>
+__device__ __forceinline__ void pool_split(int c, bool pfast, int& p, int& y, int& x) {
+  using PL = Pool<RS, WS>;
+  constexpr int WD = PL::WD, CW = PL::CW, PER = PL::RD * WD / CW;
+  const int r = pfast ? c / kTP : c % PER;
+  p = pfast ? c % kTP : c / PER;
+  y = r / (WD / CW);
+  x = (r % (WD / CW)) * CW;
+}
+
 template <typename To, int RS, int WS, int PS>
 __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* __restrict__ out,
                                              int64_t plane0, int64_t plane_stride, int gy0, int gx0,
-                                             int wl, bool tl, float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
+                                             int wl, int layout, float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
   using PL = Pool<RS, WS>;
-  constexpr int WD = PL::WD, CW = PL::CW;
+  constexpr int CW = PL::CW;
   const int t = threadIdx.x;
+  const bool pfast = layout == kPlaneGroups;
 #pragma unroll
   for (int it = 0; it < PL::NIT; ++it) {
     const int c = it * kThreads + t;
     if (PL::NCH % kThreads == 0 || c < PL::NCH) {
-      const int p = c / (PL::RD * WD / CW), r = c % (PL::RD * WD / CW);
-      const int y = r / (WD / CW), x = (r % (WD / CW)) * CW;
+      int p, y, x;
+      pool_split<RS, WS>(c, pfast, p, y, x);
       const float* a = src + p * PS + (2 * y) * WS + 2 * x;
       OutV<To, CW> o;
 #pragma unroll
@@ -112,25 +120,26 @@ __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* 
         v[it][e] = 0.25f * ((a[2 * e] + a[2 * e + 1]) + (a[WS + 2 * e] + a[WS + 2 * e + 1]));
         o.v[e] = from_f<To>(v[it][e]);
       }
-      const int gy = gy0 + y, gx = gx0 + x;
-      const int off = tl ? plane_offset(gy, gx, wl) : gy * wl + gx;
-      *reinterpret_cast<OutV<To, CW>*>(out + (plane0 + p) * plane_stride + off) = o;
+      // an interleaved level has CW == 8 and gx a multiple of 8: the chunk is one segment
+      *reinterpret_cast<OutV<To, CW>*>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl,
+                                                            plane_stride)) = o;
     }
   }
 }
 
 template <int RS, int WS>
-__device__ __forceinline__ void pool_stash(float* __restrict__ dst,
+__device__ __forceinline__ void pool_stash(float* __restrict__ dst, int layout,
                                            const float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
   using PL = Pool<RS, WS>;
   constexpr int WD = PL::WD, CW = PL::CW;
   const int t = threadIdx.x;
+  const bool pfast = layout == kPlaneGroups;
 #pragma unroll
   for (int it = 0; it < PL::NIT; ++it) {
     const int c = it * kThreads + t;
     if (PL::NCH % kThreads == 0 || c < PL::NCH) {
-      const int p = c / (PL::RD * WD / CW), r = c % (PL::RD * WD / CW);
-      const int y = r / (WD / CW), x = (r % (WD / CW)) * CW;
+      int p, y, x;
+      pool_split<RS, WS>(c, pfast, p, y, x);
 #pragma unroll
       for (int e = 0; e < CW; ++e) dst[p * PL::PD + y * WD + x + e] = v[it][e];
     }
@@ -149,7 +158,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   const int t = threadIdx.x;
   // level 0: row segments of 8 elements (one row of an 8x8 block in the tiled layout), one
   // vector store each; consecutive threads read consecutive segments of the LDS tile
-  const bool tile0 = tiled && (h % 8 == 0) && (w % 8 == 0);
+  const int lay0 = pyramid_layout(0, HW, h, w, tiled);
   constexpr int NCH0 = kTP * NQ / 8;
 #pragma unroll
   for (int c0 = 0; c0 < NCH0; c0 += kThreads) {
@@ -160,9 +169,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
     OutV<To, 8> o;
 #pragma unroll
     for (int e = 0; e < 8; ++e) o.v[e] = from_f<To>(src[e]);
-    const int qy = y0 + y, qx = x0 + x;
-    const int off = tile0 ? plane_offset(qy, qx, w) : qy * w + qx;
-    *reinterpret_cast<OutV<To, 8>*>(l0 + (pl0 + p) * HW + off) = o;
+    *reinterpret_cast<OutV<To, 8>*>(l0 + pyramid_offset(lay0, pl0 + p, y0 + y, x0 + x, w, HW)) = o;
   }
   if (levels < 2) {
     __syncthreads();
@@ -177,31 +184,28 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   float* s2 = s1 + kTP * P1::PD;
   {
     float v[P1::NIT][P1::CW];
-    const int hl = h >> 1, wl = w >> 1;
-    pool_compute<To, kRows, WT, SQ>(sout_in, l1, pl0, (int64_t)hl * wl, y0 >> 1, x0 >> 1, wl,
-                                    tiled && hl % 8 == 0 && wl % 8 == 0, v);
+    const int hl = h >> 1, wl = w >> 1, lay = pyramid_layout(1, HW, hl, wl, tiled);
+    pool_compute<To, kRows, WT, SQ>(sout_in, l1, pl0, (int64_t)hl * wl, y0 >> 1, x0 >> 1, wl, lay, v);
     __syncthreads();                               // every read of the f32 tile is done
     if (levels < 3) return;
-    pool_stash<kRows, WT>(s1, v);
+    pool_stash<kRows, WT>(s1, lay, v);
   }
   __syncthreads();
   {
     float v[P2::NIT][P2::CW];
-    const int hl = h >> 2, wl = w >> 2;
-    pool_compute<To, kRows / 2, WT / 2, P1::PD>(s1, l2, pl0, (int64_t)hl * wl, y0 >> 2, x0 >> 2, wl,
-                                                tiled && hl % 8 == 0 && wl % 8 == 0, v);
+    const int hl = h >> 2, wl = w >> 2, lay = pyramid_layout(2, HW, hl, wl, tiled);
+    pool_compute<To, kRows / 2, WT / 2, P1::PD>(s1, l2, pl0, (int64_t)hl * wl, y0 >> 2, x0 >> 2, wl, lay, v);
     if (levels < 4) {
       __syncthreads();
       return;
     }
-    pool_stash<kRows / 2, WT / 2>(s2, v);           // disjoint from s1: no barrier needed first
+    pool_stash<kRows / 2, WT / 2>(s2, lay, v);      // disjoint from s1: no barrier needed first
   }
   __syncthreads();
   {
     float v[P3::NIT][P3::CW];
-    const int hl = h >> 3, wl = w >> 3;
-    pool_compute<To, kRows / 4, WT / 4, P2::PD>(s2, l3, pl0, (int64_t)hl * wl, y0 >> 3, x0 >> 3, wl,
-                                                tiled && hl % 8 == 0 && wl % 8 == 0, v);
+    const int hl = h >> 3, wl = w >> 3, lay = pyramid_layout(3, HW, hl, wl, tiled);
+    pool_compute<To, kRows / 4, WT / 4, P2::PD>(s2, l3, pl0, (int64_t)hl * wl, y0 >> 3, x0 >> 3, wl, lay, v);
   }
   __syncthreads();
 }

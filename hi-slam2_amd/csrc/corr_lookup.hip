@@ -40,17 +40,17 @@ template <typename T, typename A> __device__ __forceinline__ T store_acc(A v) {
 struct LevelTable {
   const void* vol[4];
   int h2[4], w2[4];
-  int tiled[4];            // 1: plane stored in 8x8 tiles (see tiled_offset)
+  int layout[4];           // kPlaneRows / kPlaneTiles / kPlaneGroups (pyramid_offset)
   float scale[4];          // coordinate scale of the level (1 / 2^level)
   int64_t out_offset[4];   // element offset of the level's first output channel
 };
 
-// Offset of (y, x) inside an h2 x w2 plane stored as row-major 8x8 tiles, each tile
-// row-major (64 contiguous elements).  A radius-3 window (8x8 taps) then touches at most
-// four 64-element tiles instead of eight plane rows.
-__device__ __forceinline__ int tiled_offset(int y, int x, int w2) {
-  return (((y >> 3) * (w2 >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7);
-}
+// A block's pixels pix0 .. pix0 + 63 (pix0 a multiple of 64) are one pixel group of an
+// interleaved level, so tap (p, y, x) of the block lies at pyramid_offset(layout, p, y, x, ...)
+// from the block's base (n * hw1 + pix0) * plane in every layout.  In the tiled layouts a
+// radius-3 window (8x8 taps) touches at most four 8x8 tiles instead of eight plane rows, and
+// the pooled levels' 16-byte row segments of neighbouring pixels share 128-byte lines.
+static_assert(kPix == kPlaneGroup, "lookup blocks must coincide with pixel groups");
 
 // Coordinates of pixel p of item n: x = coords[n * cb + p * cp], y = x-address + cy.
 // Output: out + n * out_batch_stride + level offset + o * hw1 + pix, o = a * (2r+1) + b.
@@ -68,7 +68,7 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   const int level = blockIdx.y;
   const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
   const int h2 = lv.h2[level], w2 = lv.w2[level];
-  const bool tiled = lv.tiled[level] != 0;
+  const int layout = lv.layout[level];
   const float scale = lv.scale[level];
   const int n = blockIdx.x / tiles_per_item;
   const int pix0 = (blockIdx.x % tiles_per_item) * kPix;
@@ -109,7 +109,7 @@ corr_lookup_fwd_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     const int x1 = s_x0[p] + tx;
     A v = 0;
     if (y1 >= 0 && y1 < h2 && x1 >= 0 && x1 < w2)
-      v = load_acc<T>(vbase + p * plane + (tiled ? tiled_offset(y1, x1, w2) : y1 * w2 + x1));
+      v = load_acc<T>(vbase + pyramid_offset(layout, p, y1, x1, w2, plane));
     taps[p * TS + tap] = v;
   }
   __syncthreads();
@@ -154,7 +154,7 @@ corr_lookup_win_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   const int level = blockIdx.y;
   const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
   const int h2 = lv.h2[level], w2 = lv.w2[level];
-  const bool tiled = lv.tiled[level] != 0;
+  const int layout = lv.layout[level];
   const float scale = lv.scale[level];
   const int n = blockIdx.x / tiles_per_item;
   const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
@@ -194,16 +194,14 @@ corr_lookup_win_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   for (int j = 0; j < PPW; ++j) {
     const int p = wave + 4 * j;
     const int xs = s_x0[p], ys = s_y0[p];
-    const T* pl = vbase + (p < npix ? p : 0) * plane;
 #pragma unroll
     for (int kt = 0; kt < KT; ++kt) {
       const int k = lane + 64 * kt;
       const int ty = k / D, tx = k - ty * D;    // constant divisor
       const int y = ys + ty, x = xs + tx;
       const bool ok = p < npix && k < D2 && (unsigned)y < (unsigned)h2 && (unsigned)x < (unsigned)w2;
-      const int off = tiled ? (((((y >> 3) * (w2 >> 3)) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7))
-                            : y * w2 + x;
-      v[j][kt] = load_acc<T>(pl + (ok ? off : 0));   // out-of-plane taps read element 0, then zero
+      // out-of-plane taps read the block's first element, then zero
+      v[j][kt] = load_acc<T>(vbase + (ok ? pyramid_offset(layout, p, y, x, w2, plane) : 0));
       okv[j][kt] = ok;
     }
   }
@@ -274,7 +272,7 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
   const int level = blockIdx.y;
   const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
   const int h2 = lv.h2[level], w2 = lv.w2[level];
-  const bool tiled = lv.tiled[level] != 0;
+  const int layout = lv.layout[level];
   const float scale = lv.scale[level];
   const int n = blockIdx.x / tiles_per_item;
   const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
@@ -318,14 +316,12 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     // the second chunk only when the window reaches into it (x0 not a multiple of 8): an
     // aligned window would otherwise pull in a neighbouring 8x8 tile's line for nothing
     const bool okb = yok && x0 != xa && xa + 8 >= 0 && xa + 8 < w2;
-    const T* pl = vbase + (int64_t)(p < npix ? p : 0) * plane;
-    const int offa = tiled ? tiled_offset(y, xa, w2) : y * w2 + xa;
-    const int offb = tiled ? tiled_offset(y, xa + 8, w2) : y * w2 + xa + 8;
+    const int pp = p < npix ? p : 0;
     // lanes whose chunk lies outside the plane issue no load (exec-masked), so no line is
     // fetched for them
     uint4 la = make_uint4(0u, 0u, 0u, 0u), lb = make_uint4(0u, 0u, 0u, 0u);
-    if (oka) la = *reinterpret_cast<const uint4*>(pl + offa);
-    if (okb) lb = *reinterpret_cast<const uint4*>(pl + offb);
+    if (oka) la = *reinterpret_cast<const uint4*>(vbase + pyramid_offset(layout, pp, y, xa, w2, plane));
+    if (okb) lb = *reinterpret_cast<const uint4*>(vbase + pyramid_offset(layout, pp, y, xa + 8, w2, plane));
     ca[it] = la;
     cbv[it] = lb;
   }
@@ -574,7 +570,7 @@ extern "C" int hsl2_corr_index_forward(const void* volume, const float* coords, 
   lv.vol[0] = volume;
   lv.h2[0] = static_cast<int>(h2);
   lv.w2[0] = static_cast<int>(w2);
-  lv.tiled[0] = 0;
+  lv.layout[0] = kPlaneRows;
   lv.scale[0] = 1.0f;
   lv.out_offset[0] = 0;
   // planar coords [batch][2][h1*w1]
@@ -597,7 +593,7 @@ extern "C" int hsl2_corr_pyramid_lookup(const void* const* levels, int num_level
     lv.vol[i] = levels[i];
     lv.h2[i] = static_cast<int>(h2);
     lv.w2[i] = static_cast<int>(w2);
-    lv.tiled[i] = (tiled && h2 % 8 == 0 && w2 % 8 == 0) ? 1 : 0;
+    lv.layout[i] = pyramid_layout(i, h1 * w1, h2, w2, tiled);
     lv.scale[i] = 1.0f / static_cast<float>(1 << i);
     lv.out_offset[i] = i * R * R * h1 * w1;
   }

@@ -77,9 +77,8 @@ int hsl2_corr_index_backward(const float* coords, const void* corr_grad, void* v
  * level on coords / 2^level, then torch.cat) with one launch.  levels[i] is the level-i volume
  * [batch, h1, w1, h1 >> i, w1 >> i] of `dtype`; coords [batch, h1, w1, 2] f32 (x, y
  * interleaved, the CorrBlock call layout); corr [batch, L*(2r+1)^2, h1, w1] of `dtype`, channel
- * order [level][x-offset][y-offset], fully written.  tiled != 0: every level whose plane sides
- * are multiples of 8 is stored as row-major 8x8 tiles (as hsl2_corr_build writes it with
- * tiled != 0); other levels are row-major. */
+ * order [level][x-offset][y-offset], fully written.  tiled != 0: the levels are in the tiled
+ * pyramid layout that hsl2_corr_build writes with tiled != 0. */
 int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const float* coords,
                              void* corr, int dtype, int64_t batch, int64_t h1, int64_t w1,
                              int radius, int tiled, void* stream);
@@ -91,9 +90,13 @@ int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const fl
  * average-pool levels.  fmap1/fmap2 are CHANNEL-LAST [batch, h * w, C] (`in_dtype`
  * F32/BF16/F16; hsl2_channels_last converts the reference's [batch, C, h, w]).  levels[i]
  * points at [batch, h, w, h >> i, w >> i] of `out_dtype` (F32/BF16).  Requires h, w multiples
- * of 8 and C a multiple of 16.  tiled != 0 stores every level plane whose sides are multiples of 8 as
- * row-major 8x8 tiles (read by hsl2_corr_pyramid_lookup with tiled != 0); tiled == 0 gives the
- * reference's row-major planes. */
+ * of 8 and C a multiple of 16.  tiled == 0 gives the reference's row-major planes
+ * (element (pix, y, x) at pix * h2 w2 + y w2 + x, pix = item * h w + pixel).  tiled != 0 (read
+ * by hsl2_corr_pyramid_lookup with tiled != 0; the same element count per level):
+ *   level 0, h2 and w2 multiples of 8: each plane as row-major 8x8 tiles, each tile row-major;
+ *   levels >= 1, w2 a multiple of 8 and h w a multiple of 64: pixel groups g = pix / 64 of
+ *     64 * h2 w2 elements, laid out [y][x / 8][pix % 64][x % 8];
+ *   any other level: row-major planes. */
 int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype, void* const* levels,
                     int out_dtype, int64_t batch, int64_t channels, int64_t h, int64_t w,
                     int num_levels, int tiled, void* stream);

@@ -91,16 +91,30 @@ def test_corr_build_pyramid(db, hw, dtype):
             assert np.max(np.abs(got - ref[i])) <= 2e-2
 
 
-def untile(level):
-    """[..., h, w] stored as row-major 8x8 tiles -> row-major planes (numpy)."""
+def untile(level, i):
+    """Level i of a tiled pyramid -> row-major planes (numpy): level 0 stored as row-major
+    8x8 tiles per plane, levels >= 1 as [pixel group of 64][y][x / 8][pixel][x % 8]."""
     a = np.asarray(level)
-    h, w = a.shape[-2:]
-    flat = a.reshape(a.shape[:-2] + (h // 8, w // 8, 8, 8))
-    return np.swapaxes(flat, -3, -2).reshape(a.shape)
+    n, h, w, h2, w2 = a.shape
+    if i > 0 and w2 % 8 == 0 and (h * w) % 64 == 0:
+        out = np.empty_like(a).reshape(n * h * w, h2, w2)
+        flat = a.reshape(-1)
+        for pix in range(n * h * w):
+            base = (pix // 64) * 64 * h2 * w2
+            for y in range(h2):
+                for xs in range(w2 // 8):
+                    o = base + ((y * (w2 // 8) + xs) * 64 + pix % 64) * 8
+                    out[pix, y, 8 * xs:8 * xs + 8] = flat[o:o + 8]
+        return out.reshape(a.shape)
+    if h2 % 8 == 0 and w2 % 8 == 0:
+        flat = a.reshape(a.shape[:-2] + (h2 // 8, w2 // 8, 8, 8))
+        return np.swapaxes(flat, -3, -2).reshape(a.shape)
+    return a
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
 def test_corr_build_tiled_layout(db, dtype):
+    from hislam2_amd import corr as hc
     rng = np.random.default_rng(31)
     f1 = rounded(rng.standard_normal((1, 2, 64, 48, 64)) / 8, dtype).cuda()
     f2 = rounded(rng.standard_normal((1, 2, 64, 48, 64)) / 8, dtype).cuda()
@@ -108,10 +122,8 @@ def test_corr_build_tiled_layout(db, dtype):
     tl = db.corr_build(f1, f2, 4, dtype, True)
     for i in range(4):
         a, b = rm[i].float().cpu().numpy(), tl[i].float().cpu().numpy()
-        h2, w2 = a.shape[-2:]
-        if h2 % 8 == 0 and w2 % 8 == 0:
-            b = untile(b)
-        assert np.array_equal(a, b), i
+        assert np.array_equal(a, untile(b, i)), i
+        assert torch.equal(hc.pyramid_planes(tl[i], i), rm[i]), i
     coords = torch.from_numpy(np.random.default_rng(32).uniform(-5, 70, (1, 2, 48, 64, 2)).astype(np.float32)).cuda()
     o_rm = db.corr_pyramid_lookup(rm, coords, 3, False)
     o_tl = db.corr_pyramid_lookup(tl, coords, 3, True)
