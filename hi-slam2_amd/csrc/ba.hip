@@ -11,9 +11,9 @@
 //   3. ba_schur_gram_lds per depth frame, 16x16 tiles of the slots' Gram matrix
 //                        S = sum_px E_a Q E_b^T on the matrix cores, one block per group
 //                        pair of row tiles staged in LDS by 64-pixel chunks (FP32 tile
-//                        sums per 256 pixels, FP64 totals);
-//                        ba_schur_ev, one wave per slot:
-//                        Ev = sum_px Q w E (EEt6x6 / Ev6x1, 1466-1560).
+//                        sums per 256 pixels, FP64 totals); the blocks of diagonal
+//                        groups also form Ev = sum_px Q w E (EEt6x6 / Ev6x1, 1466-1560)
+//                        (ba_schur_ev, one wave per slot, when motion-only).
 //   4. ba_assemble       the damped reduced camera system A - S, b - bS in FP64
 //                        (FP32 in the plain mode), written densely in column-major
 //                        order, padded to a multiple of 64 with an identity block

@@ -381,7 +381,9 @@ __device__ __forceinline__ float4 gram_load4(const float* __restrict__ row, int 
 // wave p % 4; step s of a chunk feeds lane l pixel 4 s + l / 16 of its A row (scaled by Q) and
 // B row.  FP32 tile sums are folded into FP64 every 256 pixels.  Only entries with a <= b are
 // written; a diagonal block (a == b) is written from its entries r <= c, mirrored, so it is
-// exactly symmetric.
+// exactly symmetric.  A diagonal item (ga == gb) also forms the Ev rows of its group's rows,
+// Ev = sum_px Q w E (EEt6x6 / Ev6x1's Ev, src/droid_kernels.cu:1466-1560), from the staged
+// chunks: each thread 16 pixels of one row per chunk in FP32, FP64 across chunks and threads.
 constexpr int kGramKC = 64;
 constexpr int kGramRS = 2 * 16 * kGramGroup + 16;   // LDS row stride (floats), 16 mod 64
 
@@ -389,9 +391,11 @@ __global__ void __launch_bounds__(256)
 ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
                   const int* __restrict__ items, int n_items, int item_blocks8,
                   const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
-                  const int* __restrict__ frame_pair0, int HW, double* __restrict__ S) {
+                  const int* __restrict__ frame_pair0, int HW, const float* __restrict__ wvec,
+                  double* __restrict__ S, double* __restrict__ Ev) {
   __shared__ float Ls[2][kGramKC][kGramRS];
-  __shared__ float qs[2][kGramKC];
+  __shared__ float qs[2][kGramKC], qws[2][kGramKC];
+  __shared__ double evs[4][16 * kGramGroup];
   const int blk = blockIdx.x;
   const int item = (blk % 8) * item_blocks8 + blk / 8;  // XCD-aware: runs of consecutive items
   if (item >= n_items) return;
@@ -434,26 +438,39 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
     if (gr < R) src = Eall + ((int64_t)slot_row[s0 + gr / 6] * 6 + gr % 6) * HW;
   }
   const float* qf = Q + (int64_t)f * HW;
+  const float* wf = wvec + (int64_t)f * HW;
   const bool vec = (HW % 4) == 0;
   float4 nx[8];
-  float4 nq = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 nq = make_float4(0.f, 0.f, 0.f, 0.f), nw = nq;
+  // Ev rows of a diagonal item's rows: thread t sums row t % 64 over pixels 16 (t / 64) ..
+  // + 15 of every chunk
+  const int er = t % 64, eq = t / 64;
+  const bool ev_row = diag && er < na && 16 * a0 + er < R;
+  double evd = 0.0;
   auto load = [&](int k0) {
     const int kk = k0 + 32 * h;
 #pragma unroll
     for (int j = 0; j < 8; ++j) nx[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     nq = make_float4(0.f, 0.f, 0.f, 0.f);
+    nw = nq;
     if (vec && k0 + kGramKC <= HW) {               // whole chunk inside the row: 16-byte loads
       if (src) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) nx[j] = *reinterpret_cast<const float4*>(src + kk + 4 * j);
       }
-      if (t < kGramKC / 4) nq = *reinterpret_cast<const float4*>(qf + k0 + 4 * t);
+      if (t < kGramKC / 4) {
+        nq = *reinterpret_cast<const float4*>(qf + k0 + 4 * t);
+        if (diag) nw = *reinterpret_cast<const float4*>(wf + k0 + 4 * t);
+      }
     } else {
       if (src) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) nx[j] = gram_load4(src, kk + 4 * j, HW, false);
       }
-      if (t < kGramKC / 4) nq = gram_load4(qf, k0 + 4 * t, HW, false);
+      if (t < kGramKC / 4) {
+        nq = gram_load4(qf, k0 + 4 * t, HW, false);
+        if (diag) nw = gram_load4(wf, k0 + 4 * t, HW, false);
+      }
     }
   };
   auto store = [&](int buf) {
@@ -472,6 +489,10 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
       qs[buf][4 * t + 1] = nq.y;
       qs[buf][4 * t + 2] = nq.z;
       qs[buf][4 * t + 3] = nq.w;
+      qws[buf][4 * t] = nq.x * nw.x;
+      qws[buf][4 * t + 1] = nq.y * nw.y;
+      qws[buf][4 * t + 2] = nq.z * nw.z;
+      qws[buf][4 * t + 3] = nq.w * nw.w;
     }
   };
   double dacc[4][4];
@@ -504,6 +525,12 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
           acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[u], 0, 0, 0);
         }
       }
+      if (ev_row) {
+        float e = 0.f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) e = fmaf(Ls[buf][16 * eq + j][er], qws[buf][16 * eq + j], e);
+        evd += static_cast<double>(e);
+      }
       if ((c & 3) == 3 || c + 1 == nch) {         // every 256 pixels
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
@@ -524,6 +551,14 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
     default: run(std::integral_constant<int, 4>{}); break;
   }
 
+  if (diag) {
+    evs[eq][er] = evd;
+    __syncthreads();
+    if (ev_row && eq == 0) {
+      const int gr = 16 * a0 + er;
+      Ev[(int64_t)(s0 + gr / 6) * 6 + gr % 6] = (evs[0][er] + evs[1][er]) + (evs[2][er] + evs[3][er]);
+    }
+  }
   const int64_t pair0 = frame_pair0[f];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -552,7 +587,8 @@ inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float*
                      blob + p->o_slot_row, task0, static_cast<int>(p->NT), static_cast<int>(p->HW), tb8, Ev);
 }
 
-// Schur pair blocks (matrix-core tiles) and the Ev rows, for the pair/Ev layout of the plan.
+// Schur pair blocks and Ev rows (both from the Gram kernel; Ev alone when there are no
+// Gram items, i.e. motion-only).
 inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
                               const float* wvec, double* S, double* Ev, hipStream_t s) {
   const int HW = static_cast<int>(p->HW);
@@ -560,9 +596,11 @@ inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const floa
     const int ni = static_cast<int>(p->NGGI);
     const int ib8 = (ni + 7) / 8;
     hipLaunchKernelGGL(ba_schur_gram_lds, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_group_item,
-                       ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, S);
+                       ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, wvec,
+                       S, Ev);
+  } else {
+    launch_schur_ev(p, blob, Eall, Q, wvec, Ev, s);
   }
-  launch_schur_ev(p, blob, Eall, Q, wvec, Ev, s);
 }
 
 // ---- 4. reduced camera system, dense column-major ----------------------------
