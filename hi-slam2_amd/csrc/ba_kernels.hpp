@@ -384,10 +384,12 @@ __device__ __forceinline__ float4 gram_load4(const float* __restrict__ row, int 
 // exactly symmetric.  A diagonal item (ga == gb) also forms the Ev rows of its group's rows,
 // Ev = sum_px Q w E (EEt6x6 / Ev6x1's Ev, src/droid_kernels.cu:1466-1560), from the staged
 // chunks: each thread 16 pixels of one row per chunk in FP32, FP64 across chunks and threads.
-constexpr int kGramKC = 64;
+constexpr int kGramKC = 32;                          // pixels per staged chunk
+constexpr int kGramPT = kGramKC / 2;                 // pixels of one row staged per thread
+constexpr int kGramFold = 256 / kGramKC;             // chunks per FP64 fold
 constexpr int kGramRS = 2 * 16 * kGramGroup + 16;   // LDS row stride (floats), 16 mod 64
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 3)
 ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
                   const int* __restrict__ items, int n_items, int item_blocks8,
                   const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
@@ -429,8 +431,8 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
       rowB[u] = __builtin_amdgcn_readfirstlane(rowB[u]);
     }
   }
-  // staging role: thread t stages pixels [32 h, 32 h + 32) of staged row r; rows beyond the
-  // frame's 6m stage zeros
+  // staging role: thread t stages pixels [kGramPT h, kGramPT (h + 1)) of staged row r; rows
+  // beyond the frame's 6m stage zeros
   const int r = t & 127, h = t >> 7;
   const float* src = nullptr;
   if (r < nr) {
@@ -440,59 +442,61 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
   const float* qf = Q + (int64_t)f * HW;
   const float* wf = wvec + (int64_t)f * HW;
   const bool vec = (HW % 4) == 0;
-  float4 nx[8];
-  float4 nq = make_float4(0.f, 0.f, 0.f, 0.f), nw = nq;
-  // Ev rows of a diagonal item's rows: thread t sums row t % 64 over pixels 16 (t / 64) ..
-  // + 15 of every chunk
+  // staging registers of one chunk: kGramPT pixels of one row, and Q, w of 4 pixels
+  struct Stage {
+    float4 x[kGramPT / 4], q, w;
+  };
+  // Ev rows of a diagonal item's rows: thread t sums row t % 64 over a quarter of every chunk
+  // (pixels kGramKC / 4 * (t / 64) + [0, kGramKC / 4))
   const int er = t % 64, eq = t / 64;
   const bool ev_row = diag && er < na && 16 * a0 + er < R;
   double evd = 0.0;
-  auto load = [&](int k0) {
-    const int kk = k0 + 32 * h;
+  auto load = [&](Stage& g, int k0) {
+    const int kk = k0 + kGramPT * h;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) nx[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    nq = make_float4(0.f, 0.f, 0.f, 0.f);
-    nw = nq;
+    for (int j = 0; j < kGramPT / 4; ++j) g.x[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    g.q = make_float4(0.f, 0.f, 0.f, 0.f);
+    g.w = g.q;
     if (vec && k0 + kGramKC <= HW) {               // whole chunk inside the row: 16-byte loads
       if (src) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) nx[j] = *reinterpret_cast<const float4*>(src + kk + 4 * j);
+        for (int j = 0; j < kGramPT / 4; ++j) g.x[j] = *reinterpret_cast<const float4*>(src + kk + 4 * j);
       }
       if (t < kGramKC / 4) {
-        nq = *reinterpret_cast<const float4*>(qf + k0 + 4 * t);
-        if (diag) nw = *reinterpret_cast<const float4*>(wf + k0 + 4 * t);
+        g.q = *reinterpret_cast<const float4*>(qf + k0 + 4 * t);
+        if (diag) g.w = *reinterpret_cast<const float4*>(wf + k0 + 4 * t);
       }
     } else {
       if (src) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) nx[j] = gram_load4(src, kk + 4 * j, HW, false);
+        for (int j = 0; j < kGramPT / 4; ++j) g.x[j] = gram_load4(src, kk + 4 * j, HW, false);
       }
       if (t < kGramKC / 4) {
-        nq = gram_load4(qf, k0 + 4 * t, HW, false);
-        if (diag) nw = gram_load4(wf, k0 + 4 * t, HW, false);
+        g.q = gram_load4(qf, k0 + 4 * t, HW, false);
+        if (diag) g.w = gram_load4(wf, k0 + 4 * t, HW, false);
       }
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](const Stage& g, int buf) {
     if (r < nr) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = 32 * h + 4 * j;
-        Ls[buf][k][r] = nx[j].x;
-        Ls[buf][k + 1][r] = nx[j].y;
-        Ls[buf][k + 2][r] = nx[j].z;
-        Ls[buf][k + 3][r] = nx[j].w;
+      for (int j = 0; j < kGramPT / 4; ++j) {
+        const int k = kGramPT * h + 4 * j;
+        Ls[buf][k][r] = g.x[j].x;
+        Ls[buf][k + 1][r] = g.x[j].y;
+        Ls[buf][k + 2][r] = g.x[j].z;
+        Ls[buf][k + 3][r] = g.x[j].w;
       }
     }
     if (t < kGramKC / 4) {
-      qs[buf][4 * t] = nq.x;
-      qs[buf][4 * t + 1] = nq.y;
-      qs[buf][4 * t + 2] = nq.z;
-      qs[buf][4 * t + 3] = nq.w;
-      qws[buf][4 * t] = nq.x * nw.x;
-      qws[buf][4 * t + 1] = nq.y * nw.y;
-      qws[buf][4 * t + 2] = nq.z * nw.z;
-      qws[buf][4 * t + 3] = nq.w * nw.w;
+      qs[buf][4 * t] = g.q.x;
+      qs[buf][4 * t + 1] = g.q.y;
+      qs[buf][4 * t + 2] = g.q.z;
+      qs[buf][4 * t + 3] = g.q.w;
+      qws[buf][4 * t] = g.q.x * g.w.x;
+      qws[buf][4 * t + 1] = g.q.y * g.w.y;
+      qws[buf][4 * t + 2] = g.q.z * g.w.z;
+      qws[buf][4 * t + 3] = g.q.w * g.w.w;
     }
   };
   double dacc[4][4];
@@ -508,12 +512,16 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
     gram_f4 acc[NP];
 #pragma unroll
     for (int u = 0; u < NP; ++u) acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
-    load(0);
-    store(0);
+    // chunk c + 2 is loaded while chunk c is consumed and chunk c + 1 waits in registers
+    Stage g0, g1;
+    load(g0, 0);
+    if (nch > 1) load(g1, kGramKC);
+    store(g0, 0);
     __syncthreads();
-    for (int c = 0; c < nch; ++c) {
+    // cur held chunk c (already in LDS), nxt holds chunk c + 1
+    auto chunk = [&](int c, Stage& cur, const Stage& nxt) {
       const int buf = c & 1;
-      if (c + 1 < nch) load((c + 1) * kGramKC);
+      if (c + 2 < nch) load(cur, (c + 2) * kGramKC);
 #pragma unroll
       for (int st = 0; st < kGramKC / 4; ++st) {
         const int kr = 4 * st + lane / 16;
@@ -528,10 +536,11 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
       if (ev_row) {
         float e = 0.f;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) e = fmaf(Ls[buf][16 * eq + j][er], qws[buf][16 * eq + j], e);
+        for (int j = 0; j < kGramKC / 4; ++j)
+          e = fmaf(Ls[buf][kGramKC / 4 * eq + j][er], qws[buf][kGramKC / 4 * eq + j], e);
         evd += static_cast<double>(e);
       }
-      if ((c & 3) == 3 || c + 1 == nch) {         // every 256 pixels
+      if (c % kGramFold == kGramFold - 1 || c + 1 == nch) {   // every 256 pixels
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
 #pragma unroll
@@ -539,8 +548,12 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
           acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
         }
       }
-      if (c + 1 < nch) store(buf ^ 1);
+      if (c + 1 < nch) store(nxt, buf ^ 1);
       __syncthreads();
+    };
+    for (int c = 0; c < nch; c += 2) {
+      chunk(c, g0, g1);
+      if (c + 1 < nch) chunk(c + 1, g1, g0);
     }
   };
   const int npairs = diag ? (a1 - a0) * (a1 - a0 + 1) / 2 : (a1 - a0) * (b1 - b0);
