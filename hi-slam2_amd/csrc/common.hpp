@@ -99,27 +99,36 @@ __device__ void block_reduce_store(const S (&vals)[V], T* lds, T* out) {
 
 // ---- stored correlation-pyramid levels -------------------------------------------
 // Element (pixel pix = n * hw1 + p, y, x) of a level with h2 x w2 planes, by layout:
-//   kPlaneRows   row-major planes: pix * h2 w2 + y w2 + x (the reference's layout)
-//   kPlaneTiles  each plane as row-major 8x8 tiles, each tile row-major (h2, w2 multiples of 8)
-//   kPlaneGroups the 64 pixels of a group g = pix / 64 interleaved: group g holds
-//                [y][x / 8][pix % 64][x % 8] (w2 a multiple of 8, hw1 of 64), so that the
-//                neighbouring pixels of a lookup block read neighbouring 16-byte segments
-enum { kPlaneRows = 0, kPlaneTiles = 1, kPlaneGroups = 2 };
+//   kPlaneRows     row-major planes: pix * h2 w2 + y w2 + x (the reference's layout)
+//   kPlaneTiles    each plane as row-major 8x8 tiles, each tile row-major (h2, w2 multiples of 8)
+//   kPlaneGroups   the 64 pixels of a group g = pix / 64 interleaved in 16-byte segments of one
+//                  row: group g holds [y][x / 8][pix % 64][x % 8] (w2 a multiple of 8)
+//   kPlaneGroups24 the same with segments of 2 rows x 4 columns: group g holds
+//                  [y / 2][x / 4][pix % 64][y % 2][x % 4] (h2 even, w2 a multiple of 4)
+// In the group layouts the neighbouring pixels of a lookup block read neighbouring segments,
+// so the 128-byte lines they fetch are shared (hw1 must be a multiple of 64).
+enum { kPlaneRows = 0, kPlaneTiles = 1, kPlaneGroups = 2, kPlaneGroups24 = 3 };
 constexpr int kPlaneGroup = 64;
 
 // Layout of level `level` (planes h2 x w2, hw1 pixels per item) in a tiled pyramid
-// (tiled != 0): level 0 in 8x8 tiles, the pooled levels pixel-interleaved; a level whose
-// sizes do not allow it stays row-major.
+// (tiled != 0): level 0 in 8x8 tiles, levels 1 and 2 in 2x4 segments, the further levels in
+// row segments of 8 (whichever the line counts of radius-3 windows favour); a level whose
+// sizes do not allow it falls back to the next layout that they allow.
 __host__ __device__ inline int pyramid_layout(int level, int64_t hw1, int64_t h2, int64_t w2, int tiled) {
   if (!tiled) return kPlaneRows;
-  if (level > 0 && w2 % 8 == 0 && hw1 % kPlaneGroup == 0) return kPlaneGroups;
+  if (level > 0 && hw1 % kPlaneGroup == 0) {
+    if (level <= 2 && h2 % 2 == 0 && w2 % 4 == 0) return kPlaneGroups24;
+    if (w2 % 8 == 0) return kPlaneGroups;
+  }
   return (h2 % 8 == 0 && w2 % 8 == 0) ? kPlaneTiles : kPlaneRows;
 }
 
 __host__ __device__ inline int64_t pyramid_offset(int layout, int64_t pix, int y, int x, int w2, int64_t plane) {
+  const int64_t g0 = (pix - pix % kPlaneGroup) * plane, pg = pix % kPlaneGroup;
+  if (layout == kPlaneGroups24)
+    return g0 + ((int64_t)((y >> 1) * (w2 >> 2) + (x >> 2)) * kPlaneGroup + pg) * 8 + ((y & 1) << 2) + (x & 3);
   if (layout == kPlaneGroups)
-    return (pix - pix % kPlaneGroup) * plane +
-           ((int64_t)(y * (w2 >> 3) + (x >> 3)) * kPlaneGroup + pix % kPlaneGroup) * 8 + (x & 7);
+    return g0 + ((int64_t)(y * (w2 >> 3) + (x >> 3)) * kPlaneGroup + pg) * 8 + (x & 7);
   if (layout == kPlaneTiles)
     return pix * plane + ((((y >> 3) * (w2 >> 3) + (x >> 3)) << 6) + ((y & 7) << 3) + (x & 7));
   return pix * plane + (int64_t)y * w2 + x;

@@ -106,7 +106,7 @@ __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* 
   using PL = Pool<RS, WS>;
   constexpr int CW = PL::CW;
   const int t = threadIdx.x;
-  const bool pfast = layout == kPlaneGroups;
+  const bool pfast = layout == kPlaneGroups || layout == kPlaneGroups24;
 #pragma unroll
   for (int it = 0; it < PL::NIT; ++it) {
     const int c = it * kThreads + t;
@@ -120,9 +120,23 @@ __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* 
         v[it][e] = 0.25f * ((a[2 * e] + a[2 * e + 1]) + (a[WS + 2 * e] + a[WS + 2 * e + 1]));
         o.v[e] = from_f<To>(v[it][e]);
       }
-      // an interleaved level has CW == 8 and gx a multiple of 8: the chunk is one segment
-      *reinterpret_cast<OutV<To, CW>*>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl,
-                                                            plane_stride)) = o;
+      if (layout == kPlaneGroups24) {
+        // CW is 4 or 8 here and x a multiple of CW: one 4-element half segment per 4 values
+        if constexpr (CW % 4 == 0) {
+#pragma unroll
+          for (int hseg = 0; hseg < CW / 4; ++hseg) {
+            OutV<To, 4> o4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o4.v[e] = o.v[4 * hseg + e];
+            *reinterpret_cast<OutV<To, 4>*>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x + 4 * hseg,
+                                                                 wl, plane_stride)) = o4;
+          }
+        }
+      } else {
+        // a kPlaneGroups level has CW == 8 and gx a multiple of 8: the chunk is one segment
+        *reinterpret_cast<OutV<To, CW>*>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl,
+                                                              plane_stride)) = o;
+      }
     }
   }
 }
@@ -133,7 +147,7 @@ __device__ __forceinline__ void pool_stash(float* __restrict__ dst, int layout,
   using PL = Pool<RS, WS>;
   constexpr int WD = PL::WD, CW = PL::CW;
   const int t = threadIdx.x;
-  const bool pfast = layout == kPlaneGroups;
+  const bool pfast = layout == kPlaneGroups || layout == kPlaneGroups24;
 #pragma unroll
   for (int it = 0; it < PL::NIT; ++it) {
     const int c = it * kThreads + t;
@@ -309,6 +323,14 @@ int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch
   const int64_t nblocks = (int64_t)ptiles * qtiles * batch;
   if (nblocks == 0) return HSL2_OK;
   HSL2_REQUIRE(nblocks < (1ll << 31), HSL2_ERR_SHAPE, "corr build: grid too large");
+  // pool_compute stores whole segments: 8-element chunks for kPlaneGroups levels, chunks of 4
+  // or 8 for kPlaneGroups24 levels
+  for (int i = 1; i < L; ++i) {
+    const int lay = pyramid_layout(i, h * w, h >> i, w >> i, tiled);
+    const int cw = chunk_of(WT >> i);
+    HSL2_REQUIRE(!(lay == kPlaneGroups && cw != 8) && !(lay == kPlaneGroups24 && cw % 4 != 0),
+                 HSL2_ERR_UNSUPPORTED, "corr build: pyramid layout and tile width do not match");
+  }
   To* l[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < L; ++i) l[i] = static_cast<To*>(levels[i]);
   hipLaunchKernelGGL((corr_build_kernel<Ti, To, WT>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,

@@ -258,6 +258,50 @@ template <typename T> __device__ __forceinline__ uint32_t f_to_half_bits(float v
   return static_cast<uint32_t>(*reinterpret_cast<const unsigned short*>(&h));
 }
 
+// Window gather of a kPlaneGroups24 level for the vec kernel: item (pixel p, row pair j < 5,
+// column quad s < 3) loads the 2x4 segment at rows 2 (y0 / 2 + j) + {0, 1}, columns
+// 4 (x0 / 4 + s) + {0, .., 3} with one 16-byte load (none when it misses the window or lies
+// outside the plane: zeros) and writes the elements that fall inside the 8x8 window to its
+// taps.  Every tap of a window lies in exactly one such segment.  Loads first, then the
+// writes.
+template <typename T>
+__device__ __forceinline__ void gather_segs24(const T* __restrict__ vbase, int64_t plane, int h2, int w2,
+                                              int npix, const int* s_x0, const int* s_y0, float* taps) {
+  constexpr int D = 8, TS = D * D + 1, NI = 15 * kPix, IT = (NI + kThreads - 1) / kThreads;
+  const int t = threadIdx.x;
+  uint4 sv[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int item = t + kThreads * it;
+    const int p = item / 15, rest = item - 15 * p, j = rest / 3, s = rest - 3 * j;
+    sv[it] = make_uint4(0u, 0u, 0u, 0u);
+    if (item < NI && p < npix) {
+      const int y0 = s_y0[p], x0 = s_x0[p];
+      const int yp = (y0 >> 1) + j, xq = (x0 >> 2) + s;
+      if (2 * yp <= y0 + 7 && 4 * xq <= x0 + 7 && yp >= 0 && 2 * yp < h2 && xq >= 0 && 4 * xq < w2)
+        sv[it] = *reinterpret_cast<const uint4*>(vbase + pyramid_offset(kPlaneGroups24, p, 2 * yp, 4 * xq, w2, plane));
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int item = t + kThreads * it;
+    const int p = item / 15, rest = item - 15 * p, j = rest / 3, s = rest - 3 * j;
+    if (item < NI) {
+      const int y0 = s_y0[p], x0 = s_x0[p];
+      const int ty0 = 2 * ((y0 >> 1) + j) - y0, tx0 = 4 * ((x0 >> 2) + s) - x0;
+      if (ty0 <= 7 && tx0 <= 7) {
+        const uint32_t wv[4] = {sv[it].x, sv[it].y, sv[it].z, sv[it].w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int ty = ty0 + (e >> 2), tx = tx0 + (e & 3);
+          if (ty >= 0 && ty < D && tx >= 0 && tx < D)
+            taps[p * TS + ty * D + tx] = half_bits_to_f<T>((wv[e >> 1] >> (16 * (e & 1))) & 0xffffu);
+        }
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
@@ -301,6 +345,9 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
 
   const int64_t plane = (int64_t)h2 * w2;
   const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
+  if (layout == kPlaneGroups24) {
+    gather_segs24<T>(vbase, plane, h2, w2, npix, s_x0, s_y0, taps);
+  } else {
   // gather: item = (pixel, window row); all four loads of a thread are issued first
   uint4 ca[2], cbv[2];
   int sh[2];
@@ -352,6 +399,7 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     dst[5] = half_bits_to_f<T>(d2 >> 16);
     dst[6] = half_bits_to_f<T>(d3 & 0xffffu);
     dst[7] = half_bits_to_f<T>(d3 >> 16);
+  }
   }
   __syncthreads();
 

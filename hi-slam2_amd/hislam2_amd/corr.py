@@ -81,9 +81,14 @@ def pyramid_planes(level, i):
     """Level ``i`` of a tiled CorrBlock pyramid ([B*N, h, w, h >> i, w >> i], the layout
     ``corr_build(..., tiled=True)`` writes) as the reference's row-major planes (a copy)."""
     n, h, w, h2, w2 = level.shape
-    if i > 0 and w2 % 8 == 0 and (h * w) % 64 == 0:
-        g = level.reshape(n * h * w // 64, h2, w2 // 8, 64, 8)   # [group][y][x / 8][pixel][x % 8]
-        return g.permute(0, 3, 1, 2, 4).reshape(level.shape).contiguous()
+    if i > 0 and (h * w) % 64 == 0:
+        if i <= 2 and h2 % 2 == 0 and w2 % 4 == 0:
+            # [group][y / 2][x / 4][pixel][y % 2][x % 4]
+            g = level.reshape(n * h * w // 64, h2 // 2, w2 // 4, 64, 2, 4)
+            return g.permute(0, 3, 1, 4, 2, 5).reshape(level.shape).contiguous()
+        if w2 % 8 == 0:
+            g = level.reshape(n * h * w // 64, h2, w2 // 8, 64, 8)   # [group][y][x / 8][pixel][x % 8]
+            return g.permute(0, 3, 1, 2, 4).reshape(level.shape).contiguous()
     if h2 % 8 == 0 and w2 % 8 == 0:
         t = level.reshape(n, h, w, h2 // 8, w2 // 8, 8, 8)       # [..][y / 8][x / 8][y % 8][x % 8]
         return t.permute(0, 1, 2, 3, 5, 4, 6).reshape(level.shape).contiguous()
@@ -94,8 +99,9 @@ class CorrBlock:
     """corr.py:23-71.  ``dtype`` selects the volume element type (default: fmap dtype).
 
     Without autograd the levels are stored in the tiled pyramid layout (``self.tiled``):
-    level 0 as 8x8 tiles per plane, the pooled levels with the 8-element row segments of
-    each 64 consecutive pixels interleaved (``pyramid_planes`` converts back), which the
+    level 0 as 8x8 tiles per plane, the pooled levels with the 2x4 (levels 1, 2) or 1x8
+    (further levels) segments of each 64 consecutive pixels interleaved (``pyramid_planes``
+    converts back), which the
     fused lookup reads with far fewer memory transactions than row-major planes; with
     autograd (feature maps that require grad) the reference's row-major planes are
     kept so that ``CorrSampler`` and its adjoint apply.  ``cat``/``__getitem__`` act

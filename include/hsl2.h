@@ -94,9 +94,11 @@ int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const fl
  * (element (pix, y, x) at pix * h2 w2 + y w2 + x, pix = item * h w + pixel).  tiled != 0 (read
  * by hsl2_corr_pyramid_lookup with tiled != 0; the same element count per level):
  *   level 0, h2 and w2 multiples of 8: each plane as row-major 8x8 tiles, each tile row-major;
- *   levels >= 1, w2 a multiple of 8 and h w a multiple of 64: pixel groups g = pix / 64 of
- *     64 * h2 w2 elements, laid out [y][x / 8][pix % 64][x % 8];
- *   any other level: row-major planes. */
+ *   levels 1 and 2, h2 even, w2 a multiple of 4 and h w a multiple of 64: pixel groups
+ *     g = pix / 64 of 64 * h2 w2 elements, laid out [y / 2][x / 4][pix % 64][y % 2][x % 4];
+ *   levels >= 1 otherwise, w2 a multiple of 8 and h w a multiple of 64: pixel groups laid out
+ *     [y][x / 8][pix % 64][x % 8];
+ *   any other level: 8x8 tiles when h2 and w2 are multiples of 8, else row-major planes. */
 int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype, void* const* levels,
                     int out_dtype, int64_t batch, int64_t channels, int64_t h, int64_t w,
                     int num_levels, int tiled, void* stream);

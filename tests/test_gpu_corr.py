@@ -93,9 +93,15 @@ def test_corr_build_pyramid(db, hw, dtype):
 
 def untile(level, i):
     """Level i of a tiled pyramid -> row-major planes (numpy): level 0 stored as row-major
-    8x8 tiles per plane, levels >= 1 as [pixel group of 64][y][x / 8][pixel][x % 8]."""
+    8x8 tiles per plane, levels 1 and 2 as [pixel group of 64][y / 2][x / 4][pixel][y % 2][x % 4],
+    further levels as [pixel group of 64][y][x / 8][pixel][x % 8]."""
     a = np.asarray(level)
     n, h, w, h2, w2 = a.shape
+    if 0 < i <= 2 and h2 % 2 == 0 and w2 % 4 == 0 and (h * w) % 64 == 0:
+        pix, y, x = np.meshgrid(np.arange(n * h * w), np.arange(h2), np.arange(w2), indexing='ij')
+        o = (pix // 64) * 64 * h2 * w2 + (((y // 2) * (w2 // 4) + x // 4) * 64 + pix % 64) * 8 \
+            + (y % 2) * 4 + x % 4
+        return a.reshape(-1)[o].reshape(a.shape)
     if i > 0 and w2 % 8 == 0 and (h * w) % 64 == 0:
         out = np.empty_like(a).reshape(n * h * w, h2, w2)
         flat = a.reshape(-1)
