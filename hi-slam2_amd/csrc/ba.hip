@@ -23,6 +23,9 @@
 //   6. ba_backsub        dz = Q (w - sum E^T dx) with the reference's t0 skip, and
 //                        the disparity update (EvT6x1, disp_retr, 1562-1582, 1398-1411).
 //   7. ba_pose_retr      T <- Exp(dx) T (1342-1396).
+#include <algorithm>
+#include <vector>
+
 #include "ba_kernels.hpp"
 
 namespace hsl2 {
@@ -104,6 +107,101 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
   return HSL2_OK;
 }
 
+// ---- mono-prior step pieces: proj_trans, bi_inter ------------------------------------------
+// Depth-only terms of the edges summed per ii frame (projective_transform2_kernel + accum_cuda,
+// src/droid_kernels.cu:246-362, 1781-1817): for pixel k of frame m (its edges frame_edges
+// [frame_ptr[m], frame_ptr[m+1]), all with ii = frame m's index),
+//   C[m][k] = sum_e sum_r w_r Jz_r^2,   w[m][k] = sum_e sum_r w_r r_r Jz_r
+// with the BA linearisation's pixel geometry (weights x 0.001, no contribution when the
+// transformed depth is below 0.25).  One block per (256 pixels, frame); the frame's edges are
+// taken 64 at a time, their relative poses formed once per block in LDS; sums in FP64.
+constexpr int kPtEdges = 64;
+
+__global__ void __launch_bounds__(256)
+proj_trans_kernel(const float* __restrict__ poses, const float* __restrict__ disps,
+                  const float* __restrict__ intr, const float* __restrict__ targets,
+                  const float* __restrict__ weights, const int64_t* __restrict__ ii,
+                  const int64_t* __restrict__ jj, const int* __restrict__ frame_ptr,
+                  const int* __restrict__ frame_edges, int ht, int wd, float* __restrict__ C,
+                  float* __restrict__ w) {
+  __shared__ EdgeGeom geo[kPtEdges];
+  __shared__ int eid[kPtEdges];
+  const int HW = ht * wd;
+  const int m = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e0 = frame_ptr[m], e1 = frame_ptr[m + 1];
+  if (e1 <= e0) {                                 // a frame always has an edge; kept for safety
+    if (k < HW) C[(int64_t)m * HW + k] = w[(int64_t)m * HW + k] = 0.f;
+    return;
+  }
+  const int64_t ix = ii[frame_edges[e0]];
+  const float u = static_cast<float>(k % wd), v = static_cast<float>(k / wd);
+  const float d = k < HW ? disps[ix * HW + k] : 0.f;
+  double c = 0.0, b = 0.0;
+  for (int base = e0; base < e1; base += kPtEdges) {
+    const int cnt = min(kPtEdges, e1 - base);
+    __syncthreads();                              // the previous chunk's geometry is consumed
+    if ((int)threadIdx.x < cnt) {
+      const int e = frame_edges[base + threadIdx.x];
+      eid[threadIdx.x] = e;
+      load_edge_geom(poses, intr, 0, static_cast<int>(ii[e]), static_cast<int>(jj[e]), geo[threadIdx.x]);
+    }
+    __syncthreads();
+    if (k < HW) {
+      for (int q = 0; q < cnt; ++q) {
+        const int64_t e = eid[q];
+        const float* tg = targets + e * 2 * HW;
+        const float* wt = weights + e * 2 * HW;
+        RowJac ru, rv;
+        if (pixel_rows<kSemCuda>(geo[q], u, v, d, tg[k], tg[HW + k], wt[k], wt[HW + k], ru, rv)) {
+          c += static_cast<double>(ru.w * ru.Jz * ru.Jz) + static_cast<double>(rv.w * rv.Jz * rv.Jz);
+          b += static_cast<double>(ru.w * ru.r * ru.Jz) + static_cast<double>(rv.w * rv.r * rv.Jz);
+        }
+      }
+    }
+  }
+  if (k < HW) {
+    C[(int64_t)m * HW + k] = static_cast<float>(c);
+    w[(int64_t)m * HW + k] = static_cast<float>(b);
+  }
+}
+
+// Bilinear interpolation of per-frame scale grids and its Jacobian (bi_inter_kernel,
+// src/droid_kernels.cu:1275-1315): pixel k of frame m samples scales[m] at column
+// grids[m][k][0], row grids[m][k][1]; weights wa (y0, x0), wb (y1, x0), wc (y0, x1), wd (y1, x1)
+// formed as the reference forms them.  The pixel's Jacobian row (hs * ws values) is written
+// whole: zeros except the four taps.  Taps outside the grid contribute nothing and are not
+// written (the reference indexes outside its tensors there).
+__global__ void __launch_bounds__(256)
+bi_inter_kernel(const float* __restrict__ scales, const float* __restrict__ grids, int HW, int hs,
+                int ws, float* __restrict__ out, float* __restrict__ jac) {
+  const int m = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= HW) return;
+  const int64_t pix = (int64_t)m * HW + k;
+  const float xind = grids[pix * 2], yind = grids[pix * 2 + 1];
+  const int x0 = static_cast<int>(floorf(xind)), y0 = static_cast<int>(floorf(yind));
+  const int x1 = x0 + 1, y1 = y0 + 1;
+  const float wa = (x1 - xind) * (y1 - yind);
+  const float wb = (x1 - xind) * (yind - y0);
+  const float wc = (xind - x0) * (y1 - yind);
+  const float wdd = (xind - x0) * (yind - y0);
+  const float* sc = scales + (int64_t)m * hs * ws;
+  float* jr = jac + pix * hs * ws;
+  for (int q = 0; q < hs * ws; ++q) jr[q] = 0.f;
+  const int ty[4] = {y0, y1, y0, y1}, tx[4] = {x0, x0, x1, x1};
+  const float tw[4] = {wa, wb, wc, wdd};
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (ty[j] >= 0 && ty[j] < hs && tx[j] >= 0 && tx[j] < ws) {
+      acc += tw[j] * sc[ty[j] * ws + tx[j]];
+      jr[ty[j] * ws + tx[j]] = tw[j];
+    }
+  }
+  out[pix] = acc;
+}
+
 }  // namespace
 }  // namespace hsl2
 
@@ -159,5 +257,65 @@ extern "C" int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, do
   if (vs) *vs = reinterpret_cast<double*>(ws + plan->w_vs);
   if (reduced_matrix) *reduced_matrix = ws + plan->w_A;
   if (reduced_rhs) *reduced_rhs = ws + plan->w_b;
+  return HSL2_OK;
+}
+
+// ---- mono-prior step pieces (src/droid.cpp:121-138, 244-250) --------------------------------
+
+extern "C" int hsl2_edge_frames(const int64_t* ii, int64_t num_edges, int64_t* kx, int32_t* frame_ptr,
+                                int32_t* frame_edges, int64_t* num_frames) {
+  HSL2_REQUIRE(num_edges >= 0 && num_frames && (num_edges == 0 || (ii && kx && frame_ptr && frame_edges)),
+               HSL2_ERR_ARGUMENT, "edge frames: null pointer");
+  std::vector<int64_t> u(ii, ii + num_edges);
+  std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  const int64_t M = static_cast<int64_t>(u.size());
+  std::vector<int32_t> cnt(M + 1, 0);
+  std::vector<int32_t> rank(num_edges);
+  for (int64_t e = 0; e < num_edges; ++e) {
+    rank[e] = static_cast<int32_t>(std::lower_bound(u.begin(), u.end(), ii[e]) - u.begin());
+    ++cnt[rank[e] + 1];
+  }
+  for (int64_t m = 0; m < M; ++m) cnt[m + 1] += cnt[m];
+  for (int64_t m = 0; m <= M; ++m) frame_ptr[m] = cnt[m];
+  std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t e = 0; e < num_edges; ++e) frame_edges[fill[rank[e]]++] = static_cast<int32_t>(e);
+  for (int64_t m = 0; m < M; ++m) kx[m] = u[m];
+  *num_frames = M;
+  return HSL2_OK;
+}
+
+extern "C" int hsl2_proj_trans(const float* poses, const float* disps, const float* intrinsics,
+                               const float* targets, const float* weights, const int64_t* ii,
+                               const int64_t* jj, const int32_t* frame_ptr, const int32_t* frame_edges,
+                               int64_t num_frames, int64_t ht, int64_t wd, float* C, float* w, void* stream) {
+  using namespace hsl2;
+  HSL2_REQUIRE(num_frames >= 0 && ht > 0 && wd > 0 && ht * wd < (1ll << 31), HSL2_ERR_SHAPE,
+               "proj_trans: bad sizes");
+  if (num_frames == 0) return HSL2_OK;
+  HSL2_REQUIRE(poses && disps && intrinsics && targets && weights && ii && jj && frame_ptr && frame_edges && C && w,
+               HSL2_ERR_ARGUMENT, "proj_trans: null pointer");
+  HSL2_REQUIRE(num_frames < 65536, HSL2_ERR_SHAPE, "proj_trans: too many frames");
+  const int HW = static_cast<int>(ht * wd);
+  hipLaunchKernelGGL(proj_trans_kernel, dim3((HW + 255) / 256, static_cast<unsigned>(num_frames)), dim3(256), 0,
+                     as_stream(stream), poses, disps, intrinsics, targets, weights, ii, jj, frame_ptr, frame_edges,
+                     static_cast<int>(ht), static_cast<int>(wd), C, w);
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+extern "C" int hsl2_bi_inter(const float* scales, const float* grids, int64_t num, int64_t ht, int64_t wd,
+                             int64_t hs, int64_t ws, float* outputs, float* jacobis, void* stream) {
+  using namespace hsl2;
+  HSL2_REQUIRE(num >= 0 && ht > 0 && wd > 0 && hs > 0 && ws > 0 && ht * wd < (1ll << 31) && hs * ws < (1 << 20),
+               HSL2_ERR_SHAPE, "bi_inter: bad sizes");
+  if (num == 0) return HSL2_OK;
+  HSL2_REQUIRE(scales && grids && outputs && jacobis, HSL2_ERR_ARGUMENT, "bi_inter: null pointer");
+  HSL2_REQUIRE(num < 65536, HSL2_ERR_SHAPE, "bi_inter: too many frames");
+  const int HW = static_cast<int>(ht * wd);
+  hipLaunchKernelGGL(bi_inter_kernel, dim3((HW + 255) / 256, static_cast<unsigned>(num)), dim3(256), 0,
+                     as_stream(stream), scales, grids, HW, static_cast<int>(hs), static_cast<int>(ws), outputs,
+                     jacobis);
+  HSL2_LAUNCH_CHECK();
   return HSL2_OK;
 }

@@ -481,6 +481,72 @@ std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, 
 
 }  // namespace
 
+// ---- mono-prior step pieces (src/droid.cpp:121-138, 244-250) ----------------------------
+// proj_trans(poses [N, 7], disps [N, h, w], intrinsics (first 4 values used), targets / weights
+// [E, 2, h, w], ii, jj) -> [C, w], each [M, h * w] over the sorted unique ii frames.
+std::vector<torch::Tensor> proj_trans(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                      torch::Tensor targets, torch::Tensor weights, torch::Tensor ii,
+                                      torch::Tensor jj) {
+  for (auto* t : {&poses, &disps, &intrinsics, &targets, &weights, &ii, &jj}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, "proj_trans tensors");
+  }
+  for (auto* t : {&poses, &disps, &intrinsics, &targets, &weights})
+    require(t->scalar_type() == torch::kFloat32, "proj_trans: poses, disps, intrinsics, targets, weights must be float32");
+  require(ii.scalar_type() == torch::kInt64 && jj.scalar_type() == torch::kInt64 && ii.dim() == 1 &&
+              jj.dim() == 1 && ii.size(0) == jj.size(0),
+          "proj_trans: ii, jj must be int64 [E]");
+  const int64_t N = poses.size(0), E = ii.size(0);
+  require(poses.dim() == 2 && poses.size(1) == 7, "proj_trans: poses must be [N, 7]");
+  require(disps.dim() == 3 && disps.size(0) == N, "proj_trans: disps must be [N, h, w]");
+  require(intrinsics.numel() >= 4, "proj_trans: intrinsics needs fx, fy, cx, cy");
+  const int64_t h = disps.size(1), w = disps.size(2);
+  require(targets.numel() == E * 2 * h * w && weights.numel() == E * 2 * h * w,
+          "proj_trans: targets, weights must be [E, 2, h, w]");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  torch::Tensor ii_h = ii.to(torch::kCPU).contiguous(), jj_h = jj.to(torch::kCPU).contiguous();
+  if (E > 0) {
+    auto lo = std::min(ii_h.min().item<int64_t>(), jj_h.min().item<int64_t>());
+    auto hi = std::max(ii_h.max().item<int64_t>(), jj_h.max().item<int64_t>());
+    require(lo >= 0 && hi < N, "proj_trans: an edge indexes a frame outside [0, N)");
+  }
+  torch::Tensor kx = torch::empty({std::max<int64_t>(E, 1)}, torch::kInt64);
+  torch::Tensor fptr = torch::empty({E + 1}, torch::kInt32), fedge = torch::empty({std::max<int64_t>(E, 1)}, torch::kInt32);
+  int64_t M = 0;
+  raise_status(hsl2_edge_frames(ii_h.data_ptr<int64_t>(), E, kx.data_ptr<int64_t>(), fptr.data_ptr<int32_t>(),
+                                fedge.data_ptr<int32_t>(), &M),
+               "proj_trans");
+  auto opts = poses.options();
+  torch::Tensor C = torch::empty({M, h * w}, opts), wv = torch::empty({M, h * w}, opts);
+  if (M == 0) return {C, wv};
+  torch::Tensor fptr_d = fptr.slice(0, 0, M + 1).to(poses.device()), fedge_d = fedge.slice(0, 0, E).to(poses.device());
+  raise_status(hsl2_proj_trans(poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                               targets.data_ptr<float>(), weights.data_ptr<float>(), ii.data_ptr<int64_t>(),
+                               jj.data_ptr<int64_t>(), fptr_d.data_ptr<int32_t>(), fedge_d.data_ptr<int32_t>(), M, h,
+                               w, C.data_ptr<float>(), wv.data_ptr<float>(), current_stream()),
+               "proj_trans");
+  return {C, wv};
+}
+
+// bi_inter(scales [M, hs, ws], grids [M, h, w, 2]) -> [outputs [M, h, w], Jacobis [M, h, w, hs, ws]]
+std::vector<torch::Tensor> bi_inter(torch::Tensor scales, torch::Tensor grids) {
+  for (auto* t : {&scales, &grids}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, "bi_inter tensors");
+    require(t->scalar_type() == torch::kFloat32, "bi_inter: float32 tensors");
+  }
+  require(scales.dim() == 3 && grids.dim() == 4 && grids.size(3) == 2 && grids.size(0) == scales.size(0),
+          "bi_inter: scales [M, hs, ws], grids [M, h, w, 2]");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(scales.device());
+  const int64_t M = grids.size(0), h = grids.size(1), w = grids.size(2), hs = scales.size(1), ws = scales.size(2);
+  auto opts = grids.options();
+  torch::Tensor out = torch::empty({M, h, w}, opts), J = torch::empty({M, h, w, hs, ws}, opts);
+  raise_status(hsl2_bi_inter(scales.data_ptr<float>(), grids.data_ptr<float>(), M, h, w, hs, ws,
+                             out.data_ptr<float>(), J.data_ptr<float>(), current_stream()),
+               "bi_inter");
+  return {out, J};
+}
+
 // ---- projective transform of the torch formulation (hislam2/geom/projective_ops.py:44-74) ----
 void check_projective(torch::Tensor& poses, torch::Tensor& disps, torch::Tensor& intrinsics, torch::Tensor& ii,
                       torch::Tensor& jj) {
@@ -702,6 +768,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("altcorr_forward_fp8", &altcorr_forward_fp8, "altcorr lookup on fp8 feature maps", py::arg("q1"),
         py::arg("s1"), py::arg("q2"), py::arg("s2"), py::arg("coords"), py::arg("ii"), py::arg("jj"),
         py::arg("radius"), py::arg("out_float") = false);
+  m.def("proj_trans", &proj_trans, "depth-only terms summed per ii frame: [C, w]", py::arg("poses"),
+        py::arg("disps"), py::arg("intrinsics"), py::arg("targets"), py::arg("weights"), py::arg("ii"),
+        py::arg("jj"));
+  m.def("bi_inter", &bi_inter, "bilinear interpolation of scale grids: [outputs, Jacobis]", py::arg("scales"),
+        py::arg("grids"));
   m.def("projective_transform", &projective_transform,
         "torch-formulation projective transform: [coords, valid, Ji, Jj, Jz]", py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("ii"), py::arg("jj"), py::arg("jacobian") = false,

@@ -120,3 +120,90 @@ def MoBA(target, weight, eta, poses, disps, intrinsics, ii, jj, fixedp=1, rig=1,
                             _as_f32(disps[b]), _as_f32(intrinsics[b]), ii, jj, int(fixedp), float(ep),
                             float(lm), True) for b in range(B)]
     return torch.stack(outs)
+
+
+# ---- mono-prior depth/scale step (hislam2/geom/ba.py:160-219, chol.py:80-108) ------------------
+
+def get_prior_depth_aligned(depth_prior, scales):
+    """hislam2/geom/ba.py:160-168: the prior [M, h, w] times its scale grid [M, hs, ws]
+    interpolated (``droid_backends.bi_inter``) on the reference's linspace grid.
+    Returns (aligned prior [M, h, w], Jacobian [M, h, w, hs, ws])."""
+    M, ht, wd = depth_prior.shape
+    hs, ws = scales.shape[-2:]
+    meshx, meshy = torch.meshgrid(torch.linspace(0, hs - 1 - 1e-6, ht), torch.linspace(0, ws - 1 - 1e-6, wd),
+                                  indexing='ij')
+    grid = torch.stack((meshy, meshx), -1).to(depth_prior.device)
+    grid = grid.unsqueeze(0).expand(M, -1, -1, -1).contiguous()
+    scaled, Jbi = backend().bi_inter(_as_f32(scales).contiguous(), grid)
+    return depth_prior * scaled, Jbi
+
+
+def JDSA(target, weight, eta, poses, disps, intrinsics, disps_prior, dscales, ii, jj, alpha, ep=0.1, lm=1e-4):
+    """One joint depth / scale-grid step against a mono depth prior (hislam2/geom/ba.py:171-219,
+    with schur_solve_mono_prior, chol.py:80-108).
+
+    target, weight [E, 2, h, w] (the layout ``cuda_ba`` passes, depth_video.py:229-230); eta
+    [M, h, w] (M = |unique(ii)|); poses [1, N, 7] or [N, 7]; disps [1, N, h, w]; intrinsics
+    [1, N, 4] (the first camera is used, as proj_trans does); disps_prior [>= N, h, w];
+    dscales [N, hs, ws], updated in place for the unique ii frames.  Returns (disps, dscales,
+    dzcov [M, h*w]).
+
+    The pixel-level work (the depth-only terms of every edge summed per frame, the scale-grid
+    interpolation) runs on the gfx950 kernels behind ``droid_backends.proj_trans`` and
+    ``droid_backends.bi_inter``.  The reduced system is block-diagonal (each frame's D = hs*ws
+    scale parameters couple only with its own pixels), so the reference's one Cholesky of the
+    M*D system and its ``inv(L)`` for dzcov are formed per D x D block, in FP64, with the
+    reference's slot assignment (frame kx[m] fills slot kx[m] - kx[0]; slots >= M dropped) and
+    its failure rule (any non-positive block -> zero scale update).  When that happens dzcov is
+    Q (the reference's ``inv`` of its zero factor raises there)."""
+    db = backend()
+    B, _, ht, wd = disps.shape
+    if B != 1:
+        raise ValueError('JDSA: disps must be [1, N, h, w]')
+    HW = ht * wd
+    dev = disps.device
+    P7 = _as_f32(poses).reshape(-1, 7).contiguous()
+    K = _as_f32(intrinsics).reshape(-1, 4)[0].contiguous()
+    C, w = db.proj_trans(P7, _as_f32(disps[0]).contiguous(), K, _as_f32(target).contiguous(),
+                         _as_f32(weight).contiguous(), ii, jj)
+    kx = torch.unique(ii)
+    M = kx.numel()
+    prior = disps_prior[kx]
+    m = (prior > 0).to(torch.float64).view(M, HW)
+    hs, ws = dscales.shape[-2:]
+    D = hs * ws
+    dbi, Jbi = get_prior_depth_aligned(prior, dscales[kx])
+    rd = (disps[0, kx] - dbi).view(M, HW).double()
+    Jso = -m.unsqueeze(-1) * prior.view(M, HW, 1).double() * Jbi.view(M, HW, D).double()     # [M, HW, D]
+    a = float(alpha)
+    slot = kx - kx[0]
+    keep = slot < M
+    H = torch.zeros(M, D, D, dtype=torch.float64, device=dev)
+    Es = torch.zeros(M, D, HW, dtype=torch.float64, device=dev)
+    v = torch.zeros(M, D, dtype=torch.float64, device=dev)
+    JT = Jso[keep].transpose(1, 2)                                                        # [K, D, HW]
+    H[slot[keep]] = a * JT @ Jso[keep]
+    Es[slot[keep]] = a * JT                                                               # Jd = 1
+    v[slot[keep]] = -a * (JT @ rd[keep].unsqueeze(-1)).squeeze(-1)
+    Cp = C.double() + m * a + (1 - m) * eta.reshape(M, HW).double()
+    wp = w.double() - m * a * rd
+    Q = 1.0 / Cp
+    I = torch.eye(D, dtype=torch.float64, device=dev)
+    S = H + (ep + lm * H) * I - (Es * Q.unsqueeze(1)) @ Es.transpose(1, 2)
+    vv = v - ((Es * Q.unsqueeze(1)) @ wp.unsqueeze(-1)).squeeze(-1)
+    L, info = torch.linalg.cholesky_ex(S)
+    ok = bool((info == 0).all())
+    if ok:
+        dso = torch.cholesky_solve(vv.unsqueeze(-1), L).squeeze(-1)
+        F = torch.linalg.solve_triangular(L, Es * Q.unsqueeze(1), upper=False)
+        dzcov = (F * F).sum(1) + Q
+    else:
+        dso = torch.zeros_like(vv)
+        dzcov = Q.clone()
+    dz = Q * (wp - (Es.transpose(1, 2) @ dso.unsqueeze(-1)).squeeze(-1))
+    disps = disps.clone()
+    disps[0].index_add_(0, kx, dz.view(M, ht, wd).to(disps.dtype))
+    dscales[kx] += dso.view(M, hs, ws).to(dscales.dtype)
+    disps = torch.where(disps > 10, torch.zeros_like(disps), disps)
+    disps = disps.clamp(min=0.001)
+    return disps, dscales, dzcov.to(torch.float32)

@@ -248,3 +248,30 @@ def lookup_coords(seed, n, h=48, w=64, sigma=2.0):
 
 PROBLEMS = {'small': ba_small, 'window': ba_window, 'global300': ba_global300,
             'diff16': ba_diff16, 'hires64': ba_hires64}
+
+
+def mono_problem(seed=0, h=48, w=64, hs=2, ws=2, drop_frame=None, alpha=0.01):
+    """Inputs of one JDSA step on the 4-keyframe problem: the BA problem of ``ba_small`` plus a
+    mono disparity prior (the true disparities divided by a smooth scale field, 10 % of the
+    pixels without prior, i.e. 0), a scale grid per frame near 1, eta per unique ii frame and
+    alpha.  ``drop_frame`` removes that frame's outgoing edges (a non-contiguous ii set)."""
+    p = ba_small(seed, h, w)
+    rng = np.random.default_rng(seed + 100)
+    ii, jj = p.ii, p.jj
+    targets, weights = p.targets, p.weights
+    if drop_frame is not None:
+        keep = ii != drop_frame
+        ii, jj, targets, weights = ii[keep], jj[keep], targets[keep], weights[keep]
+    n = p.poses.shape[0]
+    yy, xx = np.meshgrid(np.linspace(0, 1, h), np.linspace(0, 1, w), indexing='ij')
+    field = 1.0 + 0.2 * rng.uniform(-1, 1, size=(n, 1, 1)) * (xx + yy)[None]
+    prior = p.disps_true / field
+    prior[rng.uniform(size=prior.shape) < 0.1] = 0.0
+    dscales = 1.0 + 0.05 * rng.normal(size=(n, hs, ws))
+    M = len(np.unique(ii))
+    eta = np.full((M, h, w), 1e-4)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return dict(poses=p.poses, disps=p.disps, intrinsics=p.intrinsics, targets=f32(targets),
+                weights=f32(weights), eta=f32(eta), ii=np.ascontiguousarray(ii, np.int64),
+                jj=np.ascontiguousarray(jj, np.int64), disps_prior=f32(prior), dscales=f32(dscales),
+                alpha=alpha)

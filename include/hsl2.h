@@ -242,6 +242,35 @@ int hsl2_projective_transform_backward(const float* poses, const float* disps, c
                                        const float* grad_coords, float* grad_poses, float* grad_disps,
                                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- mono-prior depth/scale step pieces --------------------------------------------------- */
+
+/* Host only.  Groups edges by their ii frame: kx[num_frames] = sorted unique ii (torch.unique,
+ * as proj_trans_cuda and JDSA use it), frame_ptr[num_frames + 1] / frame_edges[num_edges] the
+ * edges of each frame in ascending edge order.  kx, frame_ptr, frame_edges need room for
+ * num_edges (+1) entries. */
+int hsl2_edge_frames(const int64_t* ii, int64_t num_edges, int64_t* kx, int32_t* frame_ptr,
+                     int32_t* frame_edges, int64_t* num_frames);
+
+/* Replaces proj_trans_cuda (src/droid_kernels.cu:246-362, 1781-1817; binding
+ * src/droid.cpp:121-138): the depth-only terms of every edge summed per ii frame,
+ * C[m][k] = sum w Jz^2, w[m][k] = sum w r Jz over the edges of frame kx[m] (ba's pixel geometry:
+ * weights x 0.001, no contribution for a transformed depth below 0.25).  poses [N, 7], disps
+ * [N, ht, wd], intrinsics [4] (one camera), targets / weights [E, 2, ht, wd], ii / jj int64 [E],
+ * frame_ptr / frame_edges from hsl2_edge_frames, all on the device; C, w [num_frames, ht * wd]
+ * f32 (FP64 sums). */
+int hsl2_proj_trans(const float* poses, const float* disps, const float* intrinsics,
+                    const float* targets, const float* weights, const int64_t* ii, const int64_t* jj,
+                    const int32_t* frame_ptr, const int32_t* frame_edges, int64_t num_frames,
+                    int64_t ht, int64_t wd, float* C, float* w, void* stream);
+
+/* Replaces bi_inter_cuda (src/droid_kernels.cu:1275-1315, 2145-2170; binding
+ * src/droid.cpp:244-250): scales [num, hs, ws] sampled bilinearly at grids [num, ht, wd, 2]
+ * (x = column, y = row) -> outputs [num, ht, wd]; jacobis [num, ht, wd, hs, ws] receives the
+ * interpolation weights (every entry written: zeros except the four taps).  Taps outside the
+ * grid contribute nothing (the reference reads and writes outside its tensors there). */
+int hsl2_bi_inter(const float* scales, const float* grids, int64_t num, int64_t ht, int64_t wd,
+                  int64_t hs, int64_t ws, float* outputs, float* jacobis, void* stream);
+
 /* Dense symmetric positive-definite solve A x = b (column-major n x n, F64 or F32) with the
  * tiled Cholesky that ba uses for its reduced camera system (replaces the host Eigen
  * SimplicialLLT of SparseBlock::solve, src/droid_kernels.cu:1659-1680).  A is not modified;
