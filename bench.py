@@ -39,8 +39,9 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
-    ap.add_argument('--only', choices=['all', 'lookup'], default='all',
-                    help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters)')
+    ap.add_argument('--only', choices=['all', 'lookup', 'mono'], default='all',
+                    help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
+                         'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph')
     return ap.parse_args()
 
 
@@ -101,6 +102,10 @@ def main():
         print(json.dumps({'metric': 'corr-lookup GB/s (algorithmic, 912 B per bf16 lookup)',
                           'value': round(gbps, 1), 'unit': 'GB/s', 'lookup_ms': round(ms, 4),
                           'lookups_per_launch': lookups, 'tiled': blk.tiled}), flush=True)
+        return
+
+    if args.only == 'mono':
+        mono_bench(db, p, up, dev, args)
         return
 
     def ba_call(poses, disps, precision):
@@ -237,6 +242,53 @@ def main():
             'reproj_diff_px': round(abs(res['split']['rms'] - res['plain']['rms']), 7),
         })
     print(json.dumps(line), flush=True)
+
+
+def mono_bench(db, p, up, dev, args):
+    """The mono-prior step on the global300 graph (300 keyframes, 3000 edges, 48x64, a 2x2
+    scale grid per frame): JDSA end to end, and its proj_trans kernel alone (algorithmic bytes:
+    targets + weights 16 B per pixel and edge, disparity 4 B and C, w 8 B per pixel and frame)."""
+    import numpy as np
+    import torch
+    from hislam2_amd.geom import ba as gba
+    rng = np.random.default_rng(3)
+    n, h, w = p.disps.shape
+    prior = p.disps_true / (1.0 + 0.1 * rng.uniform(-1, 1, size=(n, 1, 1)))
+    prior[rng.uniform(size=prior.shape) < 0.1] = 0.0
+    ii, jj = up(p.ii), up(p.jj)
+    M = len(np.unique(p.ii))
+    poses, disps = up(p.poses), up(p.disps)[None]
+    intr = up(p.intrinsics)[None, None].expand(1, n, 4)
+    tg, wt = up(p.targets), up(p.weights)
+    eta = torch.full((M, h, w), 1e-4, device=dev)
+    prior_t = up(prior.astype(np.float32))
+    scales0 = up((1.0 + 0.05 * rng.normal(size=(n, 2, 2))).astype(np.float32))
+
+    def jdsa():
+        return gba.JDSA(tg, wt, eta, poses[None], disps, intr, prior_t, scales0.clone(), ii, jj, 0.01)
+
+    def pt():
+        return db.proj_trans(poses, disps[0], intr[0, 0].contiguous(), tg, wt, ii, jj)
+
+    res = {}
+    for name, fn in (('jdsa', jdsa), ('proj_trans', pt)):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / args.steps
+    HW = h * w
+    pt_bytes = len(p.ii) * HW * 16 + M * HW * 12
+    print(json.dumps({'metric': 'mono-prior step (JDSA) on global300', 'jdsa_ms': round(res['jdsa'], 4),
+                      'proj_trans_ms': round(res['proj_trans'], 4),
+                      'proj_trans_GBps': round(pt_bytes / (res['proj_trans'] / 1e3) / 1e9, 1),
+                      'proj_trans_bytes': pt_bytes, 'frames': M, 'edges': len(p.ii),
+                      'note': 'proj_trans_ms includes the host-side edge grouping and its upload'}), flush=True)
 
 
 def lookup_traffic():

@@ -202,6 +202,218 @@ bi_inter_kernel(const float* __restrict__ scales, const float* __restrict__ grid
   out[pix] = acc;
 }
 
+// One JDSA step's reduced system, per slot (hislam2/geom/ba.py:171-219 with
+// schur_solve_mono_prior, chol.py:80-108), for scale grids of D = hs * ws <= 4 parameters.
+// The M*D reduced system is block-diagonal: slot s holds the D scale parameters of the frame
+// src[s] (the rank m with kx[m] - kx[0] == s; -1: none, all terms zero) and couples them with
+// the pixels of rank s.  Per pixel k, with the prior p of frame src[s], its mask q = p > 0, the
+// bilinear weights J (bi_inter on the linspace grid gx, gy) and rd = disp - p * (J . scales):
+//   Jso = -q p J,  H = a sum Jso Jso^T,  E_k = a Jso,  v = -a sum Jso rd
+//   C' = C + q_s a + (1 - q_s) eta,  w' = w - q_s a rd_s  (rank s),  Q = 1 / C'
+//   S = H + (ep + lm H) I - sum E_k Q E_k^T,  rhs = v - sum E_k Q w'.
+// jdsa_reduce: one block per slot sums these in FP64 and factors S (Cholesky, D x D); the
+// factor, dso and a failure flag go to the workspace.  jdsa_apply: one block per (256 pixels,
+// slot): dz = Q (w' - E_k . dso), dzcov = Q^2 |L^-1 E_k|^2 + Q, disps_out[kx[s]] = disps + dz
+// and dscales_out[kx[s]] = dscales + dso (the inputs are read unmodified, so slots never see
+// each other's updates); if any slot's factorisation failed every dso is taken as zero and
+// dzcov = Q.
+constexpr int kJdsaMaxD = 4;
+struct JdsaSlot {
+  double L[kJdsaMaxD][kJdsaMaxD];
+  double dso[kJdsaMaxD];
+  int fail;
+};
+
+struct JdsaPixel {
+  float jso[kJdsaMaxD];       // Jso of the slot's source frame at this pixel
+  float rd;                   // rd of the source frame
+};
+
+__device__ __forceinline__ void jdsa_pixel(const float* __restrict__ prior_f, const float* __restrict__ disp_f,
+                                           const float* __restrict__ scales_f, const float* __restrict__ gx,
+                                           const float* __restrict__ gy, int k, int wd, int hs, int ws,
+                                           JdsaPixel& px) {
+  const int y = k / wd, x = k - (k / wd) * wd;
+  const float xind = gx[x], yind = gy[y];
+  const int x0 = static_cast<int>(floorf(xind)), y0 = static_cast<int>(floorf(yind));
+  const int x1 = x0 + 1, y1 = y0 + 1;
+  const float wts[4] = {(x1 - xind) * (y1 - yind), (x1 - xind) * (yind - y0), (xind - x0) * (y1 - yind),
+                        (xind - x0) * (yind - y0)};
+  const int ty[4] = {y0, y1, y0, y1}, tx[4] = {x0, x0, x1, x1};
+  float J[kJdsaMaxD] = {0.f, 0.f, 0.f, 0.f};
+  float sc = 0.f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (ty[j] >= 0 && ty[j] < hs && tx[j] >= 0 && tx[j] < ws) {
+      const int d = ty[j] * ws + tx[j];
+      sc += wts[j] * scales_f[d];
+#pragma unroll
+      for (int q = 0; q < kJdsaMaxD; ++q)
+        if (q == d) J[q] = wts[j];
+    }
+  }
+  const float p = prior_f[k];
+  const float msk = p > 0.f ? 1.f : 0.f;
+  px.rd = disp_f[k] - p * sc;
+#pragma unroll
+  for (int q = 0; q < kJdsaMaxD; ++q) px.jso[q] = -msk * p * J[q];
+}
+
+__global__ void __launch_bounds__(256)
+jdsa_reduce(const float* __restrict__ C, const float* __restrict__ w, const float* __restrict__ disps,
+            const float* __restrict__ prior, const float* __restrict__ dscales, const float* __restrict__ eta,
+            const int64_t* __restrict__ kx, const int* __restrict__ src, const float* __restrict__ gx,
+            const float* __restrict__ gy, int HW, int wd, int hs, int ws, float alpha, float ep, float lm,
+            JdsaSlot* __restrict__ slots) {
+  // sums: H (10, upper triangle of 4 x 4), EQE (10), v (4), EQw (4)
+  constexpr int NV = 28;
+  __shared__ double lds[16 * 256];
+  __shared__ double red[NV];
+  const int s = blockIdx.x;
+  const int D = hs * ws;
+  const int m = src[s];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  if (m >= 0) {
+    const int64_t fm = kx[m], fs = kx[s];
+    const double a = alpha;
+    for (int k = threadIdx.x; k < HW; k += blockDim.x) {
+      JdsaPixel px;
+      jdsa_pixel(prior + fm * HW, disps + fm * HW, dscales + fm * D, gx, gy, k, wd, hs, ws, px);
+      // rank s's pixel: C', w' (its own prior mask and rd)
+      JdsaPixel ps;
+      jdsa_pixel(prior + fs * HW, disps + fs * HW, dscales + fs * D, gx, gy, k, wd, hs, ws, ps);
+      const double qs = prior[fs * HW + k] > 0.f ? 1.0 : 0.0;
+      const double Cp = static_cast<double>(C[(int64_t)s * HW + k]) + qs * a +
+                        (1.0 - qs) * static_cast<double>(eta[(int64_t)s * HW + k]);
+      const double wp = static_cast<double>(w[(int64_t)s * HW + k]) - qs * a * static_cast<double>(ps.rd);
+      const double Q = 1.0 / Cp;
+      int t = 0;
+#pragma unroll
+      for (int i = 0; i < kJdsaMaxD; ++i)
+#pragma unroll
+        for (int j = i; j < kJdsaMaxD; ++j, ++t) {
+          const double jj2 = static_cast<double>(px.jso[i]) * px.jso[j];
+          acc[t] += a * jj2;
+          acc[10 + t] += a * a * Q * jj2;
+        }
+#pragma unroll
+      for (int i = 0; i < kJdsaMaxD; ++i) {
+        acc[20 + i] += -a * static_cast<double>(px.jso[i]) * px.rd;
+        acc[24 + i] += a * static_cast<double>(px.jso[i]) * Q * wp;
+      }
+    }
+  }
+  block_reduce_store<NV, 256, double>(acc, lds, red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S[kJdsaMaxD][kJdsaMaxD], b[kJdsaMaxD];
+    int t = 0;
+    for (int i = 0; i < kJdsaMaxD; ++i)
+      for (int j = i; j < kJdsaMaxD; ++j, ++t) {
+        const double h = red[t];
+        double v = h - red[10 + t];
+        if (i == j) v += ep + lm * h;
+        S[i][j] = S[j][i] = v;
+      }
+    for (int i = 0; i < kJdsaMaxD; ++i) b[i] = red[20 + i] - red[24 + i];
+    // Cholesky of the leading D x D block (padding rows of a smaller grid are unused)
+    double L[kJdsaMaxD][kJdsaMaxD] = {};
+    int fail = 0;
+    for (int j = 0; j < D; ++j) {
+      double d = S[j][j];
+      for (int q = 0; q < j; ++q) d -= L[j][q] * L[j][q];
+      if (!(d > 0.0)) {
+        fail = 1;
+        break;
+      }
+      L[j][j] = sqrt(d);
+      for (int i = j + 1; i < D; ++i) {
+        double v = S[i][j];
+        for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
+        L[i][j] = v / L[j][j];
+      }
+    }
+    double x[kJdsaMaxD] = {0.0, 0.0, 0.0, 0.0};
+    if (!fail) {
+      double y[kJdsaMaxD];
+      for (int i = 0; i < D; ++i) {
+        double v = b[i];
+        for (int q = 0; q < i; ++q) v -= L[i][q] * y[q];
+        y[i] = v / L[i][i];
+      }
+      for (int i = D - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int q = i + 1; q < D; ++q) v -= L[q][i] * x[q];
+        x[i] = v / L[i][i];
+      }
+    }
+    JdsaSlot& out = slots[s];
+    for (int i = 0; i < kJdsaMaxD; ++i) {
+      out.dso[i] = x[i];
+      for (int j = 0; j < kJdsaMaxD; ++j) out.L[i][j] = L[i][j];
+    }
+    out.fail = fail;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+jdsa_apply(const float* __restrict__ C, const float* __restrict__ w, const float* __restrict__ disps,
+           const float* __restrict__ prior, const float* __restrict__ dscales, const float* __restrict__ eta,
+           const int64_t* __restrict__ kx, const int* __restrict__ src, const float* __restrict__ gx,
+           const float* __restrict__ gy, const JdsaSlot* __restrict__ slots, int M, int HW, int wd, int hs,
+           int ws, float alpha, float* __restrict__ disps_out, float* __restrict__ dscales_out,
+           float* __restrict__ dzcov) {
+  const int s = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int D = hs * ws;
+  __shared__ int any_fail;
+  if (threadIdx.x == 0) any_fail = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < M; i += blockDim.x)
+    if (slots[i].fail) any_fail = 1;             // every writer stores the same value
+  __syncthreads();
+  const bool ok = any_fail == 0;
+  const int m = src[s];
+  const JdsaSlot& sl = slots[s];
+  const int64_t fs = kx[s];
+  if (k == 0 && ok)                               // dscales[kx[s]] += dso[s]
+    for (int i = 0; i < D; ++i) dscales_out[fs * D + i] = dscales[fs * D + i] + static_cast<float>(sl.dso[i]);
+  if (k >= HW) return;
+  const double a = alpha;
+  JdsaPixel ps;
+  jdsa_pixel(prior + fs * HW, disps + fs * HW, dscales + fs * D, gx, gy, k, wd, hs, ws, ps);
+  const double qs = prior[fs * HW + k] > 0.f ? 1.0 : 0.0;
+  const double Cp = static_cast<double>(C[(int64_t)s * HW + k]) + qs * a +
+                    (1.0 - qs) * static_cast<double>(eta[(int64_t)s * HW + k]);
+  const double wp = static_cast<double>(w[(int64_t)s * HW + k]) - qs * a * static_cast<double>(ps.rd);
+  const double Q = 1.0 / Cp;
+  double dz = Q * wp, cov = Q;
+  if (m >= 0 && ok) {
+    const int64_t fm = kx[m];
+    JdsaPixel px;
+    jdsa_pixel(prior + fm * HW, disps + fm * HW, dscales + fm * D, gx, gy, k, wd, hs, ws, px);
+    double e[kJdsaMaxD], z[kJdsaMaxD];
+    double ed = 0.0;
+    for (int i = 0; i < kJdsaMaxD; ++i) {
+      e[i] = a * px.jso[i] * Q;                   // (E Q)_k
+      ed += a * px.jso[i] * sl.dso[i];
+    }
+    dz = Q * (wp - ed);
+    double f2 = 0.0;
+    for (int i = 0; i < D; ++i) {                 // z = L^-1 (E Q)_k
+      double v = e[i];
+      for (int q = 0; q < i; ++q) v -= sl.L[i][q] * z[q];
+      z[i] = v / sl.L[i][i];
+      f2 += z[i] * z[i];
+    }
+    cov = f2 + Q;
+  }
+  dzcov[(int64_t)s * HW + k] = static_cast<float>(cov);
+  disps_out[fs * HW + k] = disps[fs * HW + k] + static_cast<float>(dz);
+}
+
 }  // namespace
 }  // namespace hsl2
 
@@ -316,6 +528,43 @@ extern "C" int hsl2_bi_inter(const float* scales, const float* grids, int64_t nu
   hipLaunchKernelGGL(bi_inter_kernel, dim3((HW + 255) / 256, static_cast<unsigned>(num)), dim3(256), 0,
                      as_stream(stream), scales, grids, HW, static_cast<int>(hs), static_cast<int>(ws), outputs,
                      jacobis);
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+extern "C" size_t hsl2_jdsa_workspace_bytes(int64_t num_frames) {
+  return static_cast<size_t>(num_frames > 0 ? num_frames : 0) * sizeof(hsl2::JdsaSlot);
+}
+
+extern "C" int hsl2_jdsa_solve(const float* C, const float* w, const float* disps, const float* prior,
+                               const float* dscales, const float* eta, const int64_t* kx, const int32_t* slot_src,
+                               const float* gx, const float* gy, int64_t num_frames, int64_t ht, int64_t wd,
+                               int64_t hs, int64_t ws, float alpha, float ep, float lm, void* workspace,
+                               size_t workspace_bytes, float* disps_out, float* dscales_out, float* dzcov,
+                               void* stream) {
+  using namespace hsl2;
+  HSL2_REQUIRE(num_frames >= 0 && ht > 0 && wd > 0 && ht * wd < (1ll << 31), HSL2_ERR_SHAPE,
+               "jdsa: bad sizes");
+  HSL2_REQUIRE(hs > 0 && ws > 0 && hs * ws <= kJdsaMaxD, HSL2_ERR_UNSUPPORTED,
+               "jdsa: scale grids of at most 4 parameters");
+  if (num_frames == 0) return HSL2_OK;
+  HSL2_REQUIRE(C && w && disps && prior && dscales && eta && kx && slot_src && gx && gy && disps_out && dscales_out &&
+                   dzcov && workspace,
+               HSL2_ERR_ARGUMENT, "jdsa: null pointer");
+  HSL2_REQUIRE(workspace_bytes >= hsl2_jdsa_workspace_bytes(num_frames), HSL2_ERR_WORKSPACE,
+               "jdsa: workspace too small");
+  HSL2_REQUIRE(num_frames < 65536, HSL2_ERR_SHAPE, "jdsa: too many frames");
+  const int HW = static_cast<int>(ht * wd);
+  hipStream_t s = as_stream(stream);
+  JdsaSlot* slots = static_cast<JdsaSlot*>(workspace);
+  hipLaunchKernelGGL(jdsa_reduce, dim3(static_cast<unsigned>(num_frames)), dim3(256), 0, s, C, w, disps, prior,
+                     dscales, eta, kx, slot_src, gx, gy, HW, static_cast<int>(wd), static_cast<int>(hs),
+                     static_cast<int>(ws), alpha, ep, lm, slots);
+  HSL2_LAUNCH_CHECK();
+  hipLaunchKernelGGL(jdsa_apply, dim3((HW + 255) / 256, static_cast<unsigned>(num_frames)), dim3(256), 0, s, C, w,
+                     disps, prior, dscales, eta, kx, slot_src, gx, gy, slots, static_cast<int>(num_frames), HW,
+                     static_cast<int>(wd), static_cast<int>(hs), static_cast<int>(ws), alpha, disps_out, dscales_out,
+                     dzcov);
   HSL2_LAUNCH_CHECK();
   return HSL2_OK;
 }

@@ -547,6 +547,43 @@ std::vector<torch::Tensor> bi_inter(torch::Tensor scales, torch::Tensor grids) {
   return {out, J};
 }
 
+// jdsa_solve(C, w, disps [N, h, w], prior, dscales [N, hs, ws], eta [M, h, w], kx [M] int64,
+// slot_src [M] int32, gx [w], gy [h], alpha, ep, lm) -> [disps_out, dzcov]; dscales updated in place.
+std::vector<torch::Tensor> jdsa_solve(torch::Tensor C, torch::Tensor w, torch::Tensor disps, torch::Tensor prior,
+                                      torch::Tensor dscales, torch::Tensor eta, torch::Tensor kx,
+                                      torch::Tensor slot_src, torch::Tensor gx, torch::Tensor gy, double alpha,
+                                      double ep, double lm) {
+  for (auto* t : {&C, &w, &disps, &prior, &dscales, &eta, &kx, &slot_src, &gx, &gy}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, "jdsa tensors");
+  }
+  for (auto* t : {&C, &w, &disps, &prior, &dscales, &eta, &gx, &gy})
+    require(t->scalar_type() == torch::kFloat32, "jdsa: float32 tensors");
+  require(kx.scalar_type() == torch::kInt64 && slot_src.scalar_type() == torch::kInt32,
+          "jdsa: kx int64, slot_src int32");
+  const int64_t M = C.size(0), h = disps.size(1), wd = disps.size(2), hs = dscales.size(1), ws = dscales.size(2);
+  require(disps.dim() == 3 && dscales.dim() == 3 && dscales.size(0) == disps.size(0), "jdsa: disps [N, h, w], dscales [N, hs, ws]");
+  require(C.dim() == 2 && C.size(1) == h * wd && w.sizes() == C.sizes() && eta.numel() == M * h * wd &&
+              kx.numel() == M && slot_src.numel() == M && gx.numel() == wd && gy.numel() == h &&
+              prior.numel() >= disps.numel() && prior.size(-1) == wd,
+          "jdsa: inconsistent shapes");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(C.device());
+  torch::Tensor disps_out = disps.clone(), dscales_in = dscales.clone();
+  torch::Tensor dzcov = torch::empty({M, h * wd}, C.options());
+  torch::Tensor ws_t = torch::empty({static_cast<int64_t>(hsl2_jdsa_workspace_bytes(M)) + 256},
+                                    C.options().dtype(torch::kUInt8));
+  void* wp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(ws_t.data_ptr()) + 255) & ~uintptr_t(255));
+  raise_status(hsl2_jdsa_solve(C.data_ptr<float>(), w.data_ptr<float>(), disps.data_ptr<float>(),
+                               prior.data_ptr<float>(), dscales_in.data_ptr<float>(), eta.data_ptr<float>(),
+                               kx.data_ptr<int64_t>(), slot_src.data_ptr<int32_t>(), gx.data_ptr<float>(),
+                               gy.data_ptr<float>(), M, h, wd, hs, ws, static_cast<float>(alpha), static_cast<float>(ep),
+                               static_cast<float>(lm), wp, static_cast<size_t>(ws_t.numel() - 256),
+                               disps_out.data_ptr<float>(), dscales.data_ptr<float>(), dzcov.data_ptr<float>(),
+                               current_stream()),
+               "jdsa_solve");
+  return {disps_out, dzcov};
+}
+
 // ---- projective transform of the torch formulation (hislam2/geom/projective_ops.py:44-74) ----
 void check_projective(torch::Tensor& poses, torch::Tensor& disps, torch::Tensor& intrinsics, torch::Tensor& ii,
                       torch::Tensor& jj) {
@@ -771,6 +808,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("proj_trans", &proj_trans, "depth-only terms summed per ii frame: [C, w]", py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("targets"), py::arg("weights"), py::arg("ii"),
         py::arg("jj"));
+  m.def("jdsa_solve", &jdsa_solve, "JDSA reduced system per frame slot and update: [disps, dzcov]",
+        py::arg("C"), py::arg("w"), py::arg("disps"), py::arg("prior"), py::arg("dscales"), py::arg("eta"),
+        py::arg("kx"), py::arg("slot_src"), py::arg("gx"), py::arg("gy"), py::arg("alpha"), py::arg("ep"),
+        py::arg("lm"));
   m.def("bi_inter", &bi_inter, "bilinear interpolation of scale grids: [outputs, Jacobis]", py::arg("scales"),
         py::arg("grids"));
   m.def("projective_transform", &projective_transform,

@@ -148,14 +148,16 @@ def JDSA(target, weight, eta, poses, disps, intrinsics, disps_prior, dscales, ii
     dscales [N, hs, ws], updated in place for the unique ii frames.  Returns (disps, dscales,
     dzcov [M, h*w]).
 
-    The pixel-level work (the depth-only terms of every edge summed per frame, the scale-grid
-    interpolation) runs on the gfx950 kernels behind ``droid_backends.proj_trans`` and
-    ``droid_backends.bi_inter``.  The reduced system is block-diagonal (each frame's D = hs*ws
-    scale parameters couple only with its own pixels), so the reference's one Cholesky of the
-    M*D system and its ``inv(L)`` for dzcov are formed per D x D block, in FP64, with the
-    reference's slot assignment (frame kx[m] fills slot kx[m] - kx[0]; slots >= M dropped) and
-    its failure rule (any non-positive block -> zero scale update).  When that happens dzcov is
-    Q (the reference's ``inv`` of its zero factor raises there)."""
+    The depth-only terms of every edge summed per frame run on the gfx950 kernel behind
+    ``droid_backends.proj_trans``.  The reduced system is block-diagonal (each frame's
+    D = hs*ws scale parameters couple only with its own pixels), so the reference's one
+    Cholesky of the M*D system and its ``inv(L)`` for dzcov are formed per D x D block, in
+    FP64, with the reference's slot assignment (frame kx[m] fills slot kx[m] - kx[0]; slots
+    >= M dropped) and its failure rule (any non-positive block -> zero scale update; dzcov is
+    then Q, where the reference's ``inv`` of its zero factor raises).  For D <= 4 (HI-SLAM2's
+    2 x 2 grids) that runs as two kernels behind ``droid_backends.jdsa_solve`` (per-slot FP64
+    sums and factorisation, then the per-pixel update); larger grids take batched torch ops
+    over ``droid_backends.bi_inter``."""
     db = backend()
     B, _, ht, wd = disps.shape
     if B != 1:
@@ -168,10 +170,12 @@ def JDSA(target, weight, eta, poses, disps, intrinsics, disps_prior, dscales, ii
                          _as_f32(weight).contiguous(), ii, jj)
     kx = torch.unique(ii)
     M = kx.numel()
-    prior = disps_prior[kx]
-    m = (prior > 0).to(torch.float64).view(M, HW)
     hs, ws = dscales.shape[-2:]
     D = hs * ws
+    if D <= 4:
+        return _jdsa_native(db, C, w, eta, disps, disps_prior, dscales, kx, alpha, ep, lm)
+    prior = disps_prior[kx]
+    m = (prior > 0).to(torch.float64).view(M, HW)
     dbi, Jbi = get_prior_depth_aligned(prior, dscales[kx])
     rd = (disps[0, kx] - dbi).view(M, HW).double()
     Jso = -m.unsqueeze(-1) * prior.view(M, HW, 1).double() * Jbi.view(M, HW, D).double()     # [M, HW, D]
@@ -207,3 +211,28 @@ def JDSA(target, weight, eta, poses, disps, intrinsics, disps_prior, dscales, ii
     disps = torch.where(disps > 10, torch.zeros_like(disps), disps)
     disps = disps.clamp(min=0.001)
     return disps, dscales, dzcov.to(torch.float32)
+
+
+def _jdsa_native(db, C, w, eta, disps, disps_prior, dscales, kx, alpha, ep, lm):
+    """JDSA after proj_trans on the gfx950 kernels (scale grids of at most 4 parameters)."""
+    M = kx.numel()
+    _, _, ht, wd = disps.shape
+    hs, ws = dscales.shape[-2:]
+    dev = disps.device
+    slot = kx - kx[0]
+    keep = slot < M
+    src = torch.full((M,), -1, dtype=torch.int32, device=dev)
+    src[slot[keep]] = torch.arange(M, dtype=torch.int32, device=dev)[keep]
+    # the interpolation grid of get_prior_depth_aligned (built on the host, as the reference does)
+    gy = torch.linspace(0, hs - 1 - 1e-6, ht).to(dev)
+    gx = torch.linspace(0, ws - 1 - 1e-6, wd).to(dev)
+    inplace = dscales.dtype == torch.float32 and dscales.is_contiguous()
+    ds = dscales if inplace else dscales.float().contiguous()
+    d_out, dzcov = db.jdsa_solve(C, w, _as_f32(disps[0]).contiguous(), _as_f32(disps_prior).contiguous(), ds,
+                                 _as_f32(eta).reshape(M, ht, wd).contiguous(), kx, src, gx, gy, float(alpha),
+                                 float(ep), float(lm))
+    if not inplace:
+        dscales.copy_(ds)
+    out = d_out[None].to(disps.dtype)
+    out = torch.where(out > 10, torch.zeros_like(out), out)
+    return out.clamp(min=0.001), dscales, dzcov

@@ -271,6 +271,24 @@ int hsl2_proj_trans(const float* poses, const float* disps, const float* intrins
 int hsl2_bi_inter(const float* scales, const float* grids, int64_t num, int64_t ht, int64_t wd,
                   int64_t hs, int64_t ws, float* outputs, float* jacobis, void* stream);
 
+/* Replaces the rest of JDSA after proj_trans (hislam2/geom/ba.py:171-219 with
+ * schur_solve_mono_prior, hislam2/geom/chol.py:80-108) for scale grids of hs * ws <= 4: the
+ * reduced system of the scale parameters is block-diagonal, so one block per frame slot forms
+ * it in FP64 from C, w [M, ht*wd] (proj_trans), disps [N, ht*wd], prior [*, ht*wd], dscales
+ * [N, hs*ws], eta [M, ht*wd], kx [M] (int64, the sorted unique ii) and slot_src [M] (int32: the
+ * rank whose frame fills slot s = kx[rank] - kx[0], or -1), gx [wd] / gy [ht] (the
+ * interpolation grid), factors it and applies the update: disps_out[kx[s]] = disps + dz,
+ * dscales_out[kx[s]] = dscales + dso, dzcov [M, ht*wd].  Any failed block zeroes every dso
+ * (the reference's single Cholesky fails as a whole); dzcov is then Q.  The inputs are only
+ * read (disps_out / dscales_out must not alias disps / dscales); workspace of
+ * hsl2_jdsa_workspace_bytes(M) bytes. */
+size_t hsl2_jdsa_workspace_bytes(int64_t num_frames);
+int hsl2_jdsa_solve(const float* C, const float* w, const float* disps, const float* prior, const float* dscales,
+                    const float* eta, const int64_t* kx, const int32_t* slot_src, const float* gx, const float* gy,
+                    int64_t num_frames, int64_t ht, int64_t wd, int64_t hs, int64_t ws, float alpha, float ep,
+                    float lm, void* workspace, size_t workspace_bytes, float* disps_out, float* dscales_out,
+                    float* dzcov, void* stream);
+
 /* Dense symmetric positive-definite solve A x = b (column-major n x n, F64 or F32) with the
  * tiled Cholesky that ba uses for its reduced camera system (replaces the host Eigen
  * SimplicialLLT of SparseBlock::solve, src/droid_kernels.cu:1659-1680).  A is not modified;

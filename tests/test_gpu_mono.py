@@ -1,6 +1,6 @@
 """GPU parity of the mono-prior step pieces (droid_backends.proj_trans, droid_backends.bi_inter)
 and of hislam2_amd.geom.ba.JDSA against the float64 oracle (oracle/mono.py), on the 4-keyframe
-problem at 48x64 with a 2x2 scale grid per frame.
+problem at 48x64 with a 2x2 (and a 3x3) scale grid per frame.
 
 Tolerances (relative to the largest oracle value, max norm): proj_trans C and w 1e-5 (FP32
 pixel geometry, FP64 sums); bi_inter outputs and Jacobian 1e-6 absolute; JDSA disparities,
@@ -58,10 +58,11 @@ def test_bi_inter_matches_oracle(db):
     assert np.max(np.abs(J.cpu().numpy() - rJ)) <= 1e-6
 
 
-@pytest.mark.parametrize('drop', [None, 2])
-def test_jdsa_matches_oracle(db, drop):
+@pytest.mark.parametrize('drop,grid', [(None, 2), (2, 2), (None, 3)])
+def test_jdsa_matches_oracle(db, drop, grid):
+    """2 x 2 grids run on jdsa_solve's kernels, 3 x 3 grids on the batched torch path."""
     from hislam2_amd.geom import ba as gba
-    q = synth.mono_problem(seed=12, drop_frame=drop)
+    q = synth.mono_problem(seed=12, drop_frame=drop, hs=grid, ws=grid)
     N = q['poses'].shape[0]
     dsc = dev(q['dscales'])
     disps, dscales, dzcov = gba.JDSA(dev(q['targets']), dev(q['weights']), dev(q['eta']), dev(q['poses'])[None],
