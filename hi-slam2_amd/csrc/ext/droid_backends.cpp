@@ -481,6 +481,88 @@ std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, 
 
 }  // namespace
 
+// ---- edge-selection and point-cloud geometry (src/droid.cpp:170-241) ----------------------
+namespace {
+void check_geometry(torch::Tensor& poses, torch::Tensor& disps, torch::Tensor& intrinsics, const char* what) {
+  for (auto* t : {&poses, &disps, &intrinsics}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, what);
+    require(t->scalar_type() == torch::kFloat32, std::string(what) + ": poses, disps, intrinsics must be float32");
+  }
+  require(poses.dim() == 2 && poses.size(1) == 7 && disps.dim() == 3 && disps.size(0) <= poses.size(0),
+          std::string(what) + ": poses [N, 7], disps [N, h, w]");
+  require(intrinsics.numel() >= 4, std::string(what) + ": intrinsics needs fx, fy, cx, cy");
+}
+
+void check_index(torch::Tensor& idx, int64_t lo_ok, int64_t hi_ok, int64_t lo_shift, int64_t hi_shift,
+                 const char* what) {
+  CHECK_CONTIGUOUS(idx);
+  require_device(idx, what);
+  require(idx.scalar_type() == torch::kInt64 && idx.dim() == 1, std::string(what) + ": indices must be int64 [E]");
+  if (idx.numel() > 0) {
+    const int64_t lo = idx.min().item<int64_t>() + lo_shift, hi = idx.max().item<int64_t>() + hi_shift;
+    if (lo < lo_ok || hi > hi_ok) throw py::value_error(std::string(what) + ": an index refers to a frame outside the range");
+  }
+}
+}  // namespace
+
+torch::Tensor frame_distance(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics, torch::Tensor ii,
+                             torch::Tensor jj, double beta) {
+  check_geometry(poses, disps, intrinsics, "frame_distance");
+  const int64_t N = disps.size(0);
+  check_index(ii, 0, N - 1, 0, 0, "frame_distance");
+  check_index(jj, 0, N - 1, 0, 0, "frame_distance");
+  require(ii.numel() == jj.numel(), "frame_distance: ii and jj differ in length");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  torch::Tensor dist = torch::empty({ii.numel()}, poses.options());
+  raise_status(hsl2_frame_distance(poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                                   ii.data_ptr<int64_t>(), jj.data_ptr<int64_t>(), ii.numel(), disps.size(1),
+                                   disps.size(2), static_cast<float>(beta), dist.data_ptr<float>(), current_stream()),
+               "frame_distance");
+  return dist;
+}
+
+torch::Tensor covis_distance(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics, torch::Tensor ii) {
+  check_geometry(poses, disps, intrinsics, "covis_distance");
+  const int64_t N = disps.size(0);
+  check_index(ii, 0, N - 1, -1, 1, "covis_distance");   // ii - 1 and ii + 1 must be frames
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  torch::Tensor dist = torch::empty({ii.numel()}, poses.options());
+  raise_status(hsl2_covis_distance(poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                                   ii.data_ptr<int64_t>(), ii.numel(), disps.size(1), disps.size(2),
+                                   dist.data_ptr<float>(), current_stream()),
+               "covis_distance");
+  return dist;
+}
+
+torch::Tensor depth_filter(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics, torch::Tensor ix,
+                           torch::Tensor thresh) {
+  check_geometry(poses, disps, intrinsics, "depth_filter");
+  const int64_t N = disps.size(0);
+  check_index(ix, 0, N - 1, 0, 0, "depth_filter");
+  CHECK_CONTIGUOUS(thresh);
+  require_device(thresh, "depth_filter");
+  require(thresh.scalar_type() == torch::kFloat32 && thresh.numel() == ix.numel(),
+          "depth_filter: thresh must be float32 with one value per index");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  torch::Tensor counter = torch::empty({ix.numel(), disps.size(1), disps.size(2)}, disps.options());
+  raise_status(hsl2_depth_filter(poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                                 ix.data_ptr<int64_t>(), thresh.data_ptr<float>(), ix.numel(), N, disps.size(1),
+                                 disps.size(2), counter.data_ptr<float>(), current_stream()),
+               "depth_filter");
+  return counter;
+}
+
+torch::Tensor iproj(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics) {
+  check_geometry(poses, disps, intrinsics, "iproj");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  torch::Tensor points = torch::empty({disps.size(0), disps.size(1), disps.size(2), 3}, disps.options());
+  raise_status(hsl2_iproj(poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                          disps.size(0), disps.size(1), disps.size(2), points.data_ptr<float>(), current_stream()),
+               "iproj");
+  return points;
+}
+
 // ---- mono-prior step pieces (src/droid.cpp:121-138, 244-250) ----------------------------
 // proj_trans(poses [N, 7], disps [N, h, w], intrinsics (first 4 values used), targets / weights
 // [E, 2, h, w], ii, jj) -> [C, w], each [M, h * w] over the sorted unique ii frames.
@@ -808,6 +890,13 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("proj_trans", &proj_trans, "depth-only terms summed per ii frame: [C, w]", py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("targets"), py::arg("weights"), py::arg("ii"),
         py::arg("jj"));
+  m.def("frame_distance", &frame_distance, "beta-weighted mean flow of each edge", py::arg("poses"),
+        py::arg("disps"), py::arg("intrinsics"), py::arg("ii"), py::arg("jj"), py::arg("beta"));
+  m.def("covis_distance", &covis_distance, "fraction of pixels leaving both neighbours", py::arg("poses"),
+        py::arg("disps"), py::arg("intrinsics"), py::arg("ii"));
+  m.def("depth_filter", &depth_filter, "multi-view depth consistency counts", py::arg("poses"), py::arg("disps"),
+        py::arg("intrinsics"), py::arg("ix"), py::arg("thresh"));
+  m.def("iproj", &iproj, "pixels lifted to 3-D points", py::arg("poses"), py::arg("disps"), py::arg("intrinsics"));
   m.def("jdsa_solve", &jdsa_solve, "JDSA reduced system per frame slot and update: [disps, dzcov]",
         py::arg("C"), py::arg("w"), py::arg("disps"), py::arg("prior"), py::arg("dscales"), py::arg("eta"),
         py::arg("kx"), py::arg("slot_src"), py::arg("gx"), py::arg("gy"), py::arg("alpha"), py::arg("ep"),

@@ -242,6 +242,38 @@ int hsl2_projective_transform_backward(const float* poses, const float* disps, c
                                        const float* grad_coords, float* grad_poses, float* grad_disps,
                                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- edge-selection and point-cloud geometry (src/droid.cpp:170-241) ------------------------
+ * poses [N, 7] (tx ty tz qx qy qz qw), disps [N, ht, wd], intrinsics [4] (fx fy cx cy), all f32 on
+ * the device; index arrays int64.  Pixel arithmetic in FP32 as the reference, per-pair sums in
+ * FP64. */
+
+/* Replaces frame_distance_cuda (src/droid_kernels.cu:860-992, 2047-2069): dist[e] = the
+ * beta-weighted mean flow magnitude of edge (ii[e], jj[e]) under the full relative motion and
+ * under its translation alone, over pixels with transformed depth > 0.25; 1000 when the valid
+ * weight is below 75 %. */
+int hsl2_frame_distance(const float* poses, const float* disps, const float* intrinsics, const int64_t* ii,
+                        const int64_t* jj, int64_t num_edges, int64_t ht, int64_t wd, float beta, float* dist,
+                        void* stream);
+
+/* Replaces covis_distance_cuda (src/droid_kernels.cu:994-1092, 2071-2090): dist[e] = the fraction
+ * of frame ii[e]'s pixels that project outside the image in both ii[e] - 1 and ii[e] + 1 (both
+ * must be frames). */
+int hsl2_covis_distance(const float* poses, const float* disps, const float* intrinsics, const int64_t* ii,
+                        int64_t num, int64_t ht, int64_t wd, float* dist, void* stream);
+
+/* Replaces depth_filter_cuda (src/droid_kernels.cu:1094-1200, 2093-2117): counter[b][pixel] (f32,
+ * every entry written) = the number of neighbours inds[b] - 1, - 2, - 3, + 3, + 4, + 5 in
+ * [0, num_frames) whose disparity at one of the four integer corners around the projected pixel
+ * agrees with the projected depth within thresh[b] (|1/dj - 1/d| in double). */
+int hsl2_depth_filter(const float* poses, const float* disps, const float* intrinsics, const int64_t* inds,
+                      const float* thresh, int64_t count, int64_t num_frames, int64_t ht, int64_t wd,
+                      float* counter, void* stream);
+
+/* Replaces iproj_cuda (src/droid_kernels.cu:1204-1273, 2120-2143): points [N, ht, wd, 3] = each
+ * pixel lifted with its disparity and moved by its frame's pose, divided by the disparity. */
+int hsl2_iproj(const float* poses, const float* disps, const float* intrinsics, int64_t num_frames, int64_t ht,
+               int64_t wd, float* points, void* stream);
+
 /* ---- mono-prior depth/scale step pieces --------------------------------------------------- */
 
 /* Host only.  Groups edges by their ii frame: kx[num_frames] = sorted unique ii (torch.unique,
