@@ -39,9 +39,10 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
-    ap.add_argument('--only', choices=['all', 'lookup', 'mono'], default='all',
+    ap.add_argument('--only', choices=['all', 'lookup', 'mono', 'geometry'], default='all',
                     help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
-                         'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph')
+                         'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph; '
+                         'geometry: time frame_distance, depth_filter and iproj on it')
     return ap.parse_args()
 
 
@@ -106,6 +107,9 @@ def main():
 
     if args.only == 'mono':
         mono_bench(db, p, up, dev, args)
+        return
+    if args.only == 'geometry':
+        geometry_bench(db, p, up, args)
         return
 
     def ba_call(poses, disps, precision):
@@ -289,6 +293,36 @@ def mono_bench(db, p, up, dev, args):
                       'proj_trans_GBps': round(pt_bytes / (res['proj_trans'] / 1e3) / 1e9, 1),
                       'proj_trans_bytes': pt_bytes, 'frames': M, 'edges': len(p.ii),
                       'note': 'proj_trans_ms includes the host-side edge grouping and its upload'}), flush=True)
+
+
+def geometry_bench(db, p, up, args):
+    """The edge-selection kernels on the global300 graph: frame_distance of all 3000 edges,
+    depth_filter of all 300 frames (six neighbours each), iproj of all 300 frames."""
+    import numpy as np
+    import torch
+    poses, disps, intr = up(p.poses), up(p.disps), up(p.intrinsics)
+    ii, jj = up(p.ii), up(p.jj)
+    n = p.poses.shape[0]
+    inds = torch.arange(n, device=disps.device)
+    thresh = torch.full((n,), 0.05, device=disps.device)
+    calls = {'frame_distance': lambda: db.frame_distance(poses, disps, intr, ii, jj, 0.7),
+             'depth_filter': lambda: db.depth_filter(poses, disps, intr, inds, thresh),
+             'iproj': lambda: db.iproj(poses, disps, intr)}
+    res = {}
+    for name, fn in calls.items():
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name + '_ms'] = round(e0.elapsed_time(e1) / args.steps, 4)
+    print(json.dumps({'metric': 'edge-selection geometry on global300', **res, 'edges': len(p.ii), 'frames': n,
+                      'note': 'frame_distance_ms and depth_filter_ms include the host-side index checks'}),
+          flush=True)
 
 
 def lookup_traffic():
