@@ -302,53 +302,15 @@ __device__ __forceinline__ void gather_segs24(const T* __restrict__ vbase, int64
   }
 }
 
+// Window gather of a row-segment level (tiles, 1x8 groups or rows) for the vec kernels: each
+// thread fetches two window rows as two aligned 16-byte chunks each (exec-masked: no load, so
+// no line, for a chunk outside the plane or the unneeded second chunk of an aligned window)
+// and extracts the 8 taps with a barrel shift; all four loads of a thread are issued first.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
-                       int64_t cy_off, T* __restrict__ out, int hw1, int64_t out_batch_stride,
-                       int tiles_per_item) {
-  constexpr int RAD = 3, D = 8, R = 7;
-  constexpr int TS = D * D + 1;                 // padded per-pixel tap stride
-  __shared__ float taps[kPix * TS];
-  __shared__ int s_x0[kPix], s_y0[kPix];
-  __shared__ float s_dx[kPix], s_dy[kPix];
-
-  const int level = blockIdx.y;
-  const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
-  const int h2 = lv.h2[level], w2 = lv.w2[level];
-  const int layout = lv.layout[level];
-  const float scale = lv.scale[level];
-  const int n = blockIdx.x / tiles_per_item;
-  const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
-  const int npix = min(kPix, hw1 - pix0);
+__device__ __forceinline__ void gather_rows8(const T* __restrict__ vbase, int64_t plane, int h2, int w2, int layout,
+                                             int npix, const int* s_x0, const int* s_y0, float* taps) {
+  constexpr int D = 8, TS = D * D + 1;
   const int t = threadIdx.x;
-
-  if (t < kPix) {
-    int x0 = 0, y0 = 0;
-    float fx = 0.f, fy = 0.f;
-    if (t < npix) {
-      const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
-      const float cx = cp_[0] * scale;
-      const float cy = cp_[cy_off] * scale;
-      const float flx = floorf(cx), fly = floorf(cy);
-      x0 = static_cast<int>(flx) - RAD;
-      y0 = static_cast<int>(fly) - RAD;
-      fx = cx - flx;
-      fy = cy - fly;
-    }
-    s_x0[t] = x0;
-    s_y0[t] = y0;
-    s_dx[t] = fx;
-    s_dy[t] = fy;
-  }
-  __syncthreads();
-
-  const int64_t plane = (int64_t)h2 * w2;
-  const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
-  if (layout == kPlaneGroups24) {
-    gather_segs24<T>(vbase, plane, h2, w2, npix, s_x0, s_y0, taps);
-  } else {
-  // gather: item = (pixel, window row); all four loads of a thread are issued first
   uint4 ca[2], cbv[2];
   int sh[2];
 #pragma unroll
@@ -400,7 +362,53 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
     dst[6] = half_bits_to_f<T>(d3 & 0xffffu);
     dst[7] = half_bits_to_f<T>(d3 >> 16);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp,
+                       int64_t cy_off, T* __restrict__ out, int hw1, int64_t out_batch_stride,
+                       int tiles_per_item) {
+  constexpr int RAD = 3, D = 8, R = 7;
+  constexpr int TS = D * D + 1;                 // padded per-pixel tap stride
+  __shared__ float taps[kPix * TS];
+  __shared__ int s_x0[kPix], s_y0[kPix];
+  __shared__ float s_dx[kPix], s_dy[kPix];
+
+  const int level = blockIdx.y;
+  const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
+  const int h2 = lv.h2[level], w2 = lv.w2[level];
+  const int layout = lv.layout[level];
+  const float scale = lv.scale[level];
+  const int n = blockIdx.x / tiles_per_item;
+  const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
+  const int npix = min(kPix, hw1 - pix0);
+  const int t = threadIdx.x;
+
+  if (t < kPix) {
+    int x0 = 0, y0 = 0;
+    float fx = 0.f, fy = 0.f;
+    if (t < npix) {
+      const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
+      const float cx = cp_[0] * scale;
+      const float cy = cp_[cy_off] * scale;
+      const float flx = floorf(cx), fly = floorf(cy);
+      x0 = static_cast<int>(flx) - RAD;
+      y0 = static_cast<int>(fly) - RAD;
+      fx = cx - flx;
+      fy = cy - fly;
+    }
+    s_x0[t] = x0;
+    s_y0[t] = y0;
+    s_dx[t] = fx;
+    s_dy[t] = fy;
   }
+  __syncthreads();
+
+  const int64_t plane = (int64_t)h2 * w2;
+  const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
+  if (layout == kPlaneGroups24) gather_segs24<T>(vbase, plane, h2, w2, npix, s_x0, s_y0, taps);
+  else gather_rows8<T>(vbase, plane, h2, w2, layout, npix, s_x0, s_y0, taps);
   __syncthreads();
 
   // combine: thread = (pixel quad q, channel group g); channel o = g, g + 16, g + 32, g + 48
@@ -579,6 +587,132 @@ int launch_bwd(const float* coords, const void* grad, void* vgrad, int64_t batch
   return HSL2_OK;
 }
 
+// ---- the lookup fused with the first 1x1 convolution of its consumer ----------------------
+// corr_encoder's first layer (hislam2/modules/droid_net.py:86-90: Conv2d(196, C_out, 1) and
+// ReLU) applied to the CorrBlock lookup without writing the 196 channels: out[n, c, p] =
+// act(bias[c] + sum_k W[c, k] corr[n, k, p]), k in the lookup's channel order
+// [level][x-offset][y-offset].  One 256-thread block per 64 pixels of an item: per level the
+// vec kernel's gather and bilinear combine, each channel rounded to bf16 (as the lookup's own
+// output) into an LDS row per pixel; then the 64 x C_out x 196 product on
+// v_mfma_f32_32x32x8bf16_1k (FP32 accumulation, W's fragments read from L2), bias, ReLU and
+// bf16 stores of 4 consecutive pixels.  bf16 pyramids of 4 levels, radius 3.
+typedef short conv_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float conv_acc16 __attribute__((ext_vector_type(16)));
+constexpr int kConvK = 196;                     // 4 levels x 49 channels
+constexpr int kConvKP = 200;                    // padded to whole MFMA steps of 8
+constexpr int kConvRow = kConvKP + 8;           // LDS row (bf16), 416 bytes
+
+template <int COUT>
+__global__ void __launch_bounds__(kThreads)
+corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t cb, int cp, int64_t cy_off,
+                        const __hip_bfloat16* __restrict__ W, const float* __restrict__ bias, int relu,
+                        __hip_bfloat16* __restrict__ out, int hw1, int tiles_per_item) {
+  using T = __hip_bfloat16;
+  constexpr int RAD = 3, D = 8, R = 7, TS = D * D + 1;
+  __shared__ float taps[kPix * TS];
+  __shared__ int s_x0[kPix], s_y0[kPix];
+  __shared__ float s_dx[kPix], s_dy[kPix];
+  __shared__ __align__(16) unsigned short A[kPix][kConvRow];
+  const int n = blockIdx.x / tiles_per_item;
+  const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
+  const int npix = min(kPix, hw1 - pix0);
+  const int t = threadIdx.x;
+  if (t < kPix)
+    for (int k = kConvK; k < kConvRow; ++k) A[t][k] = 0;
+  for (int level = 0; level < 4; ++level) {
+    const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
+    const int h2 = lv.h2[level], w2 = lv.w2[level], layout = lv.layout[level];
+    const float scale = lv.scale[level];
+    if (t < kPix) {
+      int x0 = 0, y0 = 0;
+      float fx = 0.f, fy = 0.f;
+      if (t < npix) {
+        const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
+        const float cx = cp_[0] * scale, cy = cp_[cy_off] * scale;
+        const float flx = floorf(cx), fly = floorf(cy);
+        x0 = static_cast<int>(flx) - RAD;
+        y0 = static_cast<int>(fly) - RAD;
+        fx = cx - flx;
+        fy = cy - fly;
+      }
+      s_x0[t] = x0;
+      s_y0[t] = y0;
+      s_dx[t] = fx;
+      s_dy[t] = fy;
+    }
+    __syncthreads();
+    const int64_t plane = (int64_t)h2 * w2;
+    const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
+    if (layout == kPlaneGroups24) gather_segs24<T>(vbase, plane, h2, w2, npix, s_x0, s_y0, taps);
+    else gather_rows8<T>(vbase, plane, h2, w2, layout, npix, s_x0, s_y0, taps);
+    __syncthreads();
+    // combine: thread (pixel p, channel group g) forms channels o = g, g + 4, ... < 49
+    {
+      const int p = t & 63, g = t >> 6;
+      const float fx = s_dx[p], fy = s_dy[p];
+      const float w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy), w01 = (1.f - fx) * fy, w11 = fx * fy;
+      const float* tp = taps + p * TS;
+      for (int o = g; o < R * R; o += 4) {
+        const int a = o / R, b = o - (o / R) * R;
+        const float v = w00 * tp[b * D + a] + w10 * tp[b * D + a + 1] + w01 * tp[(b + 1) * D + a] +
+                        w11 * tp[(b + 1) * D + a + 1];
+        A[p][level * R * R + o] = static_cast<unsigned short>(f_to_half_bits<T>(v));
+      }
+    }
+    __syncthreads();                              // taps and the coordinates are reused
+  }
+  // 64 pixels x COUT outputs in 32 x 32 tiles: wave w takes pixel half w & 1 and the output
+  // tiles of its other index bit
+  constexpr int NT = COUT / 32 / 2;               // output tiles per wave
+  const int lane = t % kWave, wave = t / kWave;
+  const int ph = wave & 1, ob = wave >> 1;
+  const int row = ph * 32 + lane % 32, kh = 4 * (lane / 32);
+  conv_acc16 acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[j][v] = 0.f;
+  const unsigned short* arow = &A[row][0];
+#pragma unroll 5
+  for (int k0 = 0; k0 < kConvKP; k0 += 8) {
+    const conv_bf16x4 a = *reinterpret_cast<const conv_bf16x4*>(arow + k0 + kh);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int oc = (ob * NT + j) * 32 + lane % 32;
+      conv_bf16x4 b = {0, 0, 0, 0};
+      if (k0 + kh < kConvK) b = *reinterpret_cast<const conv_bf16x4*>(W + (int64_t)oc * kConvK + k0 + kh);
+      acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[j], 0, 0, 0);
+    }
+  }
+  // D value v of lane l: pixel row 8 (v / 4) + 4 (l / 32) + v % 4 of the half, column l % 32
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int oc = (ob * NT + j) * 32 + lane % 32;
+    const float bb = bias[oc];
+    T* obase = out + ((int64_t)n * COUT + oc) * hw1 + pix0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int px = ph * 32 + 8 * q + 4 * (lane / 32);
+      uint32_t h[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float y = acc[j][4 * q + v] + bb;
+        if (relu) y = fmaxf(y, 0.f);
+        h[v] = f_to_half_bits<T>(y);
+      }
+      if (px + 3 < npix) {
+        uint2 pk;
+        pk.x = h[0] | (h[1] << 16);
+        pk.y = h[2] | (h[3] << 16);
+        *reinterpret_cast<uint2*>(obase + px) = pk;
+      } else {
+        for (int v = 0; v < 4; ++v)
+          if (px + v < npix) reinterpret_cast<unsigned short*>(obase)[px + v] = static_cast<unsigned short>(h[v]);
+      }
+    }
+  }
+}
+
 int check_lookup_args(const void* a, const void* b, const void* c, int64_t batch, int64_t h1,
                       int64_t w1, int64_t h2, int64_t w2, int radius) {
   HSL2_REQUIRE(a && b && c, HSL2_ERR_ARGUMENT, "corr lookup: null pointer");
@@ -665,4 +799,44 @@ extern "C" int hsl2_corr_index_backward(const float* coords, const void* corr_gr
     case HSL2_F64: return launch_bwd<double>(coords, corr_grad, volume_grad, batch, hw1, h2, w2, radius, s);
     default: set_error("corr lookup: unknown dtype"); return HSL2_ERR_ARGUMENT;
   }
+}
+
+extern "C" int hsl2_corr_lookup_conv(const void* const* levels, int num_levels, const float* coords,
+                                     const void* weight, const float* bias, int relu, void* out, int64_t batch,
+                                     int64_t h1, int64_t w1, int radius, int tiled, int64_t cout, void* stream) {
+  HSL2_REQUIRE(levels && coords && weight && bias && out, HSL2_ERR_ARGUMENT, "corr lookup conv: null pointer");
+  HSL2_REQUIRE(num_levels == 4 && radius == 3, HSL2_ERR_UNSUPPORTED,
+               "corr lookup conv: 4 levels of radius 3 (196 channels)");
+  HSL2_REQUIRE(cout == 128 || cout == 64, HSL2_ERR_UNSUPPORTED, "corr lookup conv: 64 or 128 output channels");
+  const int64_t hw1 = h1 * w1;
+  HSL2_REQUIRE(batch >= 0 && h1 > 0 && w1 > 0 && hw1 % 4 == 0 && hw1 < (1ll << 31), HSL2_ERR_SHAPE,
+               "corr lookup conv: h * w must be a multiple of 4");
+  LevelTable lv{};
+  for (int i = 0; i < 4; ++i) {
+    const int64_t h2 = h1 >> i, w2 = w1 >> i;
+    HSL2_REQUIRE(levels[i] && w2 % 8 == 0 && h2 > 0 && reinterpret_cast<uintptr_t>(levels[i]) % 16 == 0,
+                 HSL2_ERR_UNSUPPORTED, "corr lookup conv: every level width a multiple of 8, 16-byte aligned");
+    lv.vol[i] = levels[i];
+    lv.h2[i] = static_cast<int>(h2);
+    lv.w2[i] = static_cast<int>(w2);
+    lv.layout[i] = pyramid_layout(i, hw1, h2, w2, tiled);
+    lv.scale[i] = 1.0f / static_cast<float>(1 << i);
+  }
+  HSL2_REQUIRE(reinterpret_cast<uintptr_t>(weight) % 8 == 0 && reinterpret_cast<uintptr_t>(out) % 8 == 0,
+               HSL2_ERR_ARGUMENT, "corr lookup conv: weight and out must be 8-byte aligned");
+  if (batch == 0) return HSL2_OK;
+  const int tiles = static_cast<int>((hw1 + kPix - 1) / kPix);
+  HSL2_REQUIRE(batch * tiles < (1ll << 31), HSL2_ERR_SHAPE, "corr lookup conv: grid too large");
+  dim3 grid(static_cast<unsigned>(batch * tiles));
+  const auto* Wb = static_cast<const __hip_bfloat16*>(weight);
+  auto* ob = static_cast<__hip_bfloat16*>(out);
+  // interleaved coords [batch][h1*w1][2], as hsl2_corr_pyramid_lookup
+  if (cout == 128)
+    hipLaunchKernelGGL(corr_lookup_conv_kernel<128>, grid, dim3(kThreads), 0, as_stream(stream), lv, coords,
+                       2 * hw1, 2, int64_t{1}, Wb, bias, relu, ob, static_cast<int>(hw1), tiles);
+  else
+    hipLaunchKernelGGL(corr_lookup_conv_kernel<64>, grid, dim3(kThreads), 0, as_stream(stream), lv, coords,
+                       2 * hw1, 2, int64_t{1}, Wb, bias, relu, ob, static_cast<int>(hw1), tiles);
+  HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
 }

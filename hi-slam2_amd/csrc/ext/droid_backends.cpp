@@ -307,6 +307,43 @@ torch::Tensor corr_pyramid_lookup(std::vector<torch::Tensor> levels, torch::Tens
   return out;
 }
 
+// The CorrBlock lookup fused with a 1x1 convolution (196 -> C_out) + optional ReLU: bf16
+// pyramid of 4 levels, radius 3; weight [C_out, 196] (or [C_out, 196, 1, 1]) bf16, bias [C_out]
+// float32 -> [B, N, C_out, h, w] bf16.
+torch::Tensor corr_lookup_conv(std::vector<torch::Tensor> levels, torch::Tensor coords, int radius, bool tiled,
+                               torch::Tensor weight, torch::Tensor bias, bool relu) {
+  CHECK_CONTIGUOUS(coords);
+  CHECK_CONTIGUOUS(weight);
+  CHECK_CONTIGUOUS(bias);
+  require(levels.size() == 4 && radius == 3, "corr_lookup_conv: 4 levels of radius 3");
+  require(coords.dim() == 5 && coords.size(4) == 2 && coords.scalar_type() == torch::kFloat32,
+          "corr_lookup_conv: coords must be float32 [B, N, h, w, 2]");
+  const int64_t B = coords.size(0), N = coords.size(1), h = coords.size(2), w = coords.size(3);
+  std::vector<const void*> ptrs;
+  for (size_t i = 0; i < levels.size(); ++i) {
+    CHECK_CONTIGUOUS(levels[i]);
+    require_device(levels[i], "levels");
+    require(levels[i].scalar_type() == torch::kBFloat16, "corr_lookup_conv: bf16 pyramid levels");
+    require(levels[i].dim() == 5 && levels[i].size(0) == B * N && levels[i].size(1) == h &&
+                levels[i].size(2) == w && levels[i].size(3) == (h >> i) && levels[i].size(4) == (w >> i),
+            "corr_lookup_conv: level " + std::to_string(i) + " must be [B*N, h, w, h>>i, w>>i]");
+    ptrs.push_back(levels[i].data_ptr());
+  }
+  require_device(weight, "weight");
+  require_device(bias, "bias");
+  require(weight.scalar_type() == torch::kBFloat16 && weight.size(0) > 0 && weight.numel() == weight.size(0) * 196,
+          "corr_lookup_conv: weight must be bf16 [C_out, 196]");
+  const int64_t cout = weight.size(0);
+  require(bias.scalar_type() == torch::kFloat32 && bias.numel() == cout, "corr_lookup_conv: bias must be float32 [C_out]");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(coords.device());
+  torch::Tensor out = torch::empty({B, N, cout, h, w}, levels[0].options());
+  raise_status(hsl2_corr_lookup_conv(ptrs.data(), 4, coords.data_ptr<float>(), weight.data_ptr(),
+                                     bias.data_ptr<float>(), relu ? 1 : 0, out.data_ptr(), B * N, h, w, radius,
+                                     tiled ? 1 : 0, cout, current_stream()),
+               "corr_lookup_conv");
+  return out;
+}
+
 // ---- altcorr (src/droid.cpp:277-305) ---------------------------------------------------
 struct AltDims {
   int64_t B, N1, N2, C, H, W, H2, W2, M;
@@ -880,6 +917,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("spd_solve", &spd_solve, "dense SPD solve with the tiled GPU Cholesky; returns [x, ok]");
   m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch",
         py::arg("levels"), py::arg("coords"), py::arg("radius"), py::arg("tiled") = false);
+  m.def("corr_lookup_conv", &corr_lookup_conv, "CorrBlock lookup fused with a 1x1 convolution (+ ReLU)",
+        py::arg("levels"), py::arg("coords"), py::arg("radius"), py::arg("tiled"), py::arg("weight"),
+        py::arg("bias"), py::arg("relu") = true);
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
         py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none(),
         py::arg("tiled") = false);

@@ -134,6 +134,19 @@ class CorrBlock:
             out_pyramid.append(corr.view(batch, num, -1, ht, wd))
         return torch.cat(out_pyramid, dim=2)
 
+    def lookup_conv1x1(self, coords, weight, bias, relu=True):
+        """The lookup followed by a 1x1 convolution (+ ReLU), fused: equals
+        ``relu(conv2d(self(coords).view(B*N, 196, h, w), weight, bias))`` viewed as
+        [B, N, C_out, h, w] -- corr_encoder's first layer (hislam2/modules/droid_net.py:86-90)
+        -- without writing the 196 lookup channels.  bf16 pyramids of 4 levels, radius 3, no
+        autograd; weight [C_out, 196(, 1, 1)] with C_out 64 or 128, bias [C_out]."""
+        if self.num_levels != 4 or self.radius != 3:
+            raise ValueError('lookup_conv1x1: 4 levels of radius 3 (196 channels)')
+        w = weight.detach().reshape(weight.shape[0], -1).to(torch.bfloat16).contiguous()
+        b = bias.detach().float().contiguous()
+        return _db.corr_lookup_conv(self.corr_pyramid, coords.float().contiguous(), self.radius, self.tiled,
+                                    w, b, bool(relu))
+
     def cat(self, other):
         for i in range(self.num_levels):
             self.corr_pyramid[i] = torch.cat([self.corr_pyramid[i], other.corr_pyramid[i]], 0)

@@ -83,6 +83,18 @@ int hsl2_corr_pyramid_lookup(const void* const* levels, int num_levels, const fl
                              void* corr, int dtype, int64_t batch, int64_t h1, int64_t w1,
                              int radius, int tiled, void* stream);
 
+/* No reference counterpart (the next step of the path: the lookup fused into its consumer).
+ * The CorrBlock lookup of hsl2_corr_pyramid_lookup (4 bf16 levels, radius 3, 196 channels in
+ * its channel order) followed by corr_encoder's first layer (hislam2/modules/droid_net.py:86-90:
+ * a 1x1 convolution to cout channels, bias, ReLU when relu != 0), without storing the 196
+ * channels: out [batch, cout, h1, w1] bf16 = act(bias + weight . corr), weight [cout, 196] bf16,
+ * bias [cout] f32; each lookup channel is rounded to bf16 (as the lookup's own output) before the
+ * product, which accumulates in FP32.  cout 64 or 128; h1 * w1 a multiple of 4; every level
+ * width a multiple of 8. */
+int hsl2_corr_lookup_conv(const void* const* levels, int num_levels, const float* coords, const void* weight,
+                          const float* bias, int relu, void* out, int64_t batch, int64_t h1, int64_t w1,
+                          int radius, int tiled, int64_t cout, void* stream);
+
 /* ---- correlation build ------------------------------------------------------ */
 
 /* Replaces CorrBlock.__init__ + CorrBlock.corr (hislam2/modules/corr.py:24-38, 63-71):
