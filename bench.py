@@ -100,9 +100,35 @@ def main():
         torch.cuda.synchronize()
         ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
         gbps = LOOKUP_BYTES_BF16 * lookups / (ms / 1e3) / 1e9
+        # the lookup with corr_encoder's first layer (1x1 conv 196 -> 128 + ReLU): fused, and as
+        # the lookup followed by torch's bf16 convolution
+        import torch.nn.functional as F
+        g = torch.Generator().manual_seed(0)
+        W = (torch.randn(128, 196, 1, 1, generator=g) / 14.0).to(dev).to(torch.bfloat16)
+        bias = torch.zeros(128, device=dev)
+        B_, N_, h_, w_ = coords.shape[:4]
+
+        def fused():
+            return blk.lookup_conv1x1(coords, W, bias)
+
+        def unfused():
+            return F.relu(F.conv2d(blk(coords).view(B_ * N_, 196, h_, w_), W, bias.to(torch.bfloat16)))
+
+        conv_ms = {}
+        for name, fn in (('lookup_conv_fused_ms', fused), ('lookup_then_conv_ms', unfused)):
+            for _ in range(max(args.warmup, 1)):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            conv_ms[name] = round(e0.elapsed_time(e1) / args.steps, 4)
         print(json.dumps({'metric': 'corr-lookup GB/s (algorithmic, 912 B per bf16 lookup)',
                           'value': round(gbps, 1), 'unit': 'GB/s', 'lookup_ms': round(ms, 4),
-                          'lookups_per_launch': lookups, 'tiled': blk.tiled}), flush=True)
+                          'lookups_per_launch': lookups, 'tiled': blk.tiled, **conv_ms}), flush=True)
         return
 
     if args.only == 'mono':

@@ -264,12 +264,21 @@ template <typename T> __device__ __forceinline__ uint32_t f_to_half_bits(float v
 // outside the plane: zeros) and writes the elements that fall inside the 8x8 window to its
 // taps.  Every tap of a window lies in exactly one such segment.  Loads first, then the
 // writes.
+// A thread's loads of one level's window gather, kept in registers between the load and the
+// store phase (so that a kernel can have several levels' loads in flight).
+struct WinRegs {
+  uint4 v[4];
+  int sh[2];
+};
+constexpr int kSegItems = 15 * kPix, kSegIters = (kSegItems + kThreads - 1) / kThreads;
+static_assert(kSegIters <= 4, "segment gather: four loads per thread at most");
+
 template <typename T>
-__device__ __forceinline__ void gather_segs24(const T* __restrict__ vbase, int64_t plane, int h2, int w2,
-                                              int npix, const int* s_x0, const int* s_y0, float* taps) {
-  constexpr int D = 8, TS = D * D + 1, NI = 15 * kPix, IT = (NI + kThreads - 1) / kThreads;
+__device__ __forceinline__ void load_segs24(const T* __restrict__ vbase, int64_t plane, int h2, int w2, int npix,
+                                            const int* s_x0, const int* s_y0, WinRegs& g) {
+  constexpr int NI = kSegItems, IT = kSegIters;
   const int t = threadIdx.x;
-  uint4 sv[IT];
+  uint4* sv = g.v;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int item = t + kThreads * it;
@@ -282,6 +291,13 @@ __device__ __forceinline__ void gather_segs24(const T* __restrict__ vbase, int64
         sv[it] = *reinterpret_cast<const uint4*>(vbase + pyramid_offset(kPlaneGroups24, p, 2 * yp, 4 * xq, w2, plane));
     }
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_segs24(const WinRegs& g, const int* s_x0, const int* s_y0, float* taps) {
+  constexpr int D = 8, TS = D * D + 1, NI = kSegItems, IT = kSegIters;
+  const int t = threadIdx.x;
+  const uint4* sv = g.v;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int item = t + kThreads * it;
@@ -302,17 +318,25 @@ __device__ __forceinline__ void gather_segs24(const T* __restrict__ vbase, int64
   }
 }
 
+template <typename T>
+__device__ __forceinline__ void gather_segs24(const T* __restrict__ vbase, int64_t plane, int h2, int w2,
+                                              int npix, const int* s_x0, const int* s_y0, float* taps) {
+  WinRegs g;
+  load_segs24<T>(vbase, plane, h2, w2, npix, s_x0, s_y0, g);
+  store_segs24<T>(g, s_x0, s_y0, taps);
+}
+
 // Window gather of a row-segment level (tiles, 1x8 groups or rows) for the vec kernels: each
 // thread fetches two window rows as two aligned 16-byte chunks each (exec-masked: no load, so
 // no line, for a chunk outside the plane or the unneeded second chunk of an aligned window)
 // and extracts the 8 taps with a barrel shift; all four loads of a thread are issued first.
 template <typename T>
-__device__ __forceinline__ void gather_rows8(const T* __restrict__ vbase, int64_t plane, int h2, int w2, int layout,
-                                             int npix, const int* s_x0, const int* s_y0, float* taps) {
-  constexpr int D = 8, TS = D * D + 1;
+__device__ __forceinline__ void load_rows8(const T* __restrict__ vbase, int64_t plane, int h2, int w2, int layout,
+                                           int npix, const int* s_x0, const int* s_y0, WinRegs& g) {
   const int t = threadIdx.x;
-  uint4 ca[2], cbv[2];
-  int sh[2];
+  uint4* ca = g.v;
+  uint4* cbv = g.v + 2;
+  int* sh = g.sh;
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
     const int item = t + kThreads * it;
@@ -334,6 +358,15 @@ __device__ __forceinline__ void gather_rows8(const T* __restrict__ vbase, int64_
     ca[it] = la;
     cbv[it] = lb;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_rows8(const WinRegs& g, float* taps) {
+  constexpr int D = 8, TS = D * D + 1;
+  const int t = threadIdx.x;
+  const uint4* ca = g.v;
+  const uint4* cbv = g.v + 2;
+  const int* sh = g.sh;
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
     const int item = t + kThreads * it;
@@ -362,6 +395,14 @@ __device__ __forceinline__ void gather_rows8(const T* __restrict__ vbase, int64_
     dst[6] = half_bits_to_f<T>(d3 & 0xffffu);
     dst[7] = half_bits_to_f<T>(d3 >> 16);
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void gather_rows8(const T* __restrict__ vbase, int64_t plane, int h2, int w2, int layout,
+                                             int npix, const int* s_x0, const int* s_y0, float* taps) {
+  WinRegs g;
+  load_rows8<T>(vbase, plane, h2, w2, layout, npix, s_x0, s_y0, g);
+  store_rows8<T>(g, taps);
 }
 
 template <typename T>
@@ -591,9 +632,10 @@ int launch_bwd(const float* coords, const void* grad, void* vgrad, int64_t batch
 // corr_encoder's first layer (hislam2/modules/droid_net.py:86-90: Conv2d(196, C_out, 1) and
 // ReLU) applied to the CorrBlock lookup without writing the 196 channels: out[n, c, p] =
 // act(bias[c] + sum_k W[c, k] corr[n, k, p]), k in the lookup's channel order
-// [level][x-offset][y-offset].  One 256-thread block per 64 pixels of an item: per level the
-// vec kernel's gather and bilinear combine, each channel rounded to bf16 (as the lookup's own
-// output) into an LDS row per pixel; then the 64 x C_out x 196 product on
+// [level][x-offset][y-offset].  One 256-thread block per 64 pixels of an item: the vec kernel's
+// window loads of all four levels are issued together (one memory latency per block), then per
+// level the taps go to LDS and the bilinear combine rounds each channel to bf16 (as the
+// lookup's own output) into an LDS row per pixel; then the 64 x C_out x 196 product on
 // v_mfma_f32_32x32x8bf16_1k (FP32 accumulation, W's fragments read from L2), bias, ReLU and
 // bf16 stores of 4 consecutive pixels.  bf16 pyramids of 4 levels, radius 3.
 typedef short conv_bf16x4 __attribute__((ext_vector_type(4)));
@@ -601,6 +643,7 @@ typedef float conv_acc16 __attribute__((ext_vector_type(16)));
 constexpr int kConvK = 196;                     // 4 levels x 49 channels
 constexpr int kConvKP = 200;                    // padded to whole MFMA steps of 8
 constexpr int kConvRow = kConvKP + 8;           // LDS row (bf16), 416 bytes
+static_assert(kThreads == 4 * kPix, "one thread per (level, pixel) for the window origins");
 
 template <int COUT>
 __global__ void __launch_bounds__(kThreads)
@@ -610,8 +653,8 @@ corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t
   using T = __hip_bfloat16;
   constexpr int RAD = 3, D = 8, R = 7, TS = D * D + 1;
   __shared__ float taps[kPix * TS];
-  __shared__ int s_x0[kPix], s_y0[kPix];
-  __shared__ float s_dx[kPix], s_dy[kPix];
+  __shared__ int s_x0[4][kPix], s_y0[4][kPix];
+  __shared__ float s_dx[4][kPix], s_dy[4][kPix];
   __shared__ __align__(16) unsigned short A[kPix][kConvRow];
   const int n = blockIdx.x / tiles_per_item;
   const int pix0 = (blockIdx.x - n * tiles_per_item) * kPix;
@@ -619,47 +662,56 @@ corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t
   const int t = threadIdx.x;
   if (t < kPix)
     for (int k = kConvK; k < kConvRow; ++k) A[t][k] = 0;
-  for (int level = 0; level < 4; ++level) {
-    const T* __restrict__ volume = static_cast<const T*>(lv.vol[level]);
-    const int h2 = lv.h2[level], w2 = lv.w2[level], layout = lv.layout[level];
+  // window origins of all four levels (thread t: level t / 64, pixel t % 64)
+  {
+    const int level = t / kPix, p = t % kPix;
     const float scale = lv.scale[level];
-    if (t < kPix) {
-      int x0 = 0, y0 = 0;
-      float fx = 0.f, fy = 0.f;
-      if (t < npix) {
-        const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + t) * cp;
-        const float cx = cp_[0] * scale, cy = cp_[cy_off] * scale;
-        const float flx = floorf(cx), fly = floorf(cy);
-        x0 = static_cast<int>(flx) - RAD;
-        y0 = static_cast<int>(fly) - RAD;
-        fx = cx - flx;
-        fy = cy - fly;
-      }
-      s_x0[t] = x0;
-      s_y0[t] = y0;
-      s_dx[t] = fx;
-      s_dy[t] = fy;
+    int x0 = 0, y0 = 0;
+    float fx = 0.f, fy = 0.f;
+    if (p < npix) {
+      const float* cp_ = coords + (int64_t)n * cb + (int64_t)(pix0 + p) * cp;
+      const float cx = cp_[0] * scale, cy = cp_[cy_off] * scale;
+      const float flx = floorf(cx), fly = floorf(cy);
+      x0 = static_cast<int>(flx) - RAD;
+      y0 = static_cast<int>(fly) - RAD;
+      fx = cx - flx;
+      fy = cy - fly;
     }
-    __syncthreads();
+    s_x0[level][p] = x0;
+    s_y0[level][p] = y0;
+    s_dx[level][p] = fx;
+    s_dy[level][p] = fy;
+  }
+  __syncthreads();
+  // every level's loads are issued before any is consumed: one memory latency per block
+  WinRegs g[4];
+#pragma unroll
+  for (int level = 0; level < 4; ++level) {
+    const int h2 = lv.h2[level], w2 = lv.w2[level], layout = lv.layout[level];
     const int64_t plane = (int64_t)h2 * w2;
-    const T* vbase = volume + ((int64_t)n * hw1 + pix0) * plane;
-    if (layout == kPlaneGroups24) gather_segs24<T>(vbase, plane, h2, w2, npix, s_x0, s_y0, taps);
-    else gather_rows8<T>(vbase, plane, h2, w2, layout, npix, s_x0, s_y0, taps);
+    const T* vbase = static_cast<const T*>(lv.vol[level]) + ((int64_t)n * hw1 + pix0) * plane;
+    if (layout == kPlaneGroups24) load_segs24<T>(vbase, plane, h2, w2, npix, s_x0[level], s_y0[level], g[level]);
+    else load_rows8<T>(vbase, plane, h2, w2, layout, npix, s_x0[level], s_y0[level], g[level]);
+  }
+#pragma unroll
+  for (int level = 0; level < 4; ++level) {
+    if (lv.layout[level] == kPlaneGroups24) store_segs24<T>(g[level], s_x0[level], s_y0[level], taps);
+    else store_rows8<T>(g[level], taps);
     __syncthreads();
     // combine: thread (pixel p, channel group g) forms channels o = g, g + 4, ... < 49
     {
-      const int p = t & 63, g = t >> 6;
-      const float fx = s_dx[p], fy = s_dy[p];
+      const int p = t & 63, cg = t >> 6;
+      const float fx = s_dx[level][p], fy = s_dy[level][p];
       const float w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy), w01 = (1.f - fx) * fy, w11 = fx * fy;
       const float* tp = taps + p * TS;
-      for (int o = g; o < R * R; o += 4) {
+      for (int o = cg; o < R * R; o += 4) {
         const int a = o / R, b = o - (o / R) * R;
         const float v = w00 * tp[b * D + a] + w10 * tp[b * D + a + 1] + w01 * tp[(b + 1) * D + a] +
                         w11 * tp[(b + 1) * D + a + 1];
         A[p][level * R * R + o] = static_cast<unsigned short>(f_to_half_bits<T>(v));
       }
     }
-    __syncthreads();                              // taps and the coordinates are reused
+    __syncthreads();                              // taps are reused by the next level
   }
   // 64 pixels x COUT outputs in 32 x 32 tiles: wave w takes pixel half w & 1 and the output
   // tiles of its other index bit
@@ -673,15 +725,36 @@ corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[j][v] = 0.f;
   const unsigned short* arow = &A[row][0];
-#pragma unroll 5
-  for (int k0 = 0; k0 < kConvKP; k0 += 8) {
-    const conv_bf16x4 a = *reinterpret_cast<const conv_bf16x4*>(arow + k0 + kh);
+  // W's fragments come from L2: steps in groups of KG, the next group's loads issued before this
+  // group's MFMAs
+  constexpr int KS = kConvKP / 8, KG = 5, NG = KS / KG;
+  static_assert(KS % KG == 0, "whole step groups");
+  const __hip_bfloat16* wrow[NT];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const int oc = (ob * NT + j) * 32 + lane % 32;
-      conv_bf16x4 b = {0, 0, 0, 0};
-      if (k0 + kh < kConvK) b = *reinterpret_cast<const conv_bf16x4*>(W + (int64_t)oc * kConvK + k0 + kh);
-      acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[j], 0, 0, 0);
+  for (int j = 0; j < NT; ++j) wrow[j] = W + (int64_t)((ob * NT + j) * 32 + lane % 32) * kConvK + kh;
+  auto wload = [&](int grp, conv_bf16x4 (&b)[KG][NT]) {
+#pragma unroll
+    for (int q = 0; q < KG; ++q) {
+      const int k0 = (grp * KG + q) * 8;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        b[q][j] = conv_bf16x4{0, 0, 0, 0};
+        if (k0 + kh < kConvK) b[q][j] = *reinterpret_cast<const conv_bf16x4*>(wrow[j] + k0);
+      }
+    }
+  };
+  conv_bf16x4 wb[2][KG][NT];
+  wload(0, wb[0]);
+#pragma unroll
+  for (int grp = 0; grp < NG; ++grp) {
+    if (grp + 1 < NG) wload(grp + 1, wb[(grp + 1) & 1]);
+#pragma unroll
+    for (int q = 0; q < KG; ++q) {
+      const int k0 = (grp * KG + q) * 8;
+      const conv_bf16x4 a = *reinterpret_cast<const conv_bf16x4*>(arow + k0 + kh);
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wb[grp & 1][q][j], acc[j], 0, 0, 0);
     }
   }
   // D value v of lane l: pixel row 8 (v / 4) + 4 (l / 32) + v % 4 of the half, column l % 32
