@@ -636,7 +636,8 @@ int launch_bwd(const float* coords, const void* grad, void* vgrad, int64_t batch
 // window loads of all four levels are issued together (one memory latency per block), then per
 // level the taps go to LDS and the bilinear combine rounds each channel to bf16 (as the
 // lookup's own output) into an LDS row per pixel; then the 64 x C_out x 196 product on
-// v_mfma_f32_32x32x8bf16_1k (FP32 accumulation, W's fragments read from L2), bias, ReLU and
+// v_mfma_f32_32x32x8bf16_1k (FP32 accumulation; each wave one 32-row slice of W, its fragments
+// read from L2 a step group ahead), bias, ReLU and
 // bf16 stores of 4 consecutive pixels.  bf16 pyramids of 4 levels, radius 3.
 typedef short conv_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float conv_acc16 __attribute__((ext_vector_type(16)));
@@ -713,37 +714,34 @@ corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t
     }
     __syncthreads();                              // taps are reused by the next level
   }
-  // 64 pixels x COUT outputs in 32 x 32 tiles: wave w takes pixel half w & 1 and the output
-  // tiles of its other index bit
-  constexpr int NT = COUT / 32 / 2;               // output tiles per wave
+  // 64 pixels x COUT outputs in 32 x 32 tiles: wave w takes output tile w % (COUT / 32) and
+  // PH pixel halves (both for COUT = 128, so each W fragment is fetched once per block)
+  constexpr int NOT = COUT / 32, NP = kThreads / kWave / NOT;
+  constexpr int PH = 2 / NP;
+  static_assert(NOT * NP == kThreads / kWave && PH * NP == 2, "tile split");
   const int lane = t % kWave, wave = t / kWave;
-  const int ph = wave & 1, ob = wave >> 1;
-  const int row = ph * 32 + lane % 32, kh = 4 * (lane / 32);
-  conv_acc16 acc[NT];
+  const int otile = wave % NOT, ph0 = (wave / NOT) * PH;
+  const int kh = 4 * (lane / 32);
+  conv_acc16 acc[PH];
 #pragma unroll
-  for (int j = 0; j < NT; ++j)
+  for (int j = 0; j < PH; ++j)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[j][v] = 0.f;
-  const unsigned short* arow = &A[row][0];
   // W's fragments come from L2: steps in groups of KG, the next group's loads issued before this
   // group's MFMAs
   constexpr int KS = kConvKP / 8, KG = 5, NG = KS / KG;
   static_assert(KS % KG == 0, "whole step groups");
-  const __hip_bfloat16* wrow[NT];
-#pragma unroll
-  for (int j = 0; j < NT; ++j) wrow[j] = W + (int64_t)((ob * NT + j) * 32 + lane % 32) * kConvK + kh;
-  auto wload = [&](int grp, conv_bf16x4 (&b)[KG][NT]) {
+  const int oc_l = otile * 32 + lane % 32;
+  const __hip_bfloat16* wrow = W + (int64_t)oc_l * kConvK + kh;
+  auto wload = [&](int grp, conv_bf16x4 (&b)[KG]) {
 #pragma unroll
     for (int q = 0; q < KG; ++q) {
       const int k0 = (grp * KG + q) * 8;
-#pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        b[q][j] = conv_bf16x4{0, 0, 0, 0};
-        if (k0 + kh < kConvK) b[q][j] = *reinterpret_cast<const conv_bf16x4*>(wrow[j] + k0);
-      }
+      b[q] = conv_bf16x4{0, 0, 0, 0};
+      if (k0 + kh < kConvK) b[q] = *reinterpret_cast<const conv_bf16x4*>(wrow + k0);
     }
   };
-  conv_bf16x4 wb[2][KG][NT];
+  conv_bf16x4 wb[2][KG];
   wload(0, wb[0]);
 #pragma unroll
   for (int grp = 0; grp < NG; ++grp) {
@@ -751,21 +749,21 @@ corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t
 #pragma unroll
     for (int q = 0; q < KG; ++q) {
       const int k0 = (grp * KG + q) * 8;
-      const conv_bf16x4 a = *reinterpret_cast<const conv_bf16x4*>(arow + k0 + kh);
 #pragma unroll
-      for (int j = 0; j < NT; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wb[grp & 1][q][j], acc[j], 0, 0, 0);
+      for (int j = 0; j < PH; ++j) {
+        const conv_bf16x4 a = *reinterpret_cast<const conv_bf16x4*>(&A[(ph0 + j) * 32 + lane % 32][k0 + kh]);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wb[grp & 1][q], acc[j], 0, 0, 0);
+      }
     }
   }
   // D value v of lane l: pixel row 8 (v / 4) + 4 (l / 32) + v % 4 of the half, column l % 32
+  const float bb = bias[oc_l];
+  T* obase = out + ((int64_t)n * COUT + oc_l) * hw1 + pix0;
 #pragma unroll
-  for (int j = 0; j < NT; ++j) {
-    const int oc = (ob * NT + j) * 32 + lane % 32;
-    const float bb = bias[oc];
-    T* obase = out + ((int64_t)n * COUT + oc) * hw1 + pix0;
+  for (int j = 0; j < PH; ++j) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int px = ph * 32 + 8 * q + 4 * (lane / 32);
+      const int px = (ph0 + j) * 32 + 8 * q + 4 * (lane / 32);
       uint32_t h[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
