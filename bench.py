@@ -53,8 +53,7 @@ def log(*a):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f'bench.py measures one GPU only; --gpus {args.gpus} is not supported '
-              '(the sharding design is described in DESIGN.md).', file=sys.stderr)
+        print(f'bench.py measures one GPU only; --gpus {args.gpus} is not supported.', file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
