@@ -478,10 +478,12 @@ corr_lookup_vec_kernel(LevelTable lv, const float* __restrict__ coords, int64_t 
       }
       T* dst = obase + (int64_t)o * hw1;
       if (4 * q + 3 < npix) {
-        uint2 pk;
-        pk.x = f_to_half_bits<T>(v[0]) | (f_to_half_bits<T>(v[1]) << 16);
-        pk.y = f_to_half_bits<T>(v[2]) | (f_to_half_bits<T>(v[3]) << 16);
-        *reinterpret_cast<uint2*>(dst) = pk;
+        // streamed past the caches: the output is not read again here, and the window lines
+        // that neighbouring pixels share stay cached
+        const unsigned long long pk =
+            static_cast<unsigned long long>(f_to_half_bits<T>(v[0]) | (f_to_half_bits<T>(v[1]) << 16)) |
+            (static_cast<unsigned long long>(f_to_half_bits<T>(v[2]) | (f_to_half_bits<T>(v[3]) << 16)) << 32);
+        __builtin_nontemporal_store(pk, reinterpret_cast<unsigned long long*>(dst));
       } else {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
