@@ -118,19 +118,25 @@ def untile(level, i):
     return a
 
 
-@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
-def test_corr_build_tiled_layout(db, dtype):
+@pytest.mark.parametrize('dtype,shape', [(torch.float32, (48, 64)), (torch.bfloat16, (48, 64)),
+                                         (torch.bfloat16, (48, 48)), (torch.bfloat16, (32, 32))])
+def test_corr_build_tiled_layout(db, dtype, shape):
+    """48x64: every level in its preferred layout; 48x48: level 2 (12x12) in 2x4 segments with a
+    width that is not a multiple of 8 (the general gather) and level 3 (6x6) row-major; 32x32:
+    level 2 (8x8) in 2x4 segments, level 3 (4x4) row-major."""
     from hislam2_amd import corr as hc
     rng = np.random.default_rng(31)
-    f1 = rounded(rng.standard_normal((1, 2, 64, 48, 64)) / 8, dtype).cuda()
-    f2 = rounded(rng.standard_normal((1, 2, 64, 48, 64)) / 8, dtype).cuda()
+    h, w = shape
+    f1 = rounded(rng.standard_normal((1, 2, 64, h, w)) / 8, dtype).cuda()
+    f2 = rounded(rng.standard_normal((1, 2, 64, h, w)) / 8, dtype).cuda()
     rm = db.corr_build(f1, f2, 4, dtype, False)
     tl = db.corr_build(f1, f2, 4, dtype, True)
     for i in range(4):
         a, b = rm[i].float().cpu().numpy(), tl[i].float().cpu().numpy()
         assert np.array_equal(a, untile(b, i)), i
         assert torch.equal(hc.pyramid_planes(tl[i], i), rm[i]), i
-    coords = torch.from_numpy(np.random.default_rng(32).uniform(-5, 70, (1, 2, 48, 64, 2)).astype(np.float32)).cuda()
+    coords = torch.from_numpy(np.random.default_rng(32).uniform(-5, max(h, w) + 6, (1, 2, h, w, 2))
+                              .astype(np.float32)).cuda()
     o_rm = db.corr_pyramid_lookup(rm, coords, 3, False)
     o_tl = db.corr_pyramid_lookup(tl, coords, 3, True)
     assert torch.equal(o_rm, o_tl)
