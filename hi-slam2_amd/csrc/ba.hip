@@ -75,7 +75,8 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
     }
     if (schur) {
       launch_frame_reduce(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
-                          blob + p->o_frame_pose, HW, static_cast<int>(F), 0.f, Q, wv, Eall, s);
+                          blob + p->o_frame_pose, blob + p->o_frame_order, HW, static_cast<int>(F), 0.f, Q, wv,
+                          Eall, s);
       HSL2_LAUNCH_CHECK();
       launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, s);
       HSL2_LAUNCH_CHECK();

@@ -233,9 +233,9 @@ __global__ void __launch_bounds__(256)
 ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
                 const float* __restrict__ Eii, const float* __restrict__ eta, int eta_rows,
                 const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
-                const int* __restrict__ frame_pose, int HW, float c_eps, float* __restrict__ Q,
-                float* __restrict__ w, float* __restrict__ Eall) {
-  const int f = blockIdx.y;
+                const int* __restrict__ frame_pose, const int* __restrict__ frame_order, int HW, float c_eps,
+                float* __restrict__ Q, float* __restrict__ w, float* __restrict__ Eall) {
+  const int f = frame_order[blockIdx.y];          // the frames with most edges first
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= HW) return;
   const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
@@ -292,10 +292,10 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
 
 inline void launch_frame_reduce(const float* Cii, const float* bz, const float* Eii, const float* eta,
                                 int eta_rows, const int* frame_eptr, const int* frame_edges,
-                                const int* frame_pose, int HW, int F, float c_eps, float* Q, float* w,
-                                float* Eall, hipStream_t s) {
+                                const int* frame_pose, const int* frame_order, int HW, int F, float c_eps,
+                                float* Q, float* w, float* Eall, hipStream_t s) {
   hipLaunchKernelGGL(ba_frame_reduce, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii, eta, eta_rows,
-                     frame_eptr, frame_edges, frame_pose, HW, c_eps, Q, w, Eall);
+                     frame_eptr, frame_edges, frame_pose, frame_order, HW, c_eps, Q, w, Eall);
 }
 
 // ---- 3. Ev rows, one wave per slot -------------------------------------------

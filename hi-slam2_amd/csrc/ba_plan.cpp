@@ -235,6 +235,15 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
   p->o_frame_pose = put(frame_pose);
   p->o_frame_eptr = put(frame_eptr);
   p->o_frame_edges = put(frame_edges);
+  {
+    // per-frame work grows with the frame's edge count: the heaviest frames are dispatched first
+    std::vector<int32_t> order(F);
+    for (int64_t f = 0; f < F; ++f) order[f] = static_cast<int32_t>(f);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return frame_eptr[a + 1] - frame_eptr[a] > frame_eptr[b + 1] - frame_eptr[b];
+    });
+    p->o_frame_order = put(order);
+  }
   p->o_slot_ptr = put(slot_ptr);
   p->o_slot_row = put(slot_row);
   p->o_slot_pose = put(slot_pose);

@@ -32,6 +32,7 @@ struct hsl2_ba_plan {
   int64_t o_edge_frame = 0;                    // [E] depth-frame index of ii[e]
   int64_t o_frame_kx = 0, o_frame_pose = 0;    // [F]
   int64_t o_frame_eptr = 0, o_frame_edges = 0; // [F+1], [E]
+  int64_t o_frame_order = 0;                   // [F] frames by descending edge count (stable)
   int64_t o_slot_ptr = 0;                      // [F+1]
   int64_t o_slot_row = 0, o_slot_pose = 0;     // [NS]
   int64_t o_task = 0;                          // gram tasks [NT][4] = (kind, a, b, frame)

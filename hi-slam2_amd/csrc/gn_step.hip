@@ -543,8 +543,8 @@ extern "C" int hsl2_gn_step(const hsl2_ba_plan* p, float* poses, float* disps,
   }
   if (schur) {
     launch_frame_reduce(v.Cii, v.bz, v.Eii, eta, static_cast<int>(F), v.blob + p->o_frame_eptr,
-                        v.blob + p->o_frame_edges, v.blob + p->o_frame_pose, HW, static_cast<int>(F), 1e-7f,
-                        v.Q, v.w, v.Eall, s);
+                        v.blob + p->o_frame_edges, v.blob + p->o_frame_pose, v.blob + p->o_frame_order, HW,
+                        static_cast<int>(F), 1e-7f, v.Q, v.w, v.Eall, s);
     HSL2_LAUNCH_CHECK();
     launch_schur_gram(p, v.blob, v.Eall, v.Q, v.w, v.S, v.Ev, s);
     HSL2_LAUNCH_CHECK();
