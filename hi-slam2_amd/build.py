@@ -22,7 +22,7 @@ ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 ARCH = 'gfx950'
 
 LIB_SOURCES = ['status.cpp', 'ba_plan.cpp', 'ba.hip', 'gn_step.hip', 'projective.hip', 'chol.hip', 'corr_lookup.hip',
-               'corr_build.hip', 'altcorr.hip', 'geometry.hip']
+               'corr_build.hip', 'altcorr.hip', 'geometry.hip', 'pgba.hip']
 
 HIP_FLAGS = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}', '-mcode-object-version=5',

@@ -902,10 +902,65 @@ class GNStep {
   torch::Tensor kx_, ws_;
 };
 
+// ---- pgba (src/droid.cpp:171-195) ----------------------------------------------
+// Sim3 pose-graph BA step: [dx [t1 - t0, 7], dz [F, ht * wd]], disps updated in place.
+std::vector<torch::Tensor> pgba(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                torch::Tensor targets, torch::Tensor weights, torch::Tensor eta, torch::Tensor Hsp,
+                                torch::Tensor vsp, torch::Tensor ii, torch::Tensor jj, torch::Tensor iip,
+                                torch::Tensor jjp, int t0, int t1, double lm, double ep) {
+  for (auto* t : {&poses, &disps, &intrinsics, &targets, &weights, &eta, &Hsp, &vsp}) {
+    CHECK_CONTIGUOUS(*t);
+    require_device(*t, "pgba tensors");
+    require(t->scalar_type() == torch::kFloat32, "pgba: poses, disps, intrinsics, targets, weights, eta, Hsp, vsp "
+                                                 "must be float32");
+  }
+  for (auto* t : {&ii, &jj, &iip, &jjp}) {
+    CHECK_CONTIGUOUS(*t);
+    require(t->dim() == 1 && t->scalar_type() == torch::kInt64, "pgba: ii, jj, iip, jjp must be int64 [*]");
+  }
+  require(poses.dim() == 2 && poses.size(1) == 8, "pgba: poses must be [N, 8] (t, q, s)");
+  require(disps.dim() == 3 && disps.size(0) >= poses.size(0), "pgba: disps must be [N, ht, wd]");
+  require(intrinsics.numel() >= 4, "pgba: intrinsics must hold fx, fy, cx, cy");
+  require(ii.size(0) == jj.size(0) && iip.size(0) == jjp.size(0), "pgba: ii / jj and iip / jjp differ in length");
+  const int64_t E = ii.size(0), Np = iip.size(0), ht = disps.size(1), wd = disps.size(2);
+  require(targets.numel() == E * 2 * ht * wd && weights.numel() == E * 2 * ht * wd,
+          "pgba: targets and weights must be [E, 2, ht, wd]");
+  require(Hsp.numel() == Np * 4 * 49 && vsp.numel() == Np * 2 * 7, "pgba: Hsp must be [4, Np, 7, 7], vsp [2, Np, 7]");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
+  torch::Tensor ii_h = ii.to(torch::kCPU), jj_h = jj.to(torch::kCPU);
+  torch::Tensor iip_h = iip.to(torch::kCPU), jjp_h = jjp.to(torch::kCPU);
+  hsl2_pgba_plan* raw = nullptr;
+  raise_status(hsl2_pgba_plan_create(ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E, iip_h.data_ptr<int64_t>(),
+                                     jjp_h.data_ptr<int64_t>(), Np, poses.size(0), ht, wd, t0, t1, &raw),
+               "pgba");
+  std::unique_ptr<hsl2_pgba_plan, void (*)(hsl2_pgba_plan*)> plan(raw, hsl2_pgba_plan_destroy);
+  const int64_t F = hsl2_pgba_plan_num_frames(plan.get());
+  require(eta.numel() % (ht * wd) == 0, "pgba: eta must hold whole [ht, wd] rows");
+  const int64_t eta_rows = eta.numel() / (ht * wd);
+  require(eta_rows == F || eta_rows == 1, "pgba: eta must have one [ht, wd] row per unique frame of [t0, t1) ++ ii (" +
+                                              std::to_string(F) + ") or a single row");
+  auto opts = poses.options();
+  torch::Tensor ws = torch::empty({static_cast<int64_t>(hsl2_pgba_workspace_bytes(plan.get())) + 256},
+                                  opts.dtype(torch::kUInt8));
+  uintptr_t base = reinterpret_cast<uintptr_t>(ws.data_ptr());
+  void* ws_ptr = reinterpret_cast<void*>((base + 255) & ~uintptr_t(255));
+  torch::Tensor dx = torch::empty({t1 - t0, 7}, opts);
+  torch::Tensor dz = torch::empty({F, ht * wd}, opts);
+  raise_status(hsl2_pgba_run(plan.get(), poses.data_ptr<float>(), disps.data_ptr<float>(), intrinsics.data_ptr<float>(),
+                             E ? targets.data_ptr<float>() : nullptr, E ? weights.data_ptr<float>() : nullptr,
+                             eta.data_ptr<float>(), eta_rows, Np ? Hsp.data_ptr<float>() : nullptr,
+                             Np ? vsp.data_ptr<float>() : nullptr, static_cast<float>(lm), static_cast<float>(ep),
+                             dx.data_ptr<float>(), dz.data_ptr<float>(), ws_ptr,
+                             static_cast<size_t>(ws.numel() - 256), nullptr, current_stream()),
+               "pgba");
+  return {dx, dz};
+}
+
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "MI355X (gfx950) dense-tracking kernels behind the droid_backends interface";
   // bundle adjustment (src/droid.cpp:327)
   m.def("ba", &ba, "dense bundle adjustment (split precision: FP32 linearization, FP64 reduced solve)");
+  m.def("pgba", &pgba, "Sim3 pose-graph bundle adjustment step: [dx, dz], disps updated in place");
   // correlation volume kernels (src/droid.cpp:337-340)
   m.def("altcorr_forward", &altcorr_forward, "on-the-fly correlation lookup");
   m.def("altcorr_backward", &altcorr_backward, "on-the-fly correlation lookup adjoint");

@@ -275,3 +275,60 @@ def mono_problem(seed=0, h=48, w=64, hs=2, ws=2, drop_frame=None, alpha=0.01):
                 weights=f32(weights), eta=f32(eta), ii=np.ascontiguousarray(ii, np.int64),
                 jj=np.ascontiguousarray(jj, np.int64), disps_prior=f32(prior), dscales=f32(dscales),
                 alpha=alpha)
+
+
+def _reproject_sim3(poses, disps, intr, ii, jj):
+    """Exact reprojection pi(T_j T_i^-1 X(u, v, d_i)) for Sim3 poses [N, 8] -> [E, 2, h, w]."""
+    fx, fy, cx, cy = intr
+    _, h, w = disps.shape
+    v, u = np.meshgrid(np.arange(h, dtype=np.float64), np.arange(w, dtype=np.float64), indexing='ij')
+    X0 = np.stack([(u - cx) / fx, (v - cy) / fy, np.ones_like(u)])
+    pi, pj = poses[ii], poses[jj]
+    qij = _quat_mul(pj[:, 3:7], np.concatenate([-pi[:, 3:6], pi[:, 6:7]], -1))
+    sij = pj[:, 7] / pi[:, 7]
+    R = _rotmat(qij) * sij[:, None, None]
+    tij = pj[:, :3] - np.einsum('eij,ej->ei', R, pi[:, :3])
+    X = np.einsum('eij,jhw->eihw', R, X0) + tij[:, :, None, None] * disps[ii][:, None]
+    z = np.where(np.abs(X[:, 2]) < 1e-6, 1e-6, X[:, 2])
+    return np.stack([fx * X[:, 0] / z + cx, fy * X[:, 1] / z + cy], axis=1)
+
+
+def pgba_problem(seed=0, n=12, h=48, w=64, radius=3, n_long=10, t0=1, n_pg=6, pw=1e-3):
+    """Inputs of one pose-graph BA step (droid_backends.pgba): Sim3 poses [n, 8] along a random
+    walk with scales in [0.8, 1.25], neighbour and long edges, targets reprojected through the
+    true poses plus 0.5 px noise, the poses of [t0, n) perturbed in pose and scale, and n_pg
+    pose-graph pairs with blocks J^T Lambda J (random 7x7 Jacobians, diagonal information) and
+    gradients -J^T Lambda r, scaled by pw as in the reference caller."""
+    rng = np.random.default_rng(seed)
+    se3_true = _random_walk(rng, n)
+    s_true = rng.uniform(0.8, 1.25, size=(n, 1))
+    s_true[0] = 1.0
+    poses_true = np.concatenate([se3_true, s_true], -1)
+    far = sum(1 for i in range(n) for j in range(n) if abs(i - j) > radius)
+    ii, jj = _window_edges(rng, n, radius, min(n_long, far))
+    disps_true = rng.uniform(0.2, 1.0, size=(n, h, w))
+    intr = np.asarray(INTRINSICS, np.float64)
+    tgt = _reproject_sim3(poses_true, disps_true, intr, ii, jj)
+    tgt += rng.normal(scale=0.5, size=tgt.shape)
+    wts = rng.uniform(0.0, 1.0, size=tgt.shape)
+    poses = poses_true.copy()
+    poses[t0:, :7] = _compose(_exp(rng.normal(scale=0.01, size=(n - t0, 6))), poses[t0:, :7])
+    poses[t0:, 7] *= np.exp(rng.normal(scale=0.01, size=n - t0))
+    disps = np.clip(disps_true * (1.0 + 0.05 * rng.normal(size=disps_true.shape)), 0.05, None)
+    frames = np.unique(np.concatenate([np.arange(t0, n), ii]))
+    eta = np.full((len(frames), h, w), 1e-3)
+    iip = rng.integers(0, n, size=n_pg)
+    jjp = (iip + rng.integers(2, n, size=n_pg)) % n
+    Ji = rng.normal(size=(n_pg, 7, 7))
+    Jj = rng.normal(size=(n_pg, 7, 7))
+    lam = rng.uniform(0.5, 2.0, size=(n_pg, 7))
+    r = 0.01 * rng.normal(size=(n_pg, 7))
+    wJiT = pw * np.transpose(Ji, (0, 2, 1)) * lam[:, None, :]
+    wJjT = pw * np.transpose(Jj, (0, 2, 1)) * lam[:, None, :]
+    Hsp = np.stack([wJiT @ Ji, wJiT @ Jj, wJjT @ Ji, wJjT @ Jj])
+    vsp = -np.stack([np.einsum('nij,nj->ni', wJiT, r), np.einsum('nij,nj->ni', wJjT, r)])
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+    return dict(poses=f32(poses), disps=f32(disps), intrinsics=f32(intr), targets=f32(tgt), weights=f32(wts),
+                eta=f32(eta), Hsp=f32(Hsp), vsp=f32(vsp), ii=i64(ii), jj=i64(jj), iip=i64(iip), jjp=i64(jjp),
+                t0=t0, t1=n, lm=1e-5, ep=1e-3)

@@ -342,6 +342,38 @@ size_t hsl2_spd_solve_workspace_bytes(int dtype, int64_t n);
 int hsl2_spd_solve(int dtype, const void* A, const void* b, void* x, int64_t n, void* workspace,
                    size_t workspace_bytes, int* ok, void* stream);
 
+/* ---- pgba: Sim3 pose-graph bundle adjustment ----------------------------------
+ * Replaces droid_backends.pgba (src/droid.cpp:171-195 -> pgba_cuda,
+ * src/droid_kernels.cu:1940-2044, pixel terms projective_transform_sim3_kernel 608-858).
+ * One Gauss-Newton step over the Sim3 poses of [t0, t1) (7 parameters: translation, rotation,
+ * scale) and the disparities of the sorted unique frames kx of [t0, t1) ++ ii, with the
+ * caller's pose-graph blocks added to the pose system.  The plan (host) holds the index
+ * structures: ii, jj [E] and iip, jjp [Np] are host int64 arrays with every index in [0, t1)
+ * (HSL2_ERR_INDEX otherwise; the reference leaves such indices undefined), 0 <= t0 < t1 <=
+ * num_poses. */
+typedef struct hsl2_pgba_plan hsl2_pgba_plan;
+int hsl2_pgba_plan_create(const int64_t* ii, const int64_t* jj, int64_t num_edges, const int64_t* iip,
+                          const int64_t* jjp, int64_t num_pose_graph, int64_t num_poses, int64_t ht, int64_t wd,
+                          int t0, int t1, hsl2_pgba_plan** plan);
+void hsl2_pgba_plan_destroy(hsl2_pgba_plan* plan);
+/* |kx| and kx itself (host int64 [F]) */
+int64_t hsl2_pgba_plan_num_frames(const hsl2_pgba_plan* plan);
+int hsl2_pgba_plan_frames(const hsl2_pgba_plan* plan, int64_t* kx);
+size_t hsl2_pgba_workspace_bytes(const hsl2_pgba_plan* plan);
+/* Device pointers: poses [N, 8] (t, q xyzw, s), disps [N, ht, wd] (updated in place: disps[kx]
+ * += dz, no clamp), intrinsics [4], targets / weights [E, 2, ht, wd], eta [F, ht, wd] (or one
+ * [ht, wd] row), Hsp [4, Np, 7, 7] (blocks (iip, iip), (iip, jjp), (jjp, iip), (jjp, jjp)), vsp
+ * [2, Np, 7]; outputs dx [t1 - t0, 7] and dz [F, ht * wd].  The lower triangle of each 7x7
+ * block enters the FP64 factorisation of the damped reduced system (diag += ep + lm * diag);
+ * when it fails dx = 0, dz is still applied and *solve_ok = 0.  The back-substitution leaves
+ * out pose slot 0, as the reference's EvT6x1_kernel does (src/droid_kernels.cu:1572).  The call
+ * synchronizes the stream before it returns.  Workspace: hsl2_pgba_workspace_bytes, 256-byte
+ * aligned. */
+int hsl2_pgba_run(const hsl2_pgba_plan* plan, const float* poses, float* disps, const float* intrinsics,
+                  const float* targets, const float* weights, const float* eta, int64_t eta_rows, const float* Hsp,
+                  const float* vsp, float lm, float ep, float* dx, float* dz, void* workspace,
+                  size_t workspace_bytes, int* solve_ok, void* stream);
+
 /* Debug/test view of the last iteration's intermediates inside a workspace after
  * hsl2_ba_run: per-edge Hessian blocks Hs [4, E, 6, 6] and gradients vs [2, E, 6], the
  * buffers of src/droid_kernels.cu:1856-1857 but accumulated in f64, and the damped reduced

@@ -180,22 +180,23 @@ def reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only=False):
     C, w, Q, Ei from accum_cuda (1901-1906).  Returns a dict with the dense
     matrix, the rhs and the depth-side terms needed for back-substitution.
     """
+    D = lin['Hs'].shape[-1]                                   # 6 (SE3) or 7 (Sim3)
     P = t1 - t0
     ii = np.asarray(ii, np.int64)
     jj = np.asarray(jj, np.int64)
     Hs, vs = lin['Hs'], lin['vs']
-    A = np.zeros((6 * P, 6 * P))
-    b = np.zeros(6 * P)
+    A = np.zeros((D * P, D * P))
+    b = np.zeros(D * P)
     rows = np.concatenate([ii, ii, jj, jj]) - t0
     cols = np.concatenate([ii, jj, ii, jj]) - t0
-    blocks = Hs.reshape(-1, 6, 6)
+    blocks = Hs.reshape(-1, D, D)
     for n in range(len(rows)):
         i, j = rows[n], cols[n]
         if i >= 0 and j >= 0:
-            A[6 * i:6 * i + 6, 6 * j:6 * j + 6] += blocks[n]
+            A[D * i:D * i + D, D * j:D * j + D] += blocks[n]
     for n, i in enumerate(np.concatenate([ii, jj]) - t0):
         if i >= 0:
-            b[6 * i:6 * i + 6] += vs.reshape(-1, 6)[n]
+            b[D * i:D * i + D] += vs.reshape(-1, D)[n]
     out = dict(A=A, b=b)
     if motion_only:
         out.update(L=A.copy(), rhs=b.copy())
@@ -209,12 +210,12 @@ def reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only=False):
     np.add.at(w, kk_exp[P:], lin['bz'])
     eta = np.asarray(eta, np.float64).reshape(-1, HW)
     Q = 1.0 / (C + eta)
-    Ei = np.zeros((P, 6, HW))
+    Ei = np.zeros((P, D, HW))
     for e in range(E):
         s = ii[e] - t0
         if 0 <= s < P:
             Ei[s] += lin['Eii'][e]
-    Eall = np.concatenate([Ei, lin['Eij']], axis=0)            # [E', 6, HW]
+    Eall = np.concatenate([Ei, lin['Eij']], axis=0)            # [E', D, HW]
     S = np.zeros_like(A)
     slot = jj_exp - t0
     inwin = (slot >= 0) & (slot < P)
@@ -227,12 +228,12 @@ def reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only=False):
         for x, na in enumerate(ns):
             for y, nb in enumerate(ns):
                 a, c = slot[na], slot[nb]
-                S[6 * a:6 * a + 6, 6 * c:6 * c + 6] += G[x, :, y, :]
+                S[D * a:D * a + D, D * c:D * c + D] += G[x, :, y, :]
     vS = np.einsum('nih,nh->ni', Eall, (Q * w)[kk_exp])
-    bS = np.zeros(6 * P)
+    bS = np.zeros(D * P)
     for n in range(len(slot)):
         if slot[n] >= 0:
-            bS[6 * slot[n]:6 * slot[n] + 6] += vS[n]
+            bS[D * slot[n]:D * slot[n] + D] += vS[n]
     out.update(S=S, bS=bS, L=A - S, rhs=b - bS, C=C, w=w, Q=Q, Eall=Eall,
                kx=kx, kk_exp=kk_exp, ii_exp=ii_exp, jj_exp=jj_exp)
     return out
@@ -263,12 +264,13 @@ def back_substitute(sys, dx, t0, t1, t0_skip=True):
     """dz = Q (w - sum_n E_n^T dx[jj_n - t0]) (src/droid_kernels.cu:1911-1920, 1562-1582).
 
     With ``t0_skip`` (the reference behaviour) slots jj - t0 <= 0 contribute 0."""
+    D = sys['Eall'].shape[1]
     P = t1 - t0
     slot = sys['jj_exp'] - t0
     lo = 1 if t0_skip else 0
     use = (slot >= lo) & (slot < P)
     dw = np.zeros((len(slot), sys['Q'].shape[1]))
-    dxr = dx.reshape(P, 6)
+    dxr = dx.reshape(P, D)
     dw[use] = np.einsum('nih,ni->nh', sys['Eall'][use], dxr[slot[use]])
     acc = np.zeros_like(sys['w'])
     np.add.at(acc, sys['kk_exp'], dw)
@@ -337,7 +339,8 @@ def full_system(lin, eta, ii, jj, t0, t1, HW):
     ii = np.asarray(ii, np.int64)
     sys = reduced_system(lin, eta, ii, jj, t0, t1, HW)
     F = len(sys['kx'])
-    n_p = 6 * P
+    D = lin['Hs'].shape[-1]
+    n_p = D * P
     n = n_p + F * HW
     H = np.zeros((n, n))
     g = np.zeros(n)
@@ -349,8 +352,8 @@ def full_system(lin, eta, ii, jj, t0, t1, HW):
         if 0 <= a < P:
             f = sys['kk_exp'][nn]
             cols = n_p + f * HW + np.arange(HW)
-            H[6 * a:6 * a + 6, cols] += sys['Eall'][nn]
-            H[cols, 6 * a:6 * a + 6] += sys['Eall'][nn].T
+            H[D * a:D * a + D, cols] += sys['Eall'][nn]
+            H[cols, D * a:D * a + D] += sys['Eall'][nn].T
     eta = np.asarray(eta, np.float64).reshape(-1, HW)
     Cd = (sys['C'] + eta).reshape(-1) if eta.shape[0] == F else (sys['C'] + eta).reshape(-1)
     H[np.arange(n_p, n), np.arange(n_p, n)] += Cd

@@ -1,0 +1,231 @@
+"""Pins the Sim3 pose-graph BA oracle: Jacobians against central differences of the projection
+under left Sim3 perturbations, the Schur solve against the joint pose + depth solve at block
+size 7, the pose-graph block placement against the stationarity of the quadratic it encodes,
+and the argument contract."""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from hislam2_amd import synth
+from oracle import ba as oba
+from oracle import pgba as opg
+from oracle import se3
+
+INTR = synth.INTRINSICS
+
+
+def sim3_mat(pose):
+    """4x4 matrix [[s R, t], [0, 1]] of a Sim3 pose (t, q, s)."""
+    T = se3.pose_to_mat(pose[:7])
+    T[:3, :3] *= pose[7]
+    return T
+
+
+def hat7(xi):
+    """4x4 generator of [tau, phi, sigma]: [[phi^ + sigma I, tau], [0, 0]]."""
+    M = se3.hat(xi[:6])
+    M[:3, :3] += xi[6] * np.eye(3)
+    return M
+
+
+def project(Ti, Tj, d, u, v):
+    fx, fy, cx, cy = INTR
+    X = np.array([(u - cx) / fx, (v - cy) / fy, 1.0, d])
+    Y = Tj @ np.linalg.inv(Ti) @ X
+    return np.array([fx * Y[0] / Y[2] + cx, fy * Y[1] / Y[2] + cy])
+
+
+def fd_columns(Ti, Tj, d, u, v, which, eps=1e-6):
+    cols = []
+    for k in range(7):
+        dl = np.zeros(7)
+        dl[k] = eps
+        if which == 'i':
+            fp = project(expm(hat7(dl)) @ Ti, Tj, d, u, v)
+            fm = project(expm(hat7(-dl)) @ Ti, Tj, d, u, v)
+        else:
+            fp = project(Ti, expm(hat7(dl)) @ Tj, d, u, v)
+            fm = project(Ti, expm(hat7(-dl)) @ Tj, d, u, v)
+        cols.append((fp - fm) / (2 * eps))
+    return np.stack(cols, -1)                                     # [2, 7]
+
+
+def terms(pi, pj, disp, h, w):
+    tg = np.zeros((1, 2, h * w))
+    wt = np.ones((1, 2, h * w))
+    return opg.edge_terms(pi[None], pj[None], disp.reshape(1, -1), INTR, tg, wt, np.array([False]), h, w)
+
+
+def random_sim3(rng, scale):
+    p = np.zeros(8)
+    p[:7] = synth._exp(rng.normal(scale=0.05, size=6))
+    p[7] = scale
+    return p
+
+
+def test_jj_jz_and_translation_of_ji_match_central_differences():
+    rng = np.random.default_rng(0)
+    h, w = 4, 5
+    u, v = oba.pixel_grid(h, w)
+    for trial in range(3):
+        pi, pj = random_sim3(rng, rng.uniform(0.7, 1.3)), random_sim3(rng, rng.uniform(0.7, 1.3))
+        disp = rng.uniform(0.3, 1.0, size=(h, w))
+        t = terms(pi, pj, disp, h, w)
+        Ti, Tj = sim3_mat(pi), sim3_mat(pj)
+        for p in (0, 7, 19):
+            assert t['valid'][0, p]
+            d = disp.reshape(-1)[p]
+            Jj = fd_columns(Ti, Tj, d, u[p], v[p], 'j')
+            Ji = fd_columns(Ti, Tj, d, u[p], v[p], 'i')
+            assert np.allclose(t['Jj'][0, :, p], Jj, atol=1e-5, rtol=1e-6)
+            assert np.allclose(t['Ji'][0, :, p, :3], Ji[:, :3], atol=1e-5, rtol=1e-6)
+            eps = 1e-6
+            Jz = (project(Ti, Tj, d + eps, u[p], v[p]) - project(Ti, Tj, d - eps, u[p], v[p])) / (2 * eps)
+            assert np.allclose(t['Jz'][0, :, p], Jz, atol=1e-5, rtol=1e-6)
+            assert np.allclose(t['coords'][0, :, p], project(Ti, Tj, d, u[p], v[p]), atol=1e-10)
+
+
+def test_scale_column_of_jj_vanishes():
+    """Scaling the camera-j coordinates about the camera centre leaves the projection unchanged,
+    so Jp . Xj = 0 for every valid pixel (up to float64 rounding)."""
+    rng = np.random.default_rng(9)
+    h, w = 6, 8
+    pi, pj = random_sim3(rng, 0.9), random_sim3(rng, 1.2)
+    t = terms(pi, pj, rng.uniform(0.3, 1.0, size=(h, w)), h, w)
+    assert np.all(t['valid'])
+    assert np.abs(t['Jj'][..., 6]).max() < 1e-12 * np.abs(t['Jj']).max()
+
+
+def test_full_ji_matches_central_differences_where_the_reference_adjoint_is_exact():
+    """sij == 1 and tij == 0: the reference's adjSim3 is the transposed Sim3 adjoint there."""
+    rng = np.random.default_rng(1)
+    h, w = 4, 5
+    u, v = oba.pixel_grid(h, w)
+    pi = random_sim3(rng, 1.1)
+    pj = random_sim3(rng, 1.1)
+    qij = se3.quat_mul(pj[3:7], se3.quat_conj(pi[3:7]))
+    pj[:3] = se3.act_so3(qij, pi[:3])                             # tj = sij R ti -> tij = 0
+    tij, _, sij = opg.relative_sim3(pi[None], pj[None])
+    assert np.allclose(tij, 0, atol=1e-12) and np.isclose(sij[0], 1.0)
+    disp = rng.uniform(0.3, 1.0, size=(h, w))
+    t = terms(pi, pj, disp, h, w)
+    Ti, Tj = sim3_mat(pi), sim3_mat(pj)
+    for p in (0, 11, 19):
+        Ji = fd_columns(Ti, Tj, disp.reshape(-1)[p], u[p], v[p], 'i')
+        assert np.allclose(t['Ji'][0, :, p], Ji, atol=1e-5, rtol=1e-6)
+
+
+def test_reference_adjoint_differs_from_the_true_one_elsewhere():
+    """Away from sij == 1, tij == 0 the rotation / scale entries of Ji follow the reference's
+    formula, which is not the derivative (so a 'corrected' oracle would fail parity)."""
+    rng = np.random.default_rng(2)
+    h, w = 4, 5
+    u, v = oba.pixel_grid(h, w)
+    pi, pj = random_sim3(rng, 0.8), random_sim3(rng, 1.2)
+    disp = rng.uniform(0.3, 1.0, size=(h, w))
+    t = terms(pi, pj, disp, h, w)
+    Ji = fd_columns(sim3_mat(pi), sim3_mat(pj), disp.reshape(-1)[3], u[3], v[3], 'i')
+    assert not np.allclose(t['Ji'][0, :, 3, 3:], Ji[:, 3:], atol=1e-4)
+
+
+def test_linearize_blocks_are_weighted_jacobian_products():
+    prob = synth.pgba_problem(seed=3, n=4, h=6, w=8, radius=2, n_long=1, n_pg=2)
+    ii, jj = prob['ii'], prob['jj']
+    lin = opg.linearize(prob['poses'], prob['disps'], INTR, prob['targets'], prob['weights'], ii, jj)
+    h, w = 6, 8
+    e = 2
+    t = opg.edge_terms(prob['poses'][ii[e:e + 1]].astype(np.float64), prob['poses'][jj[e:e + 1]].astype(np.float64),
+                       prob['disps'][ii[e]].reshape(1, -1).astype(np.float64), INTR,
+                       prob['targets'][e:e + 1].reshape(1, 2, -1).astype(np.float64),
+                       prob['weights'][e:e + 1].reshape(1, 2, -1).astype(np.float64), np.array([False]), h, w)
+    J = np.concatenate([t['Ji'][0], t['Jj'][0]], -1).reshape(-1, 14)      # [2 HW, 14]
+    wv = t['w'][0].reshape(-1)
+    H = J.T @ (wv[:, None] * J)
+    g = J.T @ (wv * t['r'][0].reshape(-1))
+    assert np.allclose(lin['Hs'][0, e], H[:7, :7]) and np.allclose(lin['Hs'][1, e], H[:7, 7:])
+    assert np.allclose(lin['Hs'][2, e], H[7:, :7]) and np.allclose(lin['Hs'][3, e], H[7:, 7:])
+    assert np.allclose(lin['vs'][0, e], g[:7]) and np.allclose(lin['vs'][1, e], g[7:])
+
+
+def test_schur_solve_equals_joint_solve_at_block_size_7():
+    prob = synth.pgba_problem(seed=4, n=4, h=4, w=5, radius=3, n_long=0, t0=1, n_pg=0)
+    ii, jj, t0, t1 = prob['ii'], prob['jj'], prob['t0'], prob['t1']
+    lin = opg.linearize(prob['poses'], prob['disps'], INTR, prob['targets'], prob['weights'], ii, jj)
+    _, _, kx, _ = oba.index_structures(ii, jj, t0, t1)
+    eta = np.full((len(kx), 20), 1e-3)
+    sys = oba.reduced_system(lin, eta, ii, jj, t0, t1, 20)
+    # the image terms leave the scales unconstrained (their Jacobian column is zero), so both
+    # systems get the same 1e-2 I on the pose block: ep on the reduced diagonal with lm = 0
+    x, ok = oba.chol_solve(oba.damp(sys['L'], 0.0, 1e-2), sys['rhs'])
+    assert ok and x.size == 7 * (t1 - t0)
+    dz = oba.back_substitute(sys, x, t0, t1, t0_skip=False)
+    H, g, _ = oba.full_system(lin, eta, ii, jj, t0, t1, 20)
+    H[np.arange(x.size), np.arange(x.size)] += 1e-2
+    sol = np.linalg.solve(H, g)
+    assert np.allclose(x, sol[:x.size], rtol=1e-7, atol=1e-10)
+    assert np.allclose(dz.reshape(-1), sol[x.size:], rtol=1e-7, atol=1e-10)
+
+
+def test_pose_graph_blocks_place_as_the_quadratic_they_encode():
+    """With no image edges and no damping, dx is the stationary point of sum_n of
+    0.5 x_n^T H_n x_n - v_n^T x_n, x_n = (x[iip], x[jjp]) and x = 0 for frames before t0."""
+    rng = np.random.default_rng(5)
+    n, t0, Np = 5, 1, 6
+    iip = np.array([0, 1, 2, 3, 4, 1])
+    jjp = np.array([1, 2, 3, 4, 0, 3])
+    Hn = []
+    for _ in range(Np):
+        J = rng.normal(size=(7, 14))
+        Hn.append(J.T @ J + 0.1 * np.eye(14))
+    Hn = np.stack(Hn)
+    vn = rng.normal(size=(Np, 14))
+    Hsp = np.stack([Hn[:, :7, :7], Hn[:, :7, 7:], Hn[:, 7:, :7], Hn[:, 7:, 7:]])
+    vsp = np.stack([vn[:, :7], vn[:, 7:]])
+    poses = np.tile(np.array([0, 0, 0, 0, 0, 0, 1, 1.0]), (n, 1))
+    disps = np.ones((n, 2, 3))
+    F = n - t0
+    dx, dz, _ = opg.pgba(poses, disps, INTR, np.zeros((0, 2, 2, 3)), np.zeros((0, 2, 2, 3)), np.ones((F, 2, 3)),
+                         Hsp, vsp, np.zeros(0, np.int64), np.zeros(0, np.int64), iip, jjp, t0, n, lm=0.0, ep=0.0)
+    x = np.zeros((n, 7))
+    x[t0:] = dx
+    grad = np.zeros((n, 7))
+    for k in range(Np):
+        xn = np.concatenate([x[iip[k]], x[jjp[k]]])
+        gk = Hn[k] @ xn - vn[k]
+        grad[iip[k]] += gk[:7]
+        grad[jjp[k]] += gk[7:]
+    assert np.abs(grad[t0:]).max() < 1e-9 * max(1.0, np.abs(vn).max())
+    assert np.allclose(dz, 0)                                    # no image terms: w = 0
+
+
+def test_pgba_reduces_reprojection_error():
+    prob = synth.pgba_problem(seed=6, n=8, h=12, w=16, radius=2, n_long=3, n_pg=0)
+    p = {k: prob[k] for k in ('poses', 'disps', 'targets', 'weights', 'ii', 'jj')}
+
+    def rms(poses, disps):
+        lin_t = opg.edge_terms(poses[p['ii']].astype(np.float64), poses[p['jj']].astype(np.float64),
+                               disps[p['ii']].reshape(len(p['ii']), -1), INTR,
+                               p['targets'].reshape(len(p['ii']), 2, -1).astype(np.float64),
+                               p['weights'].reshape(len(p['ii']), 2, -1).astype(np.float64), p['ii'] == p['jj'], 12, 16)
+        return np.sqrt(np.sum(lin_t['w'] * lin_t['r'] ** 2) / np.sum(lin_t['w']))
+
+    before = rms(p['poses'], p['disps'].astype(np.float64))
+    dx, dz, disps = opg.pgba(prob['poses'], prob['disps'], INTR, prob['targets'], prob['weights'], prob['eta'],
+                             prob['Hsp'], prob['vsp'], prob['ii'], prob['jj'], prob['iip'], prob['jjp'],
+                             prob['t0'], prob['t1'], prob['lm'], prob['ep'])
+    assert dx.shape == (prob['t1'] - prob['t0'], 7)
+    assert np.all(np.isfinite(dx)) and np.abs(dz).max() > 0
+    # depth-only effect (poses unchanged): the joint step must not make the residual worse
+    assert rms(p['poses'], disps) < before
+
+
+def test_argument_contract():
+    prob = synth.pgba_problem(seed=7, n=4, h=4, w=5, radius=1, n_long=0, n_pg=1)
+    args = [prob[k] for k in ('poses', 'disps', 'intrinsics', 'targets', 'weights', 'eta', 'Hsp', 'vsp',
+                              'ii', 'jj', 'iip', 'jjp')]
+    with pytest.raises(ValueError):
+        opg.pgba(*args[:10], np.array([5]), np.array([0]), 1, 4)
+    with pytest.raises(ValueError):
+        opg.pgba(args[0][:, :7], *args[1:], 1, 4)
+    with pytest.raises(ValueError):
+        opg.pgba(*args, 2, 2)
