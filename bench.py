@@ -39,10 +39,11 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
-    ap.add_argument('--only', choices=['all', 'lookup', 'mono', 'geometry'], default='all',
+    ap.add_argument('--only', choices=['all', 'lookup', 'mono', 'geometry', 'pgba'], default='all',
                     help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
                          'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph; '
-                         'geometry: time frame_distance, depth_filter and iproj on it')
+                         'geometry: time frame_distance, depth_filter and iproj on it; '
+                         'pgba: time the Sim3 pose-graph BA step on a 300-keyframe loop-closure graph')
     return ap.parse_args()
 
 
@@ -64,6 +65,9 @@ def main():
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
 
+    if args.only == 'pgba':
+        pgba_bench(db, dev, args)
+        return
     log('generating global300')
     p = synth.ba_global300()
     E, iters = len(p.ii), p.iterations
@@ -318,6 +322,46 @@ def mono_bench(db, p, up, dev, args):
                       'proj_trans_GBps': round(pt_bytes / (res['proj_trans'] / 1e3) / 1e9, 1),
                       'proj_trans_bytes': pt_bytes, 'frames': M, 'edges': len(p.ii),
                       'note': 'proj_trans_ms includes the host-side edge grouping and its upload'}), flush=True)
+
+
+def pgba_bench(db, dev, args):
+    """One Sim3 pose-graph BA step (droid_backends.pgba) on a loop-closure graph of 300 keyframes
+    at 48x64: poses 8..299 free (292 x 7 parameters), 638 image edges (neighbours plus 40 long
+    edges), 320 pose-graph pairs.  The timed call includes the host-side planning, the upload of
+    its tables and the final synchronisation, as every call of the reference plans on the host."""
+    import numpy as np
+    import torch
+    from hislam2_amd import synth
+    q = synth.pgba_problem(seed=5, n=300, radius=1, n_long=40, t0=8, n_pg=320)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in q.items() if isinstance(v, np.ndarray)}
+    disps0 = t['disps'].clone()
+
+    def call():
+        return db.pgba(t['poses'], t['disps'], t['intrinsics'], t['targets'], t['weights'], t['eta'], t['Hsp'],
+                       t['vsp'], t['ii'], t['jj'], t['iip'], t['jjp'], q['t0'], q['t1'], q['lm'], q['ep'])
+
+    for _ in range(max(args.warmup, 1)):
+        call()
+        t['disps'].copy_(disps0)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        t['disps'].copy_(disps0)
+    ms = 1e3 * float(np.median(times))
+    line = {'metric': 'pgba step on loop300', 'pgba_ms': round(ms, 4), 'poses': q['t1'] - q['t0'],
+            'edges': len(q['ii']), 'pose_graph_pairs': len(q['iip']), 'hw': list(q['disps'].shape[1:])}
+    if not args.no_cpu_baseline:
+        from oracle import pgba as opg
+        c0 = time.perf_counter()
+        opg.pgba(q['poses'], q['disps'], q['intrinsics'], q['targets'], q['weights'], q['eta'], q['Hsp'], q['vsp'],
+                 q['ii'], q['jj'], q['iip'], q['jjp'], q['t0'], q['t1'], q['lm'], q['ep'])
+        line['cpu_baseline'] = {'value': round(1e3 * (time.perf_counter() - c0), 1), 'unit': 'ms per step',
+                                'cores': 1, 'kind': 'port', 'sample': 'one full step of the same graph'}
+    print(json.dumps(line), flush=True)
 
 
 def geometry_bench(db, p, up, args):
