@@ -127,3 +127,46 @@ def test_pgba_rejects_out_of_range_indices(db):
     q['jjp'] = np.array([1, 8], np.int64)
     with pytest.raises(ValueError):
         run_both(db, q)
+
+
+def test_cuda_pgba_loop_matches_oracle_loop(db):
+    """hislam2_amd.geom.sim3.cuda_pgba (two pgba steps with the pose-graph constraints rebuilt
+    before each) against the same loop on the CPU with the float64 oracle's step."""
+    from hislam2_amd.geom import sim3
+    q = synth.pgba_problem(seed=31, n=10, t0=1, n_pg=4)
+    n, h, w = q['disps'].shape
+    E = len(q['ii'])
+    rng = np.random.default_rng(31)
+    iip = np.array([0, 2, 4, 1], np.int64)
+    jjp = np.array([5, 7, 9, 6], np.int64)
+    rel = torch.stack([sim3.exp(torch.tensor(rng.normal(scale=0.05, size=7) * [1, 1, 1, 1, 1, 1, 0]))[:7]
+                       for _ in range(len(iip))]).float()
+    covs = torch.tensor(rng.uniform(0.01, 0.1, size=(len(iip), 6)), dtype=torch.float32)
+    target = torch.from_numpy(q['targets']).permute(0, 2, 3, 1).contiguous()[None]      # [1, E, h, w, 2]
+    weight = torch.from_numpy(q['weights']).permute(0, 2, 3, 1).contiguous()[None]
+    poses_g = dev(q['poses'])
+    disps_g = dev(q['disps'])
+    print(f"cuda_pgba loop: E={E} Np={len(iip)} t0={q['t0']} t1={q['t1']}", flush=True)
+    sim3.cuda_pgba(poses_g, disps_g, dev(q['intrinsics']), target.to(DEV), weight.to(DEV), dev(q['eta']),
+                   dev(q['ii']), dev(q['jj']), dev(iip), dev(jjp), rel.to(DEV), covs.to(DEV), t0=q['t0'], t1=q['t1'],
+                   itrs=2, lm=q['lm'], ep=0.1)
+    torch.cuda.synchronize()
+    # the same loop on the CPU with the oracle's step
+    poses = torch.from_numpy(q['poses'])[None].clone()
+    disps = q['disps'].astype(np.float64)
+    infos = 1.0 / covs
+    infos = torch.cat((infos, infos.min(dim=1, keepdim=True)[0]), dim=1)
+    infos = infos.unsqueeze(2).expand(*infos.size(), 7) * torch.eye(7)[None]
+    for _ in range(2):
+        Hsp, vsp, _, _ = sim3.global_relative_posesim3_constraints(torch.from_numpy(iip), torch.from_numpy(jjp),
+                                                                    poses, rel[None], infos, pw=1e-3)
+        dx, dz, disps = opg.pgba(poses[0].numpy(), disps, q['intrinsics'], q['targets'], q['weights'], q['eta'],
+                                 Hsp.numpy(), vsp.numpy(), q['ii'], q['jj'], iip, jjp, q['t0'], q['t1'], q['lm'], 0.1)
+        poses[0, q['t0']:q['t1']] = sim3.retr(poses[0, q['t0']:q['t1']].double(), torch.from_numpy(dx)).float()
+    disps = np.clip(disps, 0.001, 10)
+    got = poses_g.cpu().numpy()
+    ref = poses[0].numpy()
+    assert rel_err(got[:, :3], ref[:, :3]) <= 1e-4, rel_err(got[:, :3], ref[:, :3])
+    assert np.abs(got[:, 3:7] - ref[:, 3:7]).max() <= 1e-5
+    assert np.abs(np.log(got[:, 7] / ref[:, 7])).max() <= 2e-4
+    assert rel_err(disps_g.cpu().numpy(), disps) <= 1e-4
