@@ -170,3 +170,11 @@ def test_cuda_pgba_loop_matches_oracle_loop(db):
     assert np.abs(got[:, 3:7] - ref[:, 3:7]).max() <= 1e-5
     assert np.abs(np.log(got[:, 7] / ref[:, 7])).max() <= 2e-4
     assert rel_err(disps_g.cpu().numpy(), disps) <= 1e-4
+
+
+def test_pgba_loop300_matches_oracle(db):
+    """The bench graph: 300 keyframes, poses 8..299 free, 638 edges, 320 pose-graph pairs."""
+    q = synth.pgba_problem(seed=5, n=300, radius=1, n_long=40, t0=8, n_pg=320)
+    got, ref = run_both(db, q)
+    assert ref[3]['ok']
+    check(got, ref, q['ep'])
