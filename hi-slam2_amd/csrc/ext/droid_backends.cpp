@@ -903,6 +903,53 @@ class GNStep {
 };
 
 // ---- pgba (src/droid.cpp:171-195) ----------------------------------------------
+// The last few pgba plans, keyed by the concatenated index lists ii ++ jj ++ iip ++ jjp and the
+// sizes; a hit compares the whole key.
+class PgbaPlanCache {
+ public:
+  using Args = std::array<int64_t, 5>;   // num_poses, ht, wd, t0, t1
+  using Ptr = std::shared_ptr<const hsl2_pgba_plan>;
+
+  Ptr get(const int64_t* idx, int64_t E, int64_t Np, const Args& a) {
+    const size_t n = static_cast<size_t>(2 * E + 2 * Np);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (auto it = entries_.begin(); it != entries_.end(); ++it) {
+        if (it->args == a && it->E == E && it->idx.size() == n &&
+            std::memcmp(it->idx.data(), idx, n * sizeof(int64_t)) == 0) {
+          entries_.splice(entries_.begin(), entries_, it);
+          return entries_.front().plan;
+        }
+      }
+    }
+    hsl2_pgba_plan* raw = nullptr;
+    raise_status(hsl2_pgba_plan_create(idx, idx + E, E, idx + 2 * E, idx + 2 * E + Np, Np, a[0], a[1], a[2],
+                                       static_cast<int>(a[3]), static_cast<int>(a[4]), &raw),
+                 "pgba");
+    Ptr plan(raw, hsl2_pgba_plan_destroy);
+    std::lock_guard<std::mutex> lock(mu_);
+    entries_.push_front(Entry{a, E, std::vector<int64_t>(idx, idx + n), plan});
+    if (entries_.size() > kCapacity) entries_.pop_back();
+    return plan;
+  }
+
+ private:
+  struct Entry {
+    Args args;
+    int64_t E;
+    std::vector<int64_t> idx;
+    Ptr plan;
+  };
+  static constexpr size_t kCapacity = 4;
+  std::mutex mu_;
+  std::list<Entry> entries_;
+};
+
+PgbaPlanCache& pgba_plan_cache() {
+  static PgbaPlanCache cache;
+  return cache;
+}
+
 // Sim3 pose-graph BA step: [dx [t1 - t0, 7], dz [F, ht * wd]], disps updated in place.
 std::vector<torch::Tensor> pgba(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
                                 torch::Tensor targets, torch::Tensor weights, torch::Tensor eta, torch::Tensor Hsp,
@@ -927,13 +974,12 @@ std::vector<torch::Tensor> pgba(torch::Tensor poses, torch::Tensor disps, torch:
           "pgba: targets and weights must be [E, 2, ht, wd]");
   require(Hsp.numel() == Np * 4 * 49 && vsp.numel() == Np * 2 * 7, "pgba: Hsp must be [4, Np, 7, 7], vsp [2, Np, 7]");
   c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
-  torch::Tensor ii_h = ii.to(torch::kCPU), jj_h = jj.to(torch::kCPU);
-  torch::Tensor iip_h = iip.to(torch::kCPU), jjp_h = jjp.to(torch::kCPU);
-  hsl2_pgba_plan* raw = nullptr;
-  raise_status(hsl2_pgba_plan_create(ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E, iip_h.data_ptr<int64_t>(),
-                                     jjp_h.data_ptr<int64_t>(), Np, poses.size(0), ht, wd, t0, t1, &raw),
-               "pgba");
-  std::unique_ptr<hsl2_pgba_plan, void (*)(hsl2_pgba_plan*)> plan(raw, hsl2_pgba_plan_destroy);
+  // one host copy of the four index lists (the caller's loop repeats the same graph: update_pgba
+  // makes 6 x 2 calls per loop closure), then the cached plan when the lists and sizes repeat
+  torch::Tensor idx_h = torch::cat({ii, jj, iip, jjp}).to(torch::kCPU);
+  const int64_t* ih = idx_h.data_ptr<int64_t>();
+  const std::shared_ptr<const hsl2_pgba_plan> plan =
+      pgba_plan_cache().get(ih, E, Np, {poses.size(0), ht, wd, t0, t1});
   const int64_t F = hsl2_pgba_plan_num_frames(plan.get());
   require(eta.numel() % (ht * wd) == 0, "pgba: eta must hold whole [ht, wd] rows");
   const int64_t eta_rows = eta.numel() / (ht * wd);
