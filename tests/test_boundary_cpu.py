@@ -110,3 +110,41 @@ def test_gn_step_rejects_cpu_tensors(db):
         step.forward(z(2, 7), z(2, 4, 8), z(2, 4), z(2, 2, 4, 8), z(2, 2, 4, 8), z(2, 4, 8), 1e-4, 0.1, True)
     with pytest.raises(ValueError):
         db.projective_transform(z(2, 7), z(2, 4, 8), z(2, 4), torch.tensor([0]), torch.tensor([1]))
+
+
+def _pgba_plan(lib, ii, jj, iip, jjp, n, t0, t1, h=6, w=8):
+    lib.hsl2_pgba_plan_create.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int64] + [ctypes.c_void_p] * 2 + \
+        [ctypes.c_int64] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    arrs = [np.ascontiguousarray(a, np.int64) for a in (ii, jj, iip, jjp)]
+    plan = ctypes.c_void_p()
+    st = lib.hsl2_pgba_plan_create(arrs[0].ctypes.data, arrs[1].ctypes.data, len(arrs[0]), arrs[2].ctypes.data,
+                                   arrs[3].ctypes.data, len(arrs[2]), n, h, w, t0, t1, ctypes.byref(plan))
+    return st, plan
+
+
+def test_pgba_planner_frames_and_index_checks(lib):
+    """The pgba planner's depth frames are the sorted unique frames of [t0, t1) ++ ii (as
+    torch._unique in src/droid_kernels.cu:1963-1968); indices at or past t1 are rejected on the
+    host with HSL2_ERR_INDEX, before any GPU work."""
+    lib.hsl2_pgba_plan_num_frames.restype = ctypes.c_int64
+    lib.hsl2_pgba_plan_num_frames.argtypes = [ctypes.c_void_p]
+    lib.hsl2_pgba_plan_frames.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.hsl2_pgba_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.hsl2_pgba_workspace_bytes.restype = ctypes.c_size_t
+    lib.hsl2_pgba_workspace_bytes.argtypes = [ctypes.c_void_p]
+    q = synth.pgba_problem(seed=2, n=12, t0=4, n_pg=5)
+    st, plan = _pgba_plan(lib, q['ii'], q['jj'], q['iip'], q['jjp'], 12, q['t0'], q['t1'])
+    assert st == 0
+    F = lib.hsl2_pgba_plan_num_frames(plan)
+    kx = np.zeros(F, np.int64)
+    assert lib.hsl2_pgba_plan_frames(plan, kx.ctypes.data) == 0
+    _, _, rkx, _ = oba.index_structures(q['ii'], q['jj'], q['t0'], q['t1'])
+    assert np.array_equal(kx, rkx)
+    assert lib.hsl2_pgba_workspace_bytes(plan) > 0
+    lib.hsl2_pgba_plan_destroy(plan)
+    for bad in ((q['ii'], np.r_[q['jj'][:-1], 12], q['iip'], q['jjp']),
+                (q['ii'], q['jj'], q['iip'], np.r_[q['jjp'][:-1], -1])):
+        st, _ = _pgba_plan(lib, *bad, 12, q['t0'], q['t1'])
+        assert st == 3
+    st, _ = _pgba_plan(lib, q['ii'], q['jj'], q['iip'], q['jjp'], 12, 5, 5)
+    assert st == 1
