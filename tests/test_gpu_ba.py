@@ -1,6 +1,6 @@
 """GPU parity of droid_backends.ba against the float64 oracle (ba_cuda semantics).
 
-Tolerances (north star: pose / depth updates within 1e-4 relative):
+Tolerances (pose / depth updates within 1e-4 relative):
 * the updates dx, dz of a single iteration from identical inputs: max |error| <= 1e-4 x
   max |oracle update|;
 * after several iterations, the updated poses and disparities: 1e-4 relative, and the last
