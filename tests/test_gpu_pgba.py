@@ -178,3 +178,15 @@ def test_pgba_loop300_matches_oracle(db):
     got, ref = run_both(db, q)
     assert ref[3]['ok']
     check(got, ref, q['ep'])
+
+
+@pytest.mark.parametrize('t0,t1', [(1, 10), (9, 10)])
+def test_pgba_without_pose_graph_pairs(db, t0, t1):
+    """Np = 0 (empty Hsp / vsp), and a single free pose (t0 = t1 - 1)."""
+    q = synth.pgba_problem(seed=40 + t0, n=10, t0=t0, n_pg=0)
+    q['t1'] = t1
+    frames = np.unique(np.concatenate([np.arange(t0, t1), q['ii']]))
+    q['eta'] = np.ascontiguousarray(np.full((len(frames),) + q['disps'].shape[1:], 1e-3, np.float32))
+    got, ref = run_both(db, q, ep=0.1)
+    assert ref[3]['ok']
+    check(got, ref, 0.1)
