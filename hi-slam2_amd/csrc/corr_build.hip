@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTP = 32;       // p rows per MFMA operand
+constexpr int kER = 16;       // p rows staged in LDS and emitted per pass (two passes per operand)
 constexpr int kPH = 1;        // 32-row operands per block (2 shares every fmap2 fragment twice but
                               // halves occupancy; measured no faster at 48x64, 128 channels)
 constexpr int kRows = 8;      // image-2 rows per block
@@ -75,7 +76,7 @@ struct alignas(CW * sizeof(To)) OutV {
 };
 
 // Pooled level (hierarchical: each level averages 2x2 of the level above it, as repeated
-// avg_pool2d does).  src holds the level above for the tile ([kTP][RS][WS] floats, pitch PS per
+// avg_pool2d does).  src holds the level above for the tile ([kER][RS][WS] floats, pitch PS per
 // p); the values are stored to `out` (layout `layout`, pyramid_offset) and returned in v, so
 // that the caller can place them in LDS for the next level once every thread has finished
 // reading src.  Thread chunk c covers CW consecutive x of one (p, y): with p the slowest index,
@@ -84,7 +85,7 @@ struct alignas(CW * sizeof(To)) OutV {
 template <int RS, int WS>
 struct Pool {
   static constexpr int RD = RS / 2, WD = WS / 2, CW = chunk_of(WD);
-  static constexpr int NCH = kTP * RD * WD / CW;     // chunks of the tile
+  static constexpr int NCH = kER * RD * WD / CW;     // chunks of the tile
   static constexpr int NIT = (NCH + kThreads - 1) / kThreads;
   static constexpr int PD = RD * WD + 4;             // pitch per p of this level in LDS
 };
@@ -93,8 +94,8 @@ template <int RS, int WS>
 __device__ __forceinline__ void pool_split(int c, bool pfast, int& p, int& y, int& x) {
   using PL = Pool<RS, WS>;
   constexpr int WD = PL::WD, CW = PL::CW, PER = PL::RD * WD / CW;
-  const int r = pfast ? c / kTP : c % PER;
-  p = pfast ? c % kTP : c / PER;
+  const int r = pfast ? c / kER : c % PER;
+  p = pfast ? c % kER : c / PER;
   y = r / (WD / CW);
   x = (r % (WD / CW)) * CW;
 }
@@ -160,7 +161,7 @@ __device__ __forceinline__ void pool_stash(float* __restrict__ dst, int layout,
   }
 }
 
-// The f32 tile of 32 p rows staged in LDS -> level 0 and the pooled levels of those rows.
+// The f32 tile of kER p rows staged in LDS -> level 0 and the pooled levels of those rows.
 // pl0 = index of the first p row's plane.  Ends with every thread done reading LDS.
 template <typename To, int WT>
 __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__ sout, To* __restrict__ l0,
@@ -173,7 +174,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   // level 0: row segments of 8 elements (one row of an 8x8 block in the tiled layout), one
   // vector store each; consecutive threads read consecutive segments of the LDS tile
   const int lay0 = pyramid_layout(0, HW, h, w, tiled);
-  constexpr int NCH0 = kTP * NQ / 8;
+  constexpr int NCH0 = kER * NQ / 8;
 #pragma unroll
   for (int c0 = 0; c0 < NCH0; c0 += kThreads) {
     const int c = c0 + t;
@@ -195,7 +196,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   using P2 = Pool<kRows / 2, WT / 2>;
   using P3 = Pool<kRows / 4, WT / 4>;
   float* s1 = sout;
-  float* s2 = s1 + kTP * P1::PD;
+  float* s2 = s1 + kER * P1::PD;
   {
     float v[P1::NIT][P1::CW];
     const int hl = h >> 1, wl = w >> 1, lay = pyramid_layout(1, HW, hl, wl, tiled);
@@ -237,7 +238,7 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   constexpr int SQ = NQ + 4;                       // padded LDS row of the f32 tile
   constexpr int PW = NQ / 32 / 4;                  // 32-wide MFMA blocks per wave
   extern __shared__ __align__(16) unsigned char smem[];
-  float* sout = reinterpret_cast<float*>(smem);    // [kTP][SQ]
+  float* sout = reinterpret_cast<float*>(smem);    // [kER][SQ]
 
   // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a contiguous range of
   // logical blocks (whole edges), so an edge's feature maps are fetched into one L2 only.
@@ -297,18 +298,22 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   // is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32
 #pragma unroll
   for (int half = 0; half < kPH; ++half) {
+    // rows 0-15 of the operand are values 0-7 of each lane, rows 16-31 values 8-15
 #pragma unroll
-    for (int k = 0; k < PW; ++k) {
-      const int q = (wave * PW + k) * 32 + lane % 32;
+    for (int part = 0; part < kTP / kER; ++part) {
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        sout[p * SQ + q] = (half ? acc1[k][v] : acc0[k][v]) * 0.0625f;
+      for (int k = 0; k < PW; ++k) {
+        const int q = (wave * PW + k) * 32 + lane % 32;
+#pragma unroll
+        for (int v = 8 * part; v < 8 * part + 8; ++v) {
+          const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4) - kER * part;
+          sout[p * SQ + q] = (half ? acc1[k][v] : acc0[k][v]) * 0.0625f;
+        }
       }
+      __syncthreads();
+      emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + half * kTP + part * kER, h, w, y0,
+                        x0, levels, tiled);
     }
-    __syncthreads();
-    emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + half * kTP, h, w, y0, x0,
-                      levels, tiled);
   }
 }
 
@@ -316,7 +321,7 @@ template <typename Ti, typename To, int WT>
 int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
               int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
   constexpr int NQ = kRows * WT, SQ = NQ + 4;
-  const size_t lds = sizeof(float) * kTP * SQ;
+  const size_t lds = sizeof(float) * kER * SQ;
   HSL2_REQUIRE(lds <= 160 * 1024, HSL2_ERR_UNSUPPORTED, "corr build: LDS budget exceeded");
   const int ptiles = static_cast<int>(h * w / (kPH * kTP));
   const int qtiles = static_cast<int>((h / kRows) * (w / WT));
