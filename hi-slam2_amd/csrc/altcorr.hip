@@ -41,6 +41,41 @@ channels_last_kernel(const T* __restrict__ in, T* __restrict__ out, int C, int P
   }
 }
 
+// The same conversion for P and C multiples of 64 and 16-byte aligned buffers: one block per
+// 64 pixels x 64 channels, 16-byte loads along the pixels of a channel (one 128-byte line per
+// channel row in bf16) and 16-byte stores along the channels of a pixel, transposed in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256)
+channels_last_vec_kernel(const T* __restrict__ in, T* __restrict__ out, int C, int P) {
+  constexpr int VE = 16 / sizeof(T);               // elements per 16-byte vector
+  constexpr int LD = 64 + VE;                      // padded LDS row (pixel-major)
+  __shared__ T tile[64][LD];                       // [pixel][channel]
+  const int64_t row = blockIdx.z;
+  const int c0 = blockIdx.y * 64, p0 = blockIdx.x * 64;
+  const T* src = in + row * (int64_t)C * P;
+  T* dst = out + row * (int64_t)C * P;
+  const int t = threadIdx.x;
+  constexpr int NV = 64 * 64 / VE;                 // vectors per tile
+#pragma unroll
+  for (int i = t; i < NV; i += 256) {
+    const int c = i / (64 / VE), pv = (i % (64 / VE)) * VE;
+    const uint4 v = *reinterpret_cast<const uint4*>(src + (int64_t)(c0 + c) * P + p0 + pv);
+    const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+    for (int k = 0; k < VE; ++k) tile[pv + k][c] = e[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = t; i < NV; i += 256) {
+    const int p = i / (64 / VE), cv = (i % (64 / VE)) * VE;
+    uint4 v;
+    T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+    for (int k = 0; k < VE; ++k) e[k] = tile[p][cv + k];
+    *reinterpret_cast<uint4*>(dst + (int64_t)(p0 + p) * C + c0 + cv) = v;
+  }
+}
+
 // [rows][P][C] f32 -> [rows][C][P] f32 (gradients back to the reference layout)
 __global__ void __launch_bounds__(256)
 channels_first_kernel(const float* __restrict__ in, float* __restrict__ out, int C, int P) {
@@ -404,6 +439,19 @@ extern "C" int hsl2_channels_last(const void* in, void* out, int dtype, int64_t 
             static_cast<unsigned>(rows));
   hipStream_t s = as_stream(stream);
   const int C = static_cast<int>(channels), P = static_cast<int>(pixels);
+  const bool vec = channels % 64 == 0 && pixels % 64 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(out) % 16 == 0;
+  if (vec && dtype != HSL2_F64) {
+    const dim3 g(static_cast<unsigned>(pixels / 64), static_cast<unsigned>(channels / 64), static_cast<unsigned>(rows));
+    switch (dtype) {
+      case HSL2_F32: hipLaunchKernelGGL(channels_last_vec_kernel<float>, g, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float*>(out), C, P); break;
+      case HSL2_F16: hipLaunchKernelGGL(channels_last_vec_kernel<__half>, g, dim3(256), 0, s, static_cast<const __half*>(in), static_cast<__half*>(out), C, P); break;
+      case HSL2_BF16: hipLaunchKernelGGL(channels_last_vec_kernel<__hip_bfloat16>, g, dim3(256), 0, s, static_cast<const __hip_bfloat16*>(in), static_cast<__hip_bfloat16*>(out), C, P); break;
+      default: set_error("channels_last: unsupported dtype"); return HSL2_ERR_ARGUMENT;
+    }
+    HSL2_LAUNCH_CHECK();
+    return HSL2_OK;
+  }
   switch (dtype) {
     case HSL2_F32: hipLaunchKernelGGL(channels_last_kernel<float>, grid, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float*>(out), C, P); break;
     case HSL2_F16: hipLaunchKernelGGL(channels_last_kernel<__half>, grid, dim3(256), 0, s, static_cast<const __half*>(in), static_cast<__half*>(out), C, P); break;

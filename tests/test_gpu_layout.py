@@ -1,0 +1,36 @@
+"""GPU check of the channel-last conversion hsl2_channels_last ([rows][C][P] -> [rows][P][C]),
+called through the C ABI: the 16-byte vector path (C and P multiples of 64) and the general
+path, for f32, f16 and bf16, against torch's permute (exact: pure data movement)."""
+import ctypes
+import os
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CODES = {torch.float32: 0, torch.float16: 1, torch.bfloat16: 2}
+
+
+@pytest.fixture(scope='module')
+def lib():
+    from hislam2_amd import backend
+    backend()                                   # loads libhsl2.so next to the extension
+    lib = ctypes.CDLL(os.path.join(ROOT, 'hi-slam2_amd', 'libhsl2.so'))
+    lib.hsl2_channels_last.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                       ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]
+    lib.hsl2_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float16, torch.bfloat16])
+@pytest.mark.parametrize('shape', [(3, 128, 3072), (2, 64, 192), (2, 40, 100)])
+def test_channels_last_matches_permute(lib, dtype, shape):
+    rows, C, P = shape
+    print(f"channels_last case: {dtype} {shape}", flush=True)
+    x = torch.randn(rows, C, P, device='cuda:0').to(dtype)
+    y = torch.empty(rows, P, C, device='cuda:0', dtype=dtype)
+    st = lib.hsl2_channels_last(x.data_ptr(), y.data_ptr(), CODES[dtype], rows, C, P, None)
+    assert st == 0, lib.hsl2_last_error()
+    torch.cuda.synchronize()
+    assert torch.equal(y, x.permute(0, 2, 1))
