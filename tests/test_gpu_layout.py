@@ -24,7 +24,7 @@ def lib():
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float16, torch.bfloat16])
-@pytest.mark.parametrize('shape', [(3, 128, 3072), (2, 64, 192), (2, 40, 100)])
+@pytest.mark.parametrize('shape', [(3, 128, 3072), (2, 256, 128), (2, 64, 192), (2, 40, 100)])
 def test_channels_last_matches_permute(lib, dtype, shape):
     rows, C, P = shape
     print(f"channels_last case: {dtype} {shape}", flush=True)
