@@ -24,6 +24,8 @@
 //                        the disparity update (EvT6x1, disp_retr, 1562-1582, 1398-1411).
 //   7. ba_pose_retr      T <- Exp(dx) T (1342-1396).
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "ba_kernels.hpp"
@@ -460,6 +462,58 @@ extern "C" int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps,
     *solve_ok = flags[1] == 0 ? 1 : 0;
   }
   return HSL2_OK;
+}
+
+// ---- hsl2_ba_run_static: the same run with a library-owned static workspace ------------------
+// For callers that cannot allocate device memory.  The workspace is a file-scope device array:
+// each GPU the library runs on holds its own copy (HSL2_BA_STATIC_SCRATCH_BYTES) until the
+// process ends.  Host code never passes the array's name to a kernel or a copy (on the host it
+// is a stand-in, not a device address); the device address comes from hipGetSymbolAddress,
+// taken once per device with that device current and kept in scratch_ptr[].  Every call on one
+// GPU shares the array, so calls that overlap on one GPU race.  The kernels need no zeroed
+// workspace: ba_clear_flags resets the solver flags and the Cholesky resets its own counters,
+// every other buffer is written before it is read within an iteration.
+namespace hsl2 {
+namespace {
+static __device__ __align__(256) unsigned char scratch[HSL2_BA_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kMaxStaticDevices] = {};
+
+int static_scratch(void** out) {
+  int dev = -1;
+  HSL2_HIP(hipGetDevice(&dev));
+  HSL2_REQUIRE(dev >= 0 && dev < kMaxStaticDevices, HSL2_ERR_UNSUPPORTED,
+               "ba_run_static: device index beyond the static workspace table");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    HSL2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HSL2_REQUIRE(p && (reinterpret_cast<uintptr_t>(p) % 256) == 0, HSL2_ERR_HIP,
+                 "ba_run_static: static workspace has no aligned device address");
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return HSL2_OK;
+}
+}  // namespace
+}  // namespace hsl2
+
+extern "C" int hsl2_ba_run_static(const hsl2_ba_plan* plan, float* poses, float* disps,
+                                  const float* intrinsics, const float* targets, const float* weights,
+                                  const float* eta, int64_t eta_rows, int iterations, float lm, float ep,
+                                  float* dx, float* dz, int* solve_ok, void* stream) {
+  HSL2_REQUIRE(plan, HSL2_ERR_ARGUMENT, "ba_run_static: null plan");
+  HSL2_REQUIRE(plan->total <= HSL2_BA_STATIC_SCRATCH_BYTES, HSL2_ERR_WORKSPACE,
+               "ba_run_static: the plan needs " + std::to_string(plan->total) +
+                   " bytes of workspace, more than the static " +
+                   std::to_string(static_cast<size_t>(HSL2_BA_STATIC_SCRATCH_BYTES)) + " bytes");
+  void* ws = nullptr;
+  const int st = static_scratch(&ws);
+  if (st) return st;
+  return hsl2_ba_run(plan, poses, disps, intrinsics, targets, weights, eta, eta_rows, iterations, lm, ep, dx,
+                     dz, ws, HSL2_BA_STATIC_SCRATCH_BYTES, solve_ok, stream);
 }
 
 extern "C" int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, double** Hs,

@@ -6,8 +6,8 @@
  * the last failure on the calling thread is available from hsl2_last_error().
  * Device pointers must be allocated on the device that is current when the
  * call is made; `stream` is a hipStream_t passed as void* (NULL = the null
- * stream).  No entry point allocates device memory: scratch is always supplied by the
- * caller.
+ * stream).  No entry point allocates device memory: scratch is supplied by the caller,
+ * except for hsl2_ba_run_static, whose workspace is a static device array of the library.
  *
  * Each function names the reference interface it replaces (paths relative to
  * the HI-SLAM2 source tree).
@@ -202,6 +202,20 @@ int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps, const floa
                 const float* targets, const float* weights, const float* eta, int64_t eta_rows,
                 int iterations, float lm, float ep, float* dx, float* dz,
                 void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
+
+/* No counterpart in the reference.  hsl2_ba_run for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight tracking loops, harnesses that forbid
+ * hipMalloc): the workspace is a static device array of HSL2_BA_STATIC_SCRATCH_BYTES held by
+ * the library on each GPU it runs on until the process ends, so the caller passes inputs and
+ * outputs only.  Plans needing more workspace (hsl2_ba_plan_workspace_bytes) are rejected with
+ * HSL2_ERR_WORKSPACE; frontend windows fit (12 keyframes, 70 edges at 48x64 need
+ * 12.7 MiB).  All calls on one GPU share that array and keep the last call's contents: calls
+ * that overlap on one GPU race.  Allocates nothing; otherwise identical to hsl2_ba_run. */
+#define HSL2_BA_STATIC_SCRATCH_BYTES (64u << 20)
+int hsl2_ba_run_static(const hsl2_ba_plan* plan, float* poses, float* disps, const float* intrinsics,
+                       const float* targets, const float* weights, const float* eta, int64_t eta_rows,
+                       int iterations, float lm, float ep, float* dx, float* dz, int* solve_ok,
+                       void* stream);
 
 /* One Gauss-Newton step in the torch semantics: replaces BA (hislam2/geom/ba.py:32-107, with
  * schur_solve and CholeskySolver of hislam2/geom/chol.py:5-78) and, for a motion-only plan,
