@@ -39,11 +39,12 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
-    ap.add_argument('--only', choices=['all', 'lookup', 'mono', 'geometry', 'pgba'], default='all',
+    ap.add_argument('--only', choices=['all', 'lookup', 'altcorr', 'mono', 'geometry', 'pgba'], default='all',
                     help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
                          'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph; '
                          'geometry: time frame_distance, depth_filter and iproj on it; '
-                         'pgba: time the Sim3 pose-graph BA step on a 300-keyframe loop-closure graph')
+                         'pgba: time the Sim3 pose-graph BA step on a 300-keyframe loop-closure graph; '
+                         'altcorr: time the on-the-fly correlation (AltCorrBlock) on global300 and hires64')
     return ap.parse_args()
 
 
@@ -67,6 +68,9 @@ def main():
 
     if args.only == 'pgba':
         pgba_bench(db, dev, args)
+        return
+    if args.only == 'altcorr':
+        altcorr_bench(dev, args)
         return
     log('generating global300')
     p = synth.ba_global300()
@@ -275,6 +279,71 @@ def main():
             'reproj_diff_px': round(abs(res['split']['rms'] - res['plain']['rms']), 7),
         })
     print(json.dumps(line), flush=True)
+
+
+MFMA_PEAK_TFLOPS = {'bfloat16': 2500.0, 'float16': 2500.0, 'float32': 157.3}
+
+
+def time_launches(fn, steps, warmup):
+    """Mean milliseconds of fn() over `steps` launches, HIP events on torch's current stream (the
+    stream the extension launches on)."""
+    import torch
+    for _ in range(max(warmup, 1)):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def altcorr_bench(dev, args):
+    """The on-the-fly correlation (AltCorrBlock: feature pyramid built once, all levels per call in
+    one launch on the matrix cores) over every edge of two workloads:
+      global300 -- 300 keyframes at 48x64, 3000 edges, radius 3, 4 levels (196 channels);
+      hires64   -- 64 keyframes at 96x128, 800 edges, radius 4, 4 levels (324 channels);
+    coords = pixel grid + N(0, 2 px) flow.  Per lookup (one pixel-edge): algorithmic flops
+    L (2r+2)^2 2C; algorithmic bytes = the feature maps once per pass, F HW C s (1 + 1.328), plus
+    the outputs L (2r+1)^2 s_out.  One JSON line per workload and dtype."""
+    import numpy as np
+    import torch
+    from hislam2_amd import corr as hcorr
+    from hislam2_amd import synth
+    work = [('global300', synth.ba_global300, 3, 7, 8), ('hires64', synth.ba_hires64, 4, 64, 65)]
+    for name, gen, r, fseed, cseed in work:
+        p = gen()
+        n, h, w = p.disps.shape
+        E = len(p.ii)
+        fm = torch.from_numpy(synth.fmaps(fseed, n, h=h, w=w)).to(dev)
+        coords = torch.from_numpy(synth.lookup_coords(cseed, E, h=h, w=w)).to(dev)
+        ii, jj = torch.from_numpy(p.ii).to(dev), torch.from_numpy(p.jj).to(dev)
+        lookups = E * h * w
+        L, C, D, R = 4, fm.shape[2], 2 * r + 2, 2 * r + 1
+        dtypes = [torch.bfloat16, torch.float32] + ([torch.float8_e4m3fn] if name == 'hires64' else [])
+        for dt in dtypes:
+            build_ms = time_launches(lambda: hcorr.AltCorrBlock(fm, radius=r, dtype=dt), 3, 1)
+            blk = hcorr.AltCorrBlock(fm, radius=r, dtype=dt)
+            ms = time_launches(lambda: blk(coords, ii, jj), args.steps, args.warmup)
+            s_in = 1 if dt == torch.float8_e4m3fn else torch.tensor([], dtype=dt).element_size()
+            s_out = 2 if dt == torch.float8_e4m3fn else s_in
+            nbytes = n * h * w * C * s_in * (1 + 1.328) + lookups * L * R * R * s_out
+            flops = lookups * L * D * D * 2 * C
+            dname = str(dt).replace('torch.', '')
+            line = {'metric': f'altcorr lookups/s ({name})', 'value': round(lookups / (ms / 1e3), 1),
+                    'unit': 'lookups/s', 'workload': name, 'dtype': dname, 'edges': E, 'hw': [h, w], 'radius': r,
+                    'path': 'mfma' if getattr(blk, 'mfma', False) else ('fp8 valu' if blk.fp8 else 'per-level'),
+                    'ms_per_pass': round(ms, 4), 'pyramid_build_ms': round(build_ms, 4),
+                    'GBps_algorithmic': round(nbytes / (ms / 1e3) / 1e9, 1),
+                    'hbm_frac': round(nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+                    'TFLOPs_algorithmic': round(flops / (ms / 1e3) / 1e12, 2)}
+            if dname in MFMA_PEAK_TFLOPS:
+                line['mfma_frac'] = round(flops / (ms / 1e3) / 1e12 / MFMA_PEAK_TFLOPS[dname], 4)
+            print(json.dumps(line), flush=True)
+            del blk
+            torch.cuda.empty_cache()
 
 
 def mono_bench(db, p, up, dev, args):

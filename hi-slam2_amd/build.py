@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, 'include')
 ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 ARCH = 'gfx950'
 
-LIB_SOURCES = ['status.cpp', 'ba_plan.cpp', 'ba.hip', 'gn_step.hip', 'projective.hip', 'chol.hip', 'corr_lookup.hip',
+LIB_SOURCES = ['status.cpp', 'ba_plan.cpp', 'ba.hip', 'gn_step.hip', 'projective.hip', 'chol.hip', 'corr_lookup.hip', 'altcorr_pyramid.hip',
                'corr_build.hip', 'altcorr.hip', 'geometry.hip', 'pgba.hip']
 
 HIP_FLAGS = [

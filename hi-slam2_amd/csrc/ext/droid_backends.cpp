@@ -516,6 +516,71 @@ std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, 
   return levels;
 }
 
+// ---- on-the-fly correlation pyramid (hislam2/modules/corr.py:89-117) ------------------------
+// The feature pyramid of AltCorrBlock.__init__, channel-last: level l [B, N, H >> l, W >> l, C] in
+// `dtype` (default: the fmaps' dtype), built once per block.
+std::vector<torch::Tensor> altcorr_pyramid(torch::Tensor fmaps, int num_levels, py::object dtype) {
+  require_device(fmaps, "fmaps");
+  require(fmaps.dim() == 5, "altcorr_pyramid: fmaps must be [B, N, C, H, W]");
+  require(num_levels >= 1 && num_levels <= 4, "altcorr_pyramid: 1 to 4 levels");
+  const torch::ScalarType dt = dtype.is_none() ? fmaps.scalar_type() : torch::python::detail::py_object_to_dtype(dtype);
+  require(dt == torch::kFloat32 || dt == torch::kBFloat16 || dt == torch::kFloat16,
+          "altcorr_pyramid: dtype must be float32, float16 or bfloat16");
+  const int64_t B = fmaps.size(0), N = fmaps.size(1), C = fmaps.size(2), H = fmaps.size(3), W = fmaps.size(4);
+  require((H >> (num_levels - 1)) > 0 && (W >> (num_levels - 1)) > 0 && C % 8 == 0,
+          "altcorr_pyramid: maps too small for the levels, or channels not a multiple of 8");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(fmaps.device());
+  torch::Tensor f = fmaps.to(dt).contiguous();
+  std::vector<torch::Tensor> levels;
+  levels.push_back(torch::empty({B, N, H, W, C}, f.options()));
+  raise_status(hsl2_channels_last(f.data_ptr(), levels[0].data_ptr(), dtype_code(f), B * N, C, H * W,
+                                  current_stream()),
+               "altcorr_pyramid");
+  for (int l = 1; l < num_levels; ++l) {
+    levels.push_back(torch::empty({B, N, H >> l, W >> l, C}, f.options()));
+    raise_status(hsl2_pool_channels_last(levels[0].data_ptr(), levels[l].data_ptr(), dtype_code(f), B * N, H, W, C,
+                                         l, current_stream()),
+                 "altcorr_pyramid");
+  }
+  return levels;
+}
+
+// AltCorrBlock.__call__ in one launch: coords [B, M, H, W, 2] f32 -> [B, M, L (2r+1)^2, H, W]
+torch::Tensor altcorr_pyramid_forward(std::vector<torch::Tensor> levels, torch::Tensor coords, torch::Tensor ii,
+                                      torch::Tensor jj, int radius) {
+  require(!levels.empty() && levels.size() <= 4, "altcorr_pyramid_forward: 1 to 4 levels");
+  const torch::Tensor& l0 = levels[0];
+  require_device(l0, "levels");
+  require(l0.dim() == 5, "altcorr_pyramid_forward: levels must be [B, N, H >> l, W >> l, C]");
+  const int64_t B = l0.size(0), N = l0.size(1), H = l0.size(2), W = l0.size(3), C = l0.size(4);
+  std::vector<const void*> ptrs;
+  for (size_t l = 0; l < levels.size(); ++l) {
+    const torch::Tensor& t = levels[l];
+    CHECK_CONTIGUOUS(t);
+    require(t.scalar_type() == l0.scalar_type() && t.dim() == 5 && t.size(0) == B && t.size(1) == N &&
+                t.size(2) == (H >> l) && t.size(3) == (W >> l) && t.size(4) == C,
+            "altcorr_pyramid_forward: level " + std::to_string(l) + " must be [B, N, H >> l, W >> l, C]");
+    ptrs.push_back(t.data_ptr());
+  }
+  CHECK_CONTIGUOUS(coords);
+  require(coords.scalar_type() == torch::kFloat32 && coords.dim() == 5 && coords.size(0) == B &&
+              coords.size(2) == H && coords.size(3) == W && coords.size(4) == 2,
+          "altcorr_pyramid_forward: coords must be float32 [B, M, H, W, 2]");
+  const int64_t M = coords.size(1);
+  require(ii.numel() == M && jj.numel() == M && ii.scalar_type() == torch::kInt64 &&
+              jj.scalar_type() == torch::kInt64 && ii.is_cuda() && jj.is_cuda(),
+          "altcorr_pyramid_forward: ii, jj must be int64 GPU tensors with one entry per edge");
+  c10::hip::HIPGuardMasqueradingAsCUDA guard(l0.device());
+  torch::Tensor ii_c = ii.contiguous(), jj_c = jj.contiguous();
+  const int64_t R = 2 * radius + 1;
+  torch::Tensor corr = torch::empty({B, M, static_cast<int64_t>(levels.size()) * R * R, H, W}, l0.options());
+  raise_status(hsl2_altcorr_pyramid_forward(ptrs.data(), static_cast<int>(levels.size()), coords.data_ptr<float>(),
+                                            ii_c.data_ptr<int64_t>(), jj_c.data_ptr<int64_t>(), corr.data_ptr(),
+                                            dtype_code(l0), B, N, C, H, W, M, radius, current_stream()),
+               "altcorr_pyramid_forward");
+  return corr;
+}
+
 }  // namespace
 
 // ---- edge-selection and point-cloud geometry (src/droid.cpp:170-241) ----------------------
@@ -1024,6 +1089,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
         py::arg("fmap2"), py::arg("num_levels") = 4, py::arg("out_dtype") = py::none(),
         py::arg("tiled") = false);
+  m.def("altcorr_pyramid", &altcorr_pyramid, "channel-last feature pyramid of AltCorrBlock", py::arg("fmaps"),
+        py::arg("num_levels") = 4, py::arg("dtype") = py::none());
+  m.def("altcorr_pyramid_forward", &altcorr_pyramid_forward,
+        "AltCorrBlock lookup over all levels in one launch on the matrix cores", py::arg("levels"), py::arg("coords"),
+        py::arg("ii"), py::arg("jj"), py::arg("radius") = 3);
   m.def("fp8_quantize", &fp8_quantize, "per-channel-scaled e4m3fn quantisation: [q (channel-last uint8), scales]");
   m.def("altcorr_forward_fp8", &altcorr_forward_fp8, "altcorr lookup on fp8 feature maps", py::arg("q1"),
         py::arg("s1"), py::arg("q2"), py::arg("s2"), py::arg("coords"), py::arg("ii"), py::arg("jj"),

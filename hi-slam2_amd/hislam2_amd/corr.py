@@ -183,8 +183,18 @@ class CorrLayer(torch.autograd.Function):
         return fmap1_grad, fmap2_grad, None, None, None, None
 
 
+_MFMA_DTYPES = (torch.float32, torch.bfloat16, torch.float16)
+
+
 class AltCorrBlock:
     """corr.py:89-117.
+
+    Without autograd, for 128-channel maps, radius 3 or 4 and up to 4 levels (the reference's
+    configuration), the feature pyramid is built once, channel-last (``altcorr_pyramid``), and
+    every call runs all levels in one launch on the matrix cores (``altcorr_pyramid_forward``),
+    writing the stacked [B, M, L (2r+1)^2, H, W] output directly.  ``dtype`` selects the element
+    type of that pyramid and of the output (default: the maps' dtype): bfloat16 / float16 /
+    float32.  Otherwise the reference's per-level ``CorrLayer`` path runs (with autograd).
 
     ``dtype=torch.float8_e4m3fn`` (an addition of this build for high-resolution frames) keeps
     every pyramid level as e4m3fn codes with per-channel FP32 scales (``fp8_quantize``, the
@@ -196,6 +206,15 @@ class AltCorrBlock:
         self.radius = radius
         self.fp8 = dtype == torch.float8_e4m3fn
         B, N, C, H, W = fmaps.shape
+        grad = torch.is_grad_enabled() and fmaps.requires_grad
+        dt = fmaps.dtype if dtype is None else dtype
+        self.mfma = (not self.fp8 and not grad and C == 128 and radius in (3, 4) and 1 <= num_levels <= 4
+                     and dt in _MFMA_DTYPES and fmaps.dtype in _MFMA_DTYPES)
+        if self.mfma:
+            self.levels = _db.altcorr_pyramid(fmaps.detach(), num_levels, dt)
+            return
+        if dtype is not None and not self.fp8:
+            fmaps = fmaps.to(dtype)
         if self.fp8:
             if torch.is_grad_enabled() and fmaps.requires_grad:
                 raise ValueError('AltCorrBlock: the fp8 path has no autograd')
@@ -214,6 +233,9 @@ class AltCorrBlock:
             fmaps = F.avg_pool2d(fmaps, 2, stride=2)
 
     def __call__(self, coords, ii, jj):
+        if self.mfma:
+            return _db.altcorr_pyramid_forward(self.levels, coords.detach().float().contiguous(), ii, jj,
+                                               self.radius)
         coords = coords.permute(0, 1, 4, 2, 3).contiguous()
         corr_list = []
         for i in range(self.num_levels):

@@ -147,6 +147,29 @@ int hsl2_altcorr_backward(const void* fmap1, const void* fmap2, const float* coo
                           int64_t B, int64_t N1, int64_t N2, int64_t C, int64_t H, int64_t W,
                           int64_t H2, int64_t W2, int64_t M, int radius, void* stream);
 
+/* Layout helper for the on-the-fly correlation pyramid (AltCorrBlock.__init__,
+ * hislam2/modules/corr.py:92-104, which pools the feature maps with F.avg_pool2d level by
+ * level): out [frames, h >> level, w >> level, channels] = the mean of each 2^level x 2^level
+ * block of the channel-last level-0 maps level0 [frames, h, w, channels] (F32/F16/BF16),
+ * summed in FP32 and rounded once.  channels a multiple of 8, buffers 16-byte aligned. */
+int hsl2_pool_channels_last(const void* level0, void* out, int dtype, int64_t frames, int64_t h,
+                            int64_t w, int64_t channels, int level, void* stream);
+
+/* Replaces AltCorrBlock.__call__ (hislam2/modules/corr.py:107-117: one altcorr_forward per
+ * level, src/altcorr_kernel.cu:24-75 + 132-172, on coords / 2^level, then stack + flatten)
+ * with one launch on the matrix cores.  levels[l] = level l of the CHANNEL-LAST feature
+ * pyramid [B, N, H >> l, W >> l, C] (level 0 from hsl2_channels_last, the others from
+ * hsl2_pool_channels_last) of `dtype` (BF16, F16 or F32; the dot products run on MFMA with FP32
+ * accumulation); fmap1 is level 0 of frame ii[m], fmap2 level l of frame jj[m] (an edge naming
+ * a frame outside [0, N) yields zeros).  coords [B, M, H, W, 2] f32 (x, y interleaved: the
+ * AltCorrBlock call layout, without its permute), ii / jj [M] int64 on the device, corr
+ * [B, M, num_levels (2r+1)^2, H, W] of `dtype`, channel order [level][x-offset][y-offset],
+ * fully written.  C must be 128, radius 3 or 4, num_levels 1 to 4. */
+int hsl2_altcorr_pyramid_forward(const void* const* levels, int num_levels, const float* coords,
+                                 const int64_t* ii, const int64_t* jj, void* corr, int dtype,
+                                 int64_t B, int64_t N, int64_t C, int64_t H, int64_t W, int64_t M,
+                                 int radius, void* stream);
+
 /* FP8 (OCP e4m3fn) feature maps with per-channel FP32 scales, for the on-the-fly correlation
  * of high-resolution frames (no counterpart in the reference, which runs altcorr in the fmap
  * dtype).  hsl2_fp8_quantize: x is `vectors` channel-last vectors of `channels` values (F32 or
