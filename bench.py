@@ -39,12 +39,15 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
-    ap.add_argument('--only', choices=['all', 'lookup', 'altcorr', 'mono', 'geometry', 'pgba'], default='all',
+    ap.add_argument('--only', choices=['all', 'lookup', 'altcorr', 'window', 'hires64', 'mono', 'geometry', 'pgba'],
+                    default='all',
                     help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
                          'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph; '
                          'geometry: time frame_distance, depth_filter and iproj on it; '
                          'pgba: time the Sim3 pose-graph BA step on a 300-keyframe loop-closure graph; '
-                         'altcorr: time the on-the-fly correlation (AltCorrBlock) on global300 and hires64')
+                         'altcorr: time the on-the-fly correlation (AltCorrBlock) on global300 and hires64; '
+                         'window: latency of one frontend-window ba call (12 keyframes, 70 edges, 4 iterations); '
+                         'hires64: 96x128 BA (64 keyframes, 800 edges, 4 iterations) + radius-4 on-the-fly lookup')
     return ap.parse_args()
 
 
@@ -71,6 +74,12 @@ def main():
         return
     if args.only == 'altcorr':
         altcorr_bench(dev, args)
+        return
+    if args.only == 'window':
+        window_bench(db, dev, args)
+        return
+    if args.only == 'hires64':
+        hires64_bench(db, dev, args)
         return
     log('generating global300')
     p = synth.ba_global300()
@@ -156,7 +165,7 @@ def main():
             return db.ba(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, iters, p.lm, p.ep, False)
         return ba_call(poses, disps, precision)
 
-    # ---- accuracy: both paths from the same start -------------------------------------
+    # ---- accuracy: both paths from the same start, against the float64 oracle ----------
     res = {}
     for name, prec in (('split', 0), ('plain', 1)):
         if name == 'plain' and args.no_plain:
@@ -165,11 +174,12 @@ def main():
         out = ba_call(P_, D_, prec)
         torch.cuda.synchronize()
         res[name] = dict(ok=int(out[3].item()),
-                         rms=reprojection_rms(P_, D_, intr, tg, wt, ii, jj))
+                         rms=reprojection_rms(P_, D_, intr, tg, wt, ii, jj),
+                         oracle=ba_oracle_errors(P_, D_, out[0], out[1]))
     rms0 = reprojection_rms(poses0, disps0, intr, tg, wt, ii, jj)
     log('reprojection rms start', rms0, res)
 
-    def time_path(dtype, precision):
+    def time_path(dtype, precision, check_lookup=False):
         f = fm32.to(dtype)
         f1, f2 = f[:, p.ii], f[:, p.jj]
         del f
@@ -207,6 +217,7 @@ def main():
         torch.cuda.synchronize()
         t_total = time.perf_counter() - t0
         lookup_ms = [a.elapsed_time(b) for a, b in ev]
+        lookup_err = lookup_oracle_errors(blk(coords)) if check_lookup else None
         # BA alone
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -216,10 +227,11 @@ def main():
         t_ba = time.perf_counter() - t1
         del blk
         torch.cuda.empty_cache()
-        return dict(t_total=t_total, t_ba=t_ba, lookup_ms=float(np.mean(lookup_ms)), build_ms=build_ms)
+        return dict(t_total=t_total, t_ba=t_ba, lookup_ms=float(np.mean(lookup_ms)), build_ms=build_ms,
+                    lookup_err=lookup_err)
 
     log('timing split path (bf16 volume, FP64 solve)')
-    split = time_path(torch.bfloat16, 0)
+    split = time_path(torch.bfloat16, 0, check_lookup=True)
     plain = None
     if not args.no_plain:
         log('timing plain path (fp32 volume, FP32 solve)')
@@ -233,6 +245,21 @@ def main():
     lookup_s = split['lookup_ms'] / 1e3
     achieved = LOOKUP_BYTES_BF16 * lookups / lookup_s / 1e9
     traffic = lookup_traffic()
+
+    # the on-the-fly correlation (AltCorrBlock, the reference's global-BA path) in both
+    # precisions: the same MFMA kernel with bf16 and with fp32 operands
+    onthefly = None
+    if not args.no_plain:
+        onthefly = {}
+        for dname, dt, key in (('bf16', torch.bfloat16, 'split'), ('fp32', torch.float32, 'plain')):
+            ablk = hcorr.AltCorrBlock(fm32, dtype=dt)
+            ms = time_launches(lambda: ablk(coords, ii, jj), 5, 1)
+            t_ba = (split if key == 'split' else plain)['t_ba'] / args.steps
+            onthefly[dname] = {'lookup_ms': round(ms, 4),
+                               'GN_it_s': round(iters / (t_ba + iters * ms / 1e3), 3)}
+            del ablk
+            torch.cuda.empty_cache()
+        onthefly['speedup_vs_plain'] = round(onthefly['bf16']['GN_it_s'] / onthefly['fp32']['GN_it_s'], 3)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -262,6 +289,8 @@ def main():
         'lookup_GBps_algorithmic': round(achieved, 1),
         'reproj_rms_start_px': round(rms0, 6),
         'reproj_rms_split_px': round(res['split']['rms'], 6),
+        'oracle_err_split': res['split']['oracle'],
+        'oracle_err_lookup_bf16': split['lookup_err'],
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': traffic,
                      'kernel': 'corr_lookup_vec_kernel<bf16> (4 levels, r=3, 9.216M lookups/launch, '
@@ -277,6 +306,11 @@ def main():
             'speedup_vs_plain': round(value / pv, 3),
             'reproj_rms_plain_px': round(res['plain']['rms'], 6),
             'reproj_diff_px': round(abs(res['split']['rms'] - res['plain']['rms']), 7),
+            'oracle_err_plain': res['plain']['oracle'],
+            'target_2x_met': bool(value / pv >= 2.0 and abs(res['split']['rms'] - res['plain']['rms']) <= 1e-3),
+            'plain_path': 'the same kernels instantiated in FP32 (per-edge Hessian blocks, frame sums, Schur '
+                          'Gram totals, reduced Cholesky, back-substitution) and the fp32 stored-volume lookup',
+            'onthefly_corr': onthefly,
         })
     print(json.dumps(line), flush=True)
 
@@ -344,6 +378,101 @@ def altcorr_bench(dev, args):
             print(json.dumps(line), flush=True)
             del blk
             torch.cuda.empty_cache()
+
+
+def window_bench(db, dev, args):
+    """The frontend window (12 keyframes at 48x64, 60 neighbour + 10 long edges, 4 GN iterations,
+    t0 = 2): the reference's most frequent ba call (hislam2/factor_graph.py:189-233), latency-bound.
+    Per call through droid_backends.ba exactly as the reference calls it: host copy of ii/jj,
+    planning (cached while the edge list repeats), the iterations on the device.  Reported: the
+    median wall-clock latency per call with a synchronisation after it, the device time per call
+    (HIP events around back-to-back calls), and both precision modes."""
+    import numpy as np
+    import torch
+    from hislam2_amd import synth
+    from hislam2_amd.metrics import reprojection_rms
+    p = synth.ba_window()
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    intr, tg, wt, eta, ii, jj = up(p.intrinsics), up(p.targets), up(p.weights), up(p.eta), up(p.ii), up(p.jj)
+    line = {'metric': 'frontend-window ba latency', 'workload': 'window: 12 keyframes 48x64, 70 edges, '
+            'ba(4 iterations, t0=2, t1=12)', 'edges': len(p.ii), 'iterations': p.iterations}
+    rms0 = reprojection_rms(up(p.poses), up(p.disps), intr, tg, wt, ii, jj)
+    for name, prec in (('split', 0), ('plain', 1)):
+        poses, disps = up(p.poses), up(p.disps)
+
+        def call():
+            if prec == 0:
+                return db.ba(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, p.iterations, p.lm, p.ep, False)
+            return db.ba_ex(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, p.iterations, p.lm, p.ep, False,
+                            prec)
+
+        P_, D_ = up(p.poses), up(p.disps)
+        db.ba_ex(P_, D_, intr, tg, wt, eta, ii, jj, p.t0, p.t1, p.iterations, p.lm, p.ep, False, prec)
+        rms = reprojection_rms(P_, D_, intr, tg, wt, ii, jj)
+        for _ in range(max(args.warmup, 3)):
+            call()
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(20 * args.steps):
+            t0 = time.perf_counter()
+            call()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        dev_ms = time_launches(call, 20 * args.steps, 1)
+        line[name] = {'latency_ms_median': round(1e3 * float(np.median(lat)), 4),
+                      'device_ms_per_call': round(dev_ms, 4),
+                      'GN_it_s_device': round(p.iterations / (dev_ms / 1e3), 1),
+                      'reproj_rms_px': round(rms, 6)}
+    line['reproj_rms_start_px'] = round(rms0, 6)
+    line['speedup_vs_plain_device'] = round(line['plain']['device_ms_per_call'] / line['split']['device_ms_per_call'], 3)
+    print(json.dumps(line), flush=True)
+
+
+def hires64_bench(db, dev, args):
+    """The high-resolution workload: 64 keyframes at 96x128 (HW = 12288), 800 edges, ba of 4 GN
+    iterations (t0 = 1) plus, per iteration, the radius-4 4-level on-the-fly lookup of all 800
+    edges (AltCorrBlock; a stored volume would need 400 MB per edge).  A step = that ba call + 4
+    lookups.  Split: bf16 lookup on the matrix cores + FP64 sums and solve; plain: the same
+    kernels in FP32.  Also the fp8 (e4m3, per-channel scales) lookup."""
+    import numpy as np
+    import torch
+    from hislam2_amd import corr as hcorr
+    from hislam2_amd import synth
+    from hislam2_amd.metrics import reprojection_rms
+    p = synth.ba_hires64()
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    intr, tg, wt, eta, ii, jj = up(p.intrinsics), up(p.targets), up(p.weights), up(p.eta), up(p.ii), up(p.jj)
+    n, h, w = p.disps.shape
+    fm = up(synth.fmaps(64, n, h=h, w=w))
+    coords = up(synth.lookup_coords(65, len(p.ii), h=h, w=w))
+    line = {'metric': 'hires64 GN it/s (ba + on-the-fly lookup)', 'workload': 'hires64: 64 keyframes 96x128, '
+            '800 edges, ba(4 iterations, t0=1) + radius-4 4-level AltCorrBlock lookup of all edges per iteration',
+            'edges': len(p.ii), 'iterations': p.iterations}
+    rms0 = reprojection_rms(up(p.poses), up(p.disps), intr, tg, wt, ii, jj)
+    for name, prec, dt in (('split', 0, torch.bfloat16), ('plain', 1, torch.float32)):
+        P_, D_ = up(p.poses), up(p.disps)
+        out = db.ba_ex(P_, D_, intr, tg, wt, eta, ii, jj, p.t0, p.t1, p.iterations, p.lm, p.ep, False, prec)
+        rms = reprojection_rms(P_, D_, intr, tg, wt, ii, jj)
+        blk = hcorr.AltCorrBlock(fm, radius=4, dtype=dt)
+        poses, disps = up(p.poses), up(p.disps)
+
+        def step():
+            db.ba_ex(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, p.iterations, p.lm, p.ep, False, prec)
+            for _ in range(p.iterations):
+                blk(coords, ii, jj)
+
+        ms = time_launches(step, args.steps, args.warmup)
+        look = time_launches(lambda: blk(coords, ii, jj), args.steps, 1)
+        line[name] = {'GN_it_s': round(p.iterations / (ms / 1e3), 3), 'ms_per_step': round(ms, 3),
+                      'lookup_ms': round(look, 4), 'solve_ok': int(out[3].item()), 'reproj_rms_px': round(rms, 6)}
+        del blk
+        torch.cuda.empty_cache()
+    f8 = hcorr.AltCorrBlock(fm, radius=4, dtype=torch.float8_e4m3fn)
+    line['fp8_lookup_ms'] = round(time_launches(lambda: f8(coords, ii, jj), args.steps, 1), 4)
+    line['reproj_rms_start_px'] = round(rms0, 6)
+    line['speedup_vs_plain'] = round(line['split']['GN_it_s'] / line['plain']['GN_it_s'], 3)
+    line['reproj_diff_px'] = round(abs(line['split']['reproj_rms_px'] - line['plain']['reproj_rms_px']), 7)
+    print(json.dumps(line), flush=True)
 
 
 def mono_bench(db, p, up, dev, args):
@@ -461,6 +590,44 @@ def geometry_bench(db, p, up, args):
     print(json.dumps({'metric': 'edge-selection geometry on global300', **res, 'edges': len(p.ii), 'frames': n,
                       'note': 'frame_distance_ms and depth_filter_ms include the host-side index checks'}),
           flush=True)
+
+
+GOLD = os.path.join(ROOT, 'tests', 'golden')
+
+
+def ba_oracle_errors(poses, disps, dx, dz):
+    """Errors of the 8-iteration global300 ba call against the float64 oracle's result stored in
+    tests/golden/ba_global300_8_iterations.npz: final poses and disparities relative to their
+    largest entry; the last dx and dz entry by entry relative to their own magnitude plus the
+    first iteration's largest update."""
+    import numpy as np
+    path = os.path.join(GOLD, 'ba_global300_8_iterations.npz')
+    if not os.path.exists(path):
+        return None
+    f = np.load(path)
+    rel = lambda g, r: float(np.max(np.abs(g - r)) / np.max(np.abs(r)))
+    loc = lambda g, r, fl: float(np.max(np.abs(g - r) / (np.abs(r) + fl)))
+    return {'poses': rel(poses.double().cpu().numpy(), f['out_poses']),
+            'disps': rel(disps.double().cpu().numpy().reshape(-1)[f['disps_index']], f['out_disps']),
+            'dx_last': loc(dx.double().cpu().numpy(), f['out_dx'], np.max(np.abs(f['out_dx_it1']))),
+            'dz_last': loc(dz.double().cpu().numpy().reshape(-1)[f['dz_index']], f['out_dz'], float(f['dz1_absmax']))}
+
+
+def lookup_oracle_errors(out):
+    """Max-norm relative and absolute error of a [1, 3000, 196, 48, 64] global300 lookup at the
+    edges and pixels of tests/golden/corrblock_global300_bf16.npz (float64 oracle)."""
+    import numpy as np
+    import torch
+    path = os.path.join(GOLD, 'corrblock_global300_bf16.npz')
+    if not os.path.exists(path):
+        return None
+    f = np.load(path)
+    e = torch.from_numpy(f['edges']).to(out.device)
+    o = out[0].index_select(0, e).flatten(2)
+    px = torch.from_numpy(f['pixels']).to(out.device)
+    got = torch.gather(o, 2, px[:, None, :].expand(-1, o.shape[1], -1)).double().cpu().numpy()
+    d = np.abs(got - f['out'])
+    return {'rel': float(d.max() / np.abs(f['out']).max()), 'abs': float(d.max())}
 
 
 def lookup_traffic():

@@ -10,12 +10,13 @@
 //                        Ei of the frame's edges (accum_cuda, 1413-1463, 1901-1906).
 //   3. ba_schur_gram_lds per depth frame, 16x16 tiles of the slots' Gram matrix
 //                        S = sum_px E_a Q E_b^T on the matrix cores, one block per group
-//                        pair of row tiles staged in LDS by 64-pixel chunks (FP32 tile
-//                        sums per 256 pixels, FP64 totals); the blocks of diagonal
+//                        pair of row tiles staged in LDS by 32-pixel chunks (FP32 tile
+//                        sums per 256 pixels, totals in the solve type); the blocks of diagonal
 //                        groups also form Ev = sum_px Q w E (EEt6x6 / Ev6x1, 1466-1560)
 //                        (ba_schur_ev, one wave per slot, when motion-only).
 //   4. ba_assemble       the damped reduced camera system A - S, b - bS in FP64
-//                        (FP32 in the plain mode), written densely in column-major
+//                        (FP32 in the all-FP32 mode, whose per-edge Hessian blocks, frame
+//                        sums and Gram totals are FP32 as well), written densely in column-major
 //                        order, padded to a multiple of 64 with an identity block
 //                        (SparseBlock, 1584-1686).
 //   5. cholesky_solve    tiled Cholesky factorisation and solve (chol.hip); a non-positive
@@ -45,10 +46,11 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
   float* bz = reinterpret_cast<float*>(ws + p->w_bz);
   float* Q = reinterpret_cast<float*>(ws + p->w_Q);
   float* wv = reinterpret_cast<float*>(ws + p->w_w);
-  double* Hs = reinterpret_cast<double*>(ws + p->w_Hs);
-  double* vs = reinterpret_cast<double*>(ws + p->w_vs);
-  double* S = reinterpret_cast<double*>(ws + p->w_S);
-  double* Ev = reinterpret_cast<double*>(ws + p->w_Ev);
+  // sums in the solve type: FP64 in the split mode, FP32 in the all-FP32 mode
+  R* Hs = reinterpret_cast<R*>(ws + p->w_Hs);
+  R* vs = reinterpret_cast<R*>(ws + p->w_vs);
+  R* S = reinterpret_cast<R*>(ws + p->w_S);
+  R* Ev = reinterpret_cast<R*>(ws + p->w_Ev);
   R* A = reinterpret_cast<R*>(ws + p->w_A);
   R* bvec = reinterpret_cast<R*>(ws + p->w_b);
   R* xvec = reinterpret_cast<R*>(ws + p->w_x);
@@ -66,17 +68,17 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
     const bool last = it == iterations - 1;
     if (E > 0) {
       if (schur)
-        hipLaunchKernelGGL((ba_linearize<true, kSemCuda>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+        hipLaunchKernelGGL((ba_linearize<true, kSemCuda, R>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
                            Hs, vs, Eall + (int64_t)P * 6 * HW, Eii, Cii, bz);
       else
-        hipLaunchKernelGGL((ba_linearize<false, kSemCuda>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+        hipLaunchKernelGGL((ba_linearize<false, kSemCuda, R>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
                            Hs, vs, nullptr, nullptr, nullptr, nullptr);
       HSL2_LAUNCH_CHECK();
     }
     if (schur) {
-      launch_frame_reduce(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
+      launch_frame_reduce<R>(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
                           blob + p->o_frame_pose, blob + p->o_frame_order, HW, static_cast<int>(F), 0.f, Q, wv,
                           Eall, s);
       HSL2_LAUNCH_CHECK();

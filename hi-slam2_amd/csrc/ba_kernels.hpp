@@ -128,16 +128,16 @@ __device__ __forceinline__ bool pixel_rows(const EdgeGeom& g, float u, float v, 
 __device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
 
 // ---- 1. per-edge linearization ----------------------------------------------
-template <bool kSchur, int kSem>
+template <bool kSchur, int kSem, typename Acc = double>
 __global__ void __launch_bounds__(kLinThreads)
 ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
              const float* __restrict__ intr, int intr_stride, const float* __restrict__ targets,
              const float* __restrict__ weights, const int* __restrict__ edge_ii,
              const int* __restrict__ edge_jj, int HW, int wd, int E,
-             double* __restrict__ Hs, double* __restrict__ vs, float* __restrict__ Eij,
+             Acc* __restrict__ Hs, Acc* __restrict__ vs, float* __restrict__ Eij,
              float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz) {
-  __shared__ double lds[16 * kLinThreads];
-  __shared__ double red[90];
+  __shared__ Acc lds[16 * kLinThreads];
+  __shared__ Acc red[90];
   const int e = blockIdx.x;
   const int ix = edge_ii[e], jx = edge_jj[e];
   const bool self = ix == jx;
@@ -145,7 +145,8 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   load_edge_geom(poses, intr, intr_stride, ix, jx, g);
 
   // Per-pixel products and each thread's partial sums (HW / 256 pixels, 12 at 48x64) are
-  // formed in FP32; the block reduction of the partial sums runs in FP64.
+  // formed in FP32; the block reduction of the partial sums runs in Acc (FP64 in the split
+  // mode, FP32 in the all-FP32 mode).
   float acc[90];
 #pragma unroll
   for (int k = 0; k < 90; ++k) acc[k] = 0.f;
@@ -211,7 +212,7 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
     }
   }
 
-  block_reduce_store<90, kLinThreads, double>(acc, lds, red);
+  block_reduce_store<90, kLinThreads, Acc>(acc, lds, red);
   __syncthreads();
   const int t = threadIdx.x;
   if (t < 144) {
@@ -229,6 +230,8 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
 }
 
 // ---- 2. per-frame sums: C, w, Q and the pose-i Schur block Ei ----------------
+// Sums in Acc (FP64 in the split mode, FP32 in the all-FP32 mode).
+template <typename Acc>
 __global__ void __launch_bounds__(256)
 ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
                 const float* __restrict__ Eii, const float* __restrict__ eta, int eta_rows,
@@ -239,7 +242,7 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= HW) return;
   const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
-  double c = 0.0, b = 0.0;
+  Acc c = 0, b = 0;
   int q = e0;
   for (; q + 4 <= e1; q += 4) {                  // four edges' loads in flight together
     float cv[4], bv[4];
@@ -261,11 +264,11 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
     b += bz[(int64_t)e * HW + k];
   }
   const float et = eta[(int64_t)(eta_rows == 1 ? 0 : f) * HW + k];
-  Q[(int64_t)f * HW + k] = static_cast<float>(1.0 / (c + et + c_eps));
+  Q[(int64_t)f * HW + k] = static_cast<float>(Acc(1) / (c + et + c_eps));
   w[(int64_t)f * HW + k] = static_cast<float>(b);
   const int p = frame_pose[f];
   if (p >= 0) {
-    double ei[6] = {0, 0, 0, 0, 0, 0};
+    Acc ei[6] = {0, 0, 0, 0, 0, 0};
     int q2 = e0;
     for (; q2 + 2 <= e1; q2 += 2) {
       float v[2][6];
@@ -290,22 +293,24 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
   }
 }
 
+template <typename Acc = double>
 inline void launch_frame_reduce(const float* Cii, const float* bz, const float* Eii, const float* eta,
                                 int eta_rows, const int* frame_eptr, const int* frame_edges,
                                 const int* frame_pose, const int* frame_order, int HW, int F, float c_eps,
                                 float* Q, float* w, float* Eall, hipStream_t s) {
-  hipLaunchKernelGGL(ba_frame_reduce, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii, eta, eta_rows,
+  hipLaunchKernelGGL(ba_frame_reduce<Acc>, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii, eta, eta_rows,
                      frame_eptr, frame_edges, frame_pose, frame_order, HW, c_eps, Q, w, Eall);
 }
 
 // ---- 3. Ev rows, one wave per slot -------------------------------------------
 // Ev[s] = sum_px Q w E_s for the Ev tasks [task0, NT) of the plan's task list (task0 = NPAIR:
 // the pair blocks come from ba_schur_gram_lds).
+template <typename Acc>
 __global__ void __launch_bounds__(256)
 ba_schur_ev(const float* __restrict__ Eall, const float* __restrict__ Q,
             const float* __restrict__ w, const int* __restrict__ tasks,
             const int* __restrict__ slot_row, int task0, int NT, int HW, int task_blocks8,
-            double* __restrict__ Ev) {
+            Acc* __restrict__ Ev) {
   // XCD-aware order: the 8 XCDs receive blocks round-robin, so give each XCD a
   // contiguous run of tasks (tasks of one depth frame share Q and w in L2).
   const int b = blockIdx.x;
@@ -318,15 +323,15 @@ ba_schur_ev(const float* __restrict__ Eall, const float* __restrict__ Q,
   const float* Qf = Q + (int64_t)f * HW;
   const float* Ea = Eall + (int64_t)slot_row[sa] * 6 * HW;
   const float* wf = w + (int64_t)f * HW;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  Acc acc[6] = {0, 0, 0, 0, 0, 0};
   for (int k = lane; k < HW; k += kWave) {
-    const double qw = static_cast<double>(Qf[k]) * wf[k];
+    const Acc qw = static_cast<Acc>(Qf[k]) * wf[k];
 #pragma unroll
     for (int n = 0; n < 6; ++n) acc[n] += qw * Ea[n * HW + k];
   }
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
-    const double v = wave_sum(acc[n]);
+    const Acc v = wave_sum(acc[n]);
     if (lane == n) Ev[(int64_t)sa * 6 + n] = v;
   }
 }
@@ -355,7 +360,7 @@ __device__ __forceinline__ float4 gram_load4(const float* __restrict__ row, int 
 // wave p % 4; step s of a chunk feeds lane l pixel 4 s + l / 16 of its A row (scaled by Q) and
 // B row.  FP32 tile sums are folded into FP64 every 256 pixels.  Only entries with a <= b are
 // written; a diagonal block (a == b) is written from its entries r <= c, mirrored, so it is
-// exactly symmetric.  A diagonal item (ga == gb) also forms the Ev rows of its group's rows,
+// exactly symmetric.  (Acc = float, the all-FP32 mode, folds into FP32.)  A diagonal item (ga == gb) also forms the Ev rows of its group's rows,
 // Ev = sum_px Q w E (EEt6x6 / Ev6x1's Ev, src/droid_kernels.cu:1466-1560), from the staged
 // chunks: each thread 16 pixels of one row per chunk in FP32, FP64 across chunks and threads.
 constexpr int kGramKC = 32;                          // pixels per staged chunk
@@ -363,15 +368,16 @@ constexpr int kGramPT = kGramKC / 2;                 // pixels of one row staged
 constexpr int kGramFold = 256 / kGramKC;             // chunks per FP64 fold
 constexpr int kGramRS = 2 * 16 * kGramGroup + 16;   // LDS row stride (floats), 16 mod 64
 
+template <typename Acc>
 __global__ void __launch_bounds__(256, 3)
 ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
                   const int* __restrict__ items, int n_items, int item_blocks8,
                   const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
                   const int* __restrict__ frame_pair0, int HW, const float* __restrict__ wvec,
-                  double* __restrict__ S, double* __restrict__ Ev) {
+                  Acc* __restrict__ S, Acc* __restrict__ Ev) {
   __shared__ float Ls[2][kGramKC][kGramRS];
   __shared__ float qs[2][kGramKC], qws[2][kGramKC];
-  __shared__ double evs[4][16 * kGramGroup];
+  __shared__ Acc evs[4][16 * kGramGroup];
   const int blk = blockIdx.x;
   const int item = (blk % 8) * item_blocks8 + blk / 8;  // XCD-aware: runs of consecutive items
   if (item >= n_items) return;
@@ -424,7 +430,7 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
   // (pixels kGramKC / 4 * (t / 64) + [0, kGramKC / 4))
   const int er = t % 64, eq = t / 64;
   const bool ev_row = diag && er < na && 16 * a0 + er < R;
-  double evd = 0.0;
+  Acc evd = 0;
   auto load = [&](Stage& g, int k0) {
     const int kk = k0 + kGramPT * h;
 #pragma unroll
@@ -473,7 +479,7 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
       qws[buf][4 * t + 3] = g.q.w * g.w.w;
     }
   };
-  double dacc[4][4];
+  Acc dacc[4][4];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -512,13 +518,13 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
 #pragma unroll
         for (int j = 0; j < kGramKC / 4; ++j)
           e = fmaf(Ls[buf][kGramKC / 4 * eq + j][er], qws[buf][kGramKC / 4 * eq + j], e);
-        evd += static_cast<double>(e);
+        evd += static_cast<Acc>(e);
       }
       if (c % kGramFold == kGramFold - 1 || c + 1 == nch) {   // every 256 pixels
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<double>(acc[u][i]);
+          for (int i = 0; i < 4; ++i) dacc[u][i] += static_cast<Acc>(acc[u][i]);
           acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
         }
       }
@@ -564,25 +570,27 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
 }
 
 // The Ev rows with the weight vector wvec (one wave per slot).
+template <typename Acc>
 inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
-                            const float* wvec, double* Ev, hipStream_t s) {
+                            const float* wvec, Acc* Ev, hipStream_t s) {
   const int task0 = static_cast<int>(p->NPAIR);
   const int nt = static_cast<int>(p->NT) - task0;
   if (nt <= 0) return;
   const int tb8 = ((nt + 3) / 4 + 7) / 8;
-  hipLaunchKernelGGL(ba_schur_ev, dim3(tb8 * 8), dim3(256), 0, s, Eall, Q, wvec, blob + p->o_task,
+  hipLaunchKernelGGL(ba_schur_ev<Acc>, dim3(tb8 * 8), dim3(256), 0, s, Eall, Q, wvec, blob + p->o_task,
                      blob + p->o_slot_row, task0, static_cast<int>(p->NT), static_cast<int>(p->HW), tb8, Ev);
 }
 
 // Schur pair blocks and Ev rows (both from the Gram kernel; Ev alone when there are no
 // Gram items, i.e. motion-only).
+template <typename Acc>
 inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
-                              const float* wvec, double* S, double* Ev, hipStream_t s) {
+                              const float* wvec, Acc* S, Acc* Ev, hipStream_t s) {
   const int HW = static_cast<int>(p->HW);
   if (p->NGGI > 0) {
     const int ni = static_cast<int>(p->NGGI);
     const int ib8 = (ni + 7) / 8;
-    hipLaunchKernelGGL(ba_schur_gram_lds, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_group_item,
+    hipLaunchKernelGGL(ba_schur_gram_lds<Acc>, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_group_item,
                        ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, wvec,
                        S, Ev);
   } else {
@@ -593,7 +601,7 @@ inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const floa
 // ---- 4. reduced camera system, dense column-major ----------------------------
 template <typename R>
 __global__ void __launch_bounds__(256)
-ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
+ba_assemble(const R* __restrict__ Hs, const R* __restrict__ S,
             const int* __restrict__ blk_ptr, const int* __restrict__ blk_src, int E, int P,
             int64_t n_pad, float lm, float ep, int damp_on_H, R* __restrict__ A,
             int* __restrict__ info) {
@@ -610,7 +618,7 @@ ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
   const int a = static_cast<int>(row / 6), i = static_cast<int>(row % 6);
   const int b = static_cast<int>(col / 6), j = static_cast<int>(col % 6);
   const int64_t blk = (int64_t)a * P + b;
-  double vh = 0, vs = 0;
+  R vh = 0, vs = 0;
   // contributions four at a time (their code and value loads in flight together), summed in
   // list order
   auto src = [&](int code) {
@@ -623,7 +631,7 @@ ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
   int q = blk_ptr[blk];
   for (; q + 4 <= q1; q += 4) {
     int code[4];
-    double val[4];
+    R val[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) code[u] = blk_src[q + u];
 #pragma unroll
@@ -636,23 +644,23 @@ ba_assemble(const double* __restrict__ Hs, const double* __restrict__ S,
   }
   for (; q < q1; ++q) {
     const int code = blk_src[q];
-    const double val = *src(code);
+    const R val = *src(code);
     if ((code & 7) < 4) vh += val;
     else vs += val;
   }
-  double v = vh - vs;
+  R v = vh - vs;
   // damp_on_H = 0: ba_cuda damps the reduced diagonal (src/droid_kernels.cu:1663-1664);
   // 1: the torch formulation's schur_solve damps H's diagonal before the Schur complement
   // (hislam2/geom/chol.py:56-57); 2: its block_solve (chol.py:36-37) adds (ep + lm H) * eye(6)
   // to every 6x6 block [P, P, 6, 6], off-diagonal blocks included.
   const bool damped = damp_on_H == 2 ? (i == j) : (row == col);
-  if (damped) v += static_cast<double>(ep) + static_cast<double>(lm) * (damp_on_H ? vh : v);
-  A[idx] = static_cast<R>(v);
+  if (damped) v += static_cast<R>(ep) + static_cast<R>(lm) * (damp_on_H ? vh : v);
+  A[idx] = v;
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256)
-ba_assemble_rhs(const double* __restrict__ vs, const double* __restrict__ Ev,
+ba_assemble_rhs(const R* __restrict__ vs, const R* __restrict__ Ev,
                 const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
                 int n_pad, R* __restrict__ bvec) {
   // one wave per pose (six rows): lanes take the pose's contributions in turn, each lane
@@ -660,23 +668,23 @@ ba_assemble_rhs(const double* __restrict__ vs, const double* __restrict__ Ev,
   const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   if (6 * a >= n_pad) return;
-  double v[6] = {0, 0, 0, 0, 0, 0};
+  R v[6] = {0, 0, 0, 0, 0, 0};
   if (a < P) {
     for (int q = rhs_ptr[a] + lane; q < rhs_ptr[a + 1]; q += kWave) {
       const int code = rhs_src[q];
       const int kind = code & 3;
       const int64_t id = code >> 2;
-      const double* src = kind == kRhsVi ? vs + id * 6 : (kind == kRhsVj ? vs + ((int64_t)E + id) * 6 : Ev + id * 6);
-      const double sg = kind == kRhsEv ? -1.0 : 1.0;
+      const R* src = kind == kRhsVi ? vs + id * 6 : (kind == kRhsVj ? vs + ((int64_t)E + id) * 6 : Ev + id * 6);
+      const R sg = kind == kRhsEv ? R(-1) : R(1);
 #pragma unroll
       for (int i = 0; i < 6; ++i) v[i] += sg * src[i];
     }
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    const double tot = wave_sum(v[i]);
+    const R tot = wave_sum(v[i]);
     const int row = 6 * a + i;
-    if (lane == i && row < n_pad) bvec[row] = static_cast<R>(row < 6 * P ? tot : 0.0);
+    if (lane == i && row < n_pad) bvec[row] = row < 6 * P ? tot : R(0);
   }
 }
 

@@ -44,7 +44,8 @@ enum {
 /* ---- BA precision modes ---------------------------------------------------- */
 enum {
   HSL2_BA_SPLIT = 0,  /* FP32 linearization + Schur Gram, FP64 reduced Cholesky and back-substitution */
-  HSL2_BA_FP32 = 1    /* the same kernels with an FP32 reduced Cholesky and back-substitution */
+  HSL2_BA_FP32 = 1    /* the same kernels all in FP32: per-edge Hessian blocks, frame sums, Schur Gram
+                         totals, reduced Cholesky and back-substitution */
 };
 
 /* ---- BA semantics ------------------------------------------------------------- */
