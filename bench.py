@@ -39,9 +39,11 @@ def parse():
     ap.add_argument('--no-plain', action='store_true', help='skip the all-FP32 comparison path')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run one 2 GiB device copy (counter calibration for rocprofv3 --pmc)')
-    ap.add_argument('--only', choices=['all', 'lookup', 'altcorr', 'window', 'hires64', 'mono', 'geometry', 'pgba'],
+    ap.add_argument('--only', choices=['all', 'ba', 'lookup', 'altcorr', 'window', 'hires64', 'mono', 'geometry',
+                                       'pgba'],
                     default='all',
-                    help='lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
+                    help='ba: time only the global300 ba call (split precision; short runs for rocprofv3 counters); '
+                         'lookup: time only the bf16 lookup (short runs for rocprofv3 counters); '
                          'mono: time the mono-prior step (JDSA, proj_trans) on the global300 graph; '
                          'geometry: time frame_distance, depth_filter and iproj on it; '
                          'pgba: time the Sim3 pose-graph BA step on a 300-keyframe loop-closure graph; '
@@ -98,6 +100,18 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+
+    if args.only == 'ba':
+        poses, disps = poses0.clone(), disps0.clone()
+
+        def call():
+            return db.ba(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, iters, p.lm, p.ep, False)
+
+        ms = time_launches(call, args.steps, args.warmup)
+        print(json.dumps({'metric': 'BA-only GN it/s (global300, split precision)',
+                          'value': round(iters / (ms / 1e3), 3), 'unit': 'GN it/s', 'ms_per_call': round(ms, 4),
+                          'iterations_per_call': iters, 'edges': E}), flush=True)
+        return
 
     if args.only == 'lookup':
         f = fm32.to(torch.bfloat16)

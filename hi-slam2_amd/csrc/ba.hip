@@ -51,6 +51,9 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
   R* vs = reinterpret_cast<R*>(ws + p->w_vs);
   R* S = reinterpret_cast<R*>(ws + p->w_S);
   R* Ev = reinterpret_cast<R*>(ws + p->w_Ev);
+  R* Spart = reinterpret_cast<R*>(ws + p->w_Spart);
+  R* Hpart = reinterpret_cast<R*>(ws + p->w_Hpart);
+  const int zl = p->lin_slices;
   R* A = reinterpret_cast<R*>(ws + p->w_A);
   R* bvec = reinterpret_cast<R*>(ws + p->w_b);
   R* xvec = reinterpret_cast<R*>(ws + p->w_x);
@@ -68,21 +71,26 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
     const bool last = it == iterations - 1;
     if (E > 0) {
       if (schur)
-        hipLaunchKernelGGL((ba_linearize<true, kSemCuda, R>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+        hipLaunchKernelGGL((ba_linearize<true, kSemCuda, R>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
-                           Hs, vs, Eall + (int64_t)P * 6 * HW, Eii, Cii, bz);
+                           Hs, vs, Eall + (int64_t)P * 6 * HW, Eii, Cii, bz, zl, Hpart);
       else
-        hipLaunchKernelGGL((ba_linearize<false, kSemCuda, R>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+        hipLaunchKernelGGL((ba_linearize<false, kSemCuda, R>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
-                           Hs, vs, nullptr, nullptr, nullptr, nullptr);
+                           Hs, vs, nullptr, nullptr, nullptr, nullptr, zl, Hpart);
       HSL2_LAUNCH_CHECK();
+      if (zl > 1) {
+        hipLaunchKernelGGL(lin_reduce<R>, dim3(static_cast<unsigned>((E * 156 + 255) / 256)), dim3(256), 0, s, Hpart,
+                           zl, E, Hs, vs);
+        HSL2_LAUNCH_CHECK();
+      }
     }
     if (schur) {
       launch_frame_reduce<R>(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
                           blob + p->o_frame_pose, blob + p->o_frame_order, HW, static_cast<int>(F), 0.f, Q, wv,
                           Eall, s);
       HSL2_LAUNCH_CHECK();
-      launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, s);
+      launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, Spart, s);
       HSL2_LAUNCH_CHECK();
     }
     const int64_t nn = n_pad * n_pad;

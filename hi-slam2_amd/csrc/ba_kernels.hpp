@@ -135,10 +135,15 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
              const float* __restrict__ weights, const int* __restrict__ edge_ii,
              const int* __restrict__ edge_jj, int HW, int wd, int E,
              Acc* __restrict__ Hs, Acc* __restrict__ vs, float* __restrict__ Eij,
-             float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz) {
+             float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz,
+             int slices = 1, Acc* __restrict__ Hpart = nullptr) {
   __shared__ Acc lds[16 * kLinThreads];
   __shared__ Acc red[90];
   const int e = blockIdx.x;
+  // pixel slice of this block (blockIdx.y of `slices`): with several slices per edge the block
+  // writes its 90 partial sums to Hpart [slices][E][90] and lin_reduce forms Hs and vs
+  const int span = (HW + slices - 1) / slices;
+  const int k_begin = blockIdx.y * span, k_end = min(HW, k_begin + span);
   const int ix = edge_ii[e], jx = edge_jj[e];
   const bool self = ix == jx;
   EdgeGeom g;
@@ -156,14 +161,14 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   const float* wt = weights + (int64_t)e * 2 * HW;
   // the next pixel's five loads are issued before this pixel's arithmetic
   float nd = 0.f, ntu = 0.f, ntv = 0.f, nwu = 0.f, nwv = 0.f;
-  if (threadIdx.x < HW) {
-    const int k = threadIdx.x;
+  if (k_begin + (int)threadIdx.x < k_end) {
+    const int k = k_begin + threadIdx.x;
     nd = dsp[k], ntu = tg[k], ntv = tg[HW + k], nwu = wt[k], nwv = wt[HW + k];
   }
-  for (int k = threadIdx.x; k < HW; k += kLinThreads) {
+  for (int k = k_begin + threadIdx.x; k < k_end; k += kLinThreads) {
     const float d = nd, tu = ntu, tv = ntv, wu = nwu, wv = nwv;
     const int kn = k + kLinThreads;
-    if (kn < HW) nd = dsp[kn], ntu = tg[kn], ntv = tg[HW + kn], nwu = wt[kn], nwv = wt[HW + kn];
+    if (kn < k_end) nd = dsp[kn], ntu = tg[kn], ntv = tg[HW + kn], nwu = wt[kn], nwv = wt[HW + kn];
     __builtin_amdgcn_sched_barrier(0);
     const int row = k / wd;
     const int col = k - row * wd;
@@ -215,6 +220,10 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   block_reduce_store<90, kLinThreads, Acc>(acc, lds, red);
   __syncthreads();
   const int t = threadIdx.x;
+  if (slices > 1) {
+    if (t < 90) Hpart[((int64_t)blockIdx.y * E + e) * 90 + t] = red[t];
+    return;
+  }
   if (t < 144) {
     const int q = t / 36, i = (t % 36) / 6, j = t % 6;
     int l;
@@ -226,6 +235,35 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   } else if (t < 156) {
     const int k = t - 144;
     vs[((int64_t)(k / 6) * E + e) * 6 + (k % 6)] = red[78 + k];
+  }
+}
+
+// Hs [4, E, 6, 6] and vs [2, E, 6] from the slices' partial sums Hpart [slices][E][90] (the
+// packed lower triangle of the edge's 12 x 12 block, then its 12-vector), summed in slice order.
+template <typename Acc>
+__global__ void __launch_bounds__(256)
+lin_reduce(const Acc* __restrict__ Hpart, int slices, int E, Acc* __restrict__ Hs, Acc* __restrict__ vs) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)E * 156) return;
+  const int e = static_cast<int>(idx / 156), t = static_cast<int>(idx % 156);
+  int l;
+  if (t < 144) {
+    const int q = t / 36, i = (t % 36) / 6, j = t % 6;
+    if (q == 0) l = tri(max(i, j), min(i, j));
+    else if (q == 1) l = tri(6 + j, i);
+    else if (q == 2) l = tri(6 + i, j);
+    else l = tri(6 + max(i, j), 6 + min(i, j));
+  } else {
+    l = 78 + (t - 144);
+  }
+  Acc v = Hpart[(int64_t)e * 90 + l];
+  for (int z = 1; z < slices; ++z) v += Hpart[((int64_t)z * E + e) * 90 + l];
+  if (t < 144) {
+    const int q = t / 36;
+    Hs[((int64_t)q * E + e) * 36 + (t % 36)] = v;
+  } else {
+    const int k = t - 144;
+    vs[((int64_t)(k / 6) * E + e) * 6 + (k % 6)] = v;
   }
 }
 
@@ -363,6 +401,10 @@ __device__ __forceinline__ float4 gram_load4(const float* __restrict__ row, int 
 // exactly symmetric.  (Acc = float, the all-FP32 mode, folds into FP32.)  A diagonal item (ga == gb) also forms the Ev rows of its group's rows,
 // Ev = sum_px Q w E (EEt6x6 / Ev6x1's Ev, src/droid_kernels.cu:1466-1560), from the staged
 // chunks: each thread 16 pixels of one row per chunk in FP32, FP64 across chunks and threads.
+// When the plan has few items (small problems: one item per frame for a frontend window) the
+// pixel range is cut into `slices` runs of whole 256-pixel folds, one block per (item, slice):
+// each block writes its partial S and Ev to its slice of Spart ([NPAIR 36 | NS 6] per slice) and
+// gram_reduce sums the slices in order (so the result does not depend on the schedule).
 constexpr int kGramKC = 32;                          // pixels per staged chunk
 constexpr int kGramPT = kGramKC / 2;                 // pixels of one row staged per thread
 constexpr int kGramFold = 256 / kGramKC;             // chunks per FP64 fold
@@ -374,13 +416,15 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
                   const int* __restrict__ items, int n_items, int item_blocks8,
                   const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
                   const int* __restrict__ frame_pair0, int HW, const float* __restrict__ wvec,
-                  Acc* __restrict__ S, Acc* __restrict__ Ev) {
+                  Acc* __restrict__ S, Acc* __restrict__ Ev, int slices, int64_t npair, int64_t part_stride,
+                  Acc* __restrict__ Spart) {
   __shared__ float Ls[2][kGramKC][kGramRS];
   __shared__ float qs[2][kGramKC], qws[2][kGramKC];
   __shared__ Acc evs[4][16 * kGramGroup];
   const int blk = blockIdx.x;
-  const int item = (blk % 8) * item_blocks8 + blk / 8;  // XCD-aware: runs of consecutive items
-  if (item >= n_items) return;
+  const int idx = (blk % 8) * item_blocks8 + blk / 8;   // XCD-aware: runs of consecutive (item, slice)
+  if (idx >= n_items * slices) return;
+  const int item = idx / slices, slice = idx - item * slices;
   const int t = threadIdx.x, lane = t % kWave;
   const int wv = __builtin_amdgcn_readfirstlane(t / kWave);
   const int f = items[3 * item], ga = items[3 * item + 1], gb = items[3 * item + 2];
@@ -484,7 +528,10 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int i = 0; i < 4; ++i) dacc[u][i] = 0.0;
-  const int nch = (HW + kGramKC - 1) / kGramKC;
+  const int nch_all = (HW + kGramKC - 1) / kGramKC;
+  const int nfold = (nch_all + kGramFold - 1) / kGramFold;
+  const int c_begin = (slice * nfold / slices) * kGramFold;
+  const int nch = min(nch_all, ((slice + 1) * nfold / slices) * kGramFold) - c_begin;   // chunks of this block
   // NP tile pairs per wave, the same for the whole block: waves with fewer pairs run
   // copies of a staged tile whose sums are not written
   auto run = [&](auto np_c) {
@@ -494,14 +541,14 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
     for (int u = 0; u < NP; ++u) acc[u] = gram_f4{0.f, 0.f, 0.f, 0.f};
     // chunk c + 2 is loaded while chunk c is consumed and chunk c + 1 waits in registers
     Stage g0, g1;
-    load(g0, 0);
-    if (nch > 1) load(g1, kGramKC);
+    load(g0, c_begin * kGramKC);
+    if (nch > 1) load(g1, (c_begin + 1) * kGramKC);
     store(g0, 0);
     __syncthreads();
     // cur held chunk c (already in LDS), nxt holds chunk c + 1
     auto chunk = [&](int c, Stage& cur, const Stage& nxt) {
       const int buf = c & 1;
-      if (c + 2 < nch) load(cur, (c + 2) * kGramKC);
+      if (c + 2 < nch) load(cur, (c_begin + c + 2) * kGramKC);
 #pragma unroll
       for (int st = 0; st < kGramKC / 4; ++st) {
         const int kr = 4 * st + lane / 16;
@@ -520,7 +567,7 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
           e = fmaf(Ls[buf][kGramKC / 4 * eq + j][er], qws[buf][kGramKC / 4 * eq + j], e);
         evd += static_cast<Acc>(e);
       }
-      if (c % kGramFold == kGramFold - 1 || c + 1 == nch) {   // every 256 pixels
+      if (c % kGramFold == kGramFold - 1 || c + 1 == nch) {   // every 256 pixels (c_begin: whole folds)
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
 #pragma unroll
@@ -544,6 +591,11 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
     default: run(std::integral_constant<int, 4>{}); break;
   }
 
+  if (slices > 1) {                                      // partial sums of this slice
+    Acc* part = Spart + (int64_t)slice * part_stride;
+    S = part;
+    Ev = part + npair * 36;
+  }
   if (diag) {
     evs[eq][er] = evd;
     __syncthreads();
@@ -569,6 +621,19 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
   }
 }
 
+// Sum of the slices' partial S and Ev (in slice order): part [slices][npair 36 + ns 6].
+template <typename Acc>
+__global__ void __launch_bounds__(256)
+gram_reduce(const Acc* __restrict__ part, int slices, int64_t npair, int64_t stride, Acc* __restrict__ S,
+            Acc* __restrict__ Ev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= stride) return;
+  Acc v = part[i];
+  for (int z = 1; z < slices; ++z) v += part[(int64_t)z * stride + i];
+  if (i < npair * 36) S[i] = v;
+  else Ev[i - npair * 36] = v;
+}
+
 // The Ev rows with the weight vector wvec (one wave per slot).
 template <typename Acc>
 inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
@@ -585,14 +650,19 @@ inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float*
 // Gram items, i.e. motion-only).
 template <typename Acc>
 inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
-                              const float* wvec, Acc* S, Acc* Ev, hipStream_t s) {
+                              const float* wvec, Acc* S, Acc* Ev, Acc* Spart, hipStream_t s) {
   const int HW = static_cast<int>(p->HW);
   if (p->NGGI > 0) {
     const int ni = static_cast<int>(p->NGGI);
-    const int ib8 = (ni + 7) / 8;
+    const int z = p->gram_slices;
+    const int ib8 = (ni * z + 7) / 8;
+    const int64_t stride = p->NPAIR * 36 + p->NS * 6;
     hipLaunchKernelGGL(ba_schur_gram_lds<Acc>, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_group_item,
                        ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, wvec,
-                       S, Ev);
+                       S, Ev, z, p->NPAIR, stride, Spart);
+    if (z > 1)
+      hipLaunchKernelGGL(gram_reduce<Acc>, dim3(static_cast<unsigned>((stride + 255) / 256)), dim3(256), 0, s, Spart,
+                         z, p->NPAIR, stride, S, Ev);
   } else {
     launch_schur_ev(p, blob, Eall, Q, wvec, Ev, s);
   }

@@ -275,6 +275,13 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
       }
     }
     p->NGGI = static_cast<int64_t>(items.size() / 3);
+    // few items (small problems): cut each item's pixels into slices of whole 256-pixel folds
+    // so that the launch has about kGramTargetBlocks blocks
+    if (p->NGGI > 0) {
+      const int64_t nfold = (HW + 255) / 256;
+      const int64_t want = (hsl2::kGramTargetBlocks + p->NGGI - 1) / p->NGGI;
+      p->gram_slices = static_cast<int>(std::max<int64_t>(1, std::min(want, nfold)));
+    }
     p->o_frame_pair0 = put(pair0);
     p->o_gram_group_item = put(items);
   }
@@ -301,6 +308,12 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
   p->w_vs = take(2 * E * 6 * f8);
   p->w_S = take(p->NPAIR * 36 * f8);
   p->w_Ev = take(p->NS * 6 * f8);
+  if (E > 0) {
+    const int64_t want = (hsl2::kLinTargetBlocks + E - 1) / E;
+    p->lin_slices = static_cast<int>(std::max<int64_t>(1, std::min(want, HW / hsl2::kLinMinSlice)));
+  }
+  p->w_Hpart = take(p->lin_slices > 1 ? p->lin_slices * E * 90 * real : 0);
+  p->w_Spart = take(p->gram_slices > 1 ? p->gram_slices * (p->NPAIR * 36 + p->NS * 6) * real : 0);
   p->n_pad = ((6 * P + 63) / 64) * 64;
   p->w_A = take(static_cast<size_t>(p->n_pad) * p->n_pad * real);
   p->w_b = take(static_cast<size_t>(p->n_pad) * real);

@@ -42,6 +42,8 @@ struct hsl2_ba_plan {
   int64_t o_frame_pair0 = 0;                   // [F] first Schur pair of each frame
   int64_t o_gram_group_item = 0;               // [NGGI][3] Gram group items (frame, row group, column group)
   int64_t NGGI = 0;
+  int gram_slices = 1;                         // pixel slices per Gram item (partial sums when > 1)
+  int lin_slices = 1;                          // pixel slices per edge in the linearisation
 
   // ---- workspace layout (byte offsets) ----
   size_t w_blob = 0;
@@ -55,6 +57,8 @@ struct hsl2_ba_plan {
   size_t w_vs = 0;       // f64 [2][E][6]
   size_t w_S = 0;        // f64 [NPAIR][6][6]
   size_t w_Ev = 0;       // f64 [NS][6]
+  size_t w_Hpart = 0;    // f64 or f32 [lin_slices][E][90]: partial edge sums (lin_slices > 1)
+  size_t w_Spart = 0;    // f64 or f32 [gram_slices][NPAIR 36 + NS 6]: partial Gram sums (gram_slices > 1)
   int64_t n_pad = 0;     // 6P rounded up to a multiple of 64 (identity padding)
   size_t w_A = 0;        // f64 or f32 [n_pad][n_pad], column-major
   size_t w_b = 0;        // f64 or f32 [n_pad]: rhs, then the forward-substitution result
@@ -79,4 +83,10 @@ enum { kTaskPair = 0, kTaskEv = 1 };
 // Gram items: a frame's 6m x 6m Gram matrix is cut into 16 x 16 tiles, and the row tiles into
 // groups of kGramGroup; one item (one block) covers the tile pairs of one or two groups
 constexpr int kGramGroup = 4;
+// below this many Gram items, items are cut into pixel slices to reach about this many blocks
+constexpr int kGramTargetBlocks = 1024;
+// below this many edges, each edge's pixels are cut into slices (of at least kLinMinSlice
+// pixels) so that the linearisation launch has about kLinTargetBlocks blocks
+constexpr int kLinTargetBlocks = 512;
+constexpr int kLinMinSlice = 768;
 }  // namespace hsl2

@@ -464,7 +464,7 @@ __global__ void gn_adjoint_poses(const double* __restrict__ pose_part, const int
 struct Views {
   const int* blob;
   float *Eall, *Eii, *Cii, *bz, *Q, *w, *lz;
-  double *Hs, *vs, *S, *Ev, *A, *b, *x, *Ld, *W, *Sinv;
+  double *Hs, *vs, *S, *Ev, *Spart, *Hpart, *A, *b, *x, *Ld, *W, *Sinv;
   int* info;
 };
 
@@ -482,6 +482,8 @@ Views views(const hsl2_ba_plan* p, char* ws) {
   v.vs = reinterpret_cast<double*>(ws + p->w_vs);
   v.S = reinterpret_cast<double*>(ws + p->w_S);
   v.Ev = reinterpret_cast<double*>(ws + p->w_Ev);
+  v.Spart = reinterpret_cast<double*>(ws + p->w_Spart);
+  v.Hpart = reinterpret_cast<double*>(ws + p->w_Hpart);
   v.A = reinterpret_cast<double*>(ws + p->w_A);
   v.b = reinterpret_cast<double*>(ws + p->w_b);
   v.x = reinterpret_cast<double*>(ws + p->w_x);
@@ -531,22 +533,28 @@ extern "C" int hsl2_gn_step(const hsl2_ba_plan* p, float* poses, float* disps,
   hipLaunchKernelGGL(ba_clear_flags, dim3(1), dim3(64), 0, s, v.info);
   HSL2_LAUNCH_CHECK();
   if (E > 0) {
+    const int zl = p->lin_slices;
     if (schur)
-      hipLaunchKernelGGL((ba_linearize<true, kSemTorch>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+      hipLaunchKernelGGL((ba_linearize<true, kSemTorch>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                          intrinsics, 4, targets, weights, v.blob + p->o_edge_ii, v.blob + p->o_edge_jj,
-                         HW, wd, E, v.Hs, v.vs, v.Eall + (int64_t)P * 6 * HW, v.Eii, v.Cii, v.bz);
+                         HW, wd, E, v.Hs, v.vs, v.Eall + (int64_t)P * 6 * HW, v.Eii, v.Cii, v.bz, zl, v.Hpart);
     else
-      hipLaunchKernelGGL((ba_linearize<false, kSemTorch>), dim3(E), dim3(kLinThreads), 0, s, poses, disps,
+      hipLaunchKernelGGL((ba_linearize<false, kSemTorch>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                          intrinsics, 4, targets, weights, v.blob + p->o_edge_ii, v.blob + p->o_edge_jj,
-                         HW, wd, E, v.Hs, v.vs, nullptr, nullptr, nullptr, nullptr);
+                         HW, wd, E, v.Hs, v.vs, nullptr, nullptr, nullptr, nullptr, zl, v.Hpart);
     HSL2_LAUNCH_CHECK();
+    if (zl > 1) {
+      hipLaunchKernelGGL(lin_reduce<double>, dim3(blocks_for(int64_t(E) * 156)), dim3(256), 0, s, v.Hpart, zl, E,
+                         v.Hs, v.vs);
+      HSL2_LAUNCH_CHECK();
+    }
   }
   if (schur) {
     launch_frame_reduce(v.Cii, v.bz, v.Eii, eta, static_cast<int>(F), v.blob + p->o_frame_eptr,
                         v.blob + p->o_frame_edges, v.blob + p->o_frame_pose, v.blob + p->o_frame_order, HW,
                         static_cast<int>(F), 1e-7f, v.Q, v.w, v.Eall, s);
     HSL2_LAUNCH_CHECK();
-    launch_schur_gram(p, v.blob, v.Eall, v.Q, v.w, v.S, v.Ev, s);
+    launch_schur_gram(p, v.blob, v.Eall, v.Q, v.w, v.S, v.Ev, v.Spart, s);
     HSL2_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(ba_assemble<double>, dim3(blocks_for(n_pad * n_pad)), dim3(256), 0, s, v.Hs, v.S,
