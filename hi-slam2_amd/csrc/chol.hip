@@ -6,11 +6,11 @@
 // right-looking tiled algorithm over 64x64 tiles:
 //
 //   for each tile column k:
-//     chol_panel   one block per tile row i >= k: in every block one wave factors the
-//                  diagonal tile in registers (readlane broadcasts; redundantly in every
-//                  block, so no inter-block wait is needed) and publishes each finished column
-//                  of L to LDS behind a workgroup-scope counter; a second wave consumes the
-//                  columns four at a time as they arrive (no barrier per column): block 0 forms
+//     chol_panel   one block per tile row i >= k: in every block two waves factor the
+//                  diagonal tile in registers (columns 0-31 and 32-63; redundantly in every
+//                  block, so no inter-block wait is needed) and publish the finished columns
+//                  of L to LDS behind a workgroup-scope counter, four at a time; a third wave
+//                  consumes the columns four at a time as they arrive (no barrier): block 0 forms
 //                  L_kk^-1 (kept in a separate diagonal buffer, A_kk is still being read by the
 //                  other blocks) and y_k = L_kk^-1 b_k, the others solve L_ik L_kk^T = A_ik
 //                  row by row
@@ -73,6 +73,13 @@ __device__ __forceinline__ double fast_rsqrt(double x) {
   y = y * (1.5 - 0.5 * x * y * y);
   return y;
 }
+// one Newton step from the hardware estimate: ~1.5 ulp, for the factor's critical path, where
+// each pivot's latency adds to every column
+__device__ __forceinline__ double pivot_rsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ __forceinline__ float pivot_rsqrt(float x) { return fast_rsqrt(x); }
 __device__ __forceinline__ float fast_rsqrt(float x) {
   float y = __builtin_amdgcn_rsqf(x);
   return y * (1.5f - 0.5f * x * y * y);
@@ -118,64 +125,15 @@ __device__ __forceinline__ bool wait_flag(const int* flag, int v) {
   return left != 0;
 }
 
-// LDS shared by the two waves of a panel block.
+// LDS shared by the waves of a panel block.
 template <typename R> struct PanelShared {
   R Lc[kTile][kLd];        // Lc[c][r] = L_kk[r][c]
   R dinv[kTile];           // 1 / L_kk[c][c]
   int published;           // columns of L_kk published so far
 };
 
-// Producer (wave 0 of every panel block): factor the diagonal tile with ONE wave.  Lane r
-// holds row r in registers.  At step c the pivot is broadcast from lane c (v_readlane); column
-// c of L and 1 / L[c][c] go to LDS and `published` is raised with a workgroup-scope release
-// store; the rank-1 update of the trailing rows then reads column c back from LDS.  The
-// consumer wave follows through the counter, so neither wave waits at a barrier per column.  (Every lane stores dinv[c]: the same value to the
-// same address, and no per-column branch.)  A non-positive pivot sets info[0] = 1 when
-// `report` and is replaced by 1 so that the arithmetic stays finite.
-// Trailing update of producer step C in groups of kProdGroup columns s: the group's LDS
-// broadcast reads are issued, then (fenced) the previous group's FMAs, so one group's reads
-// are in flight while the other's FMAs run.
+// Factor waves broadcast LDS reads in groups of kProdGroup columns.
 constexpr int kProdGroup = 16;
-
-template <typename R, int C, int G>
-__device__ __forceinline__ void prod_read(const PanelShared<R>& sh, R (&v)[kProdGroup]) {
-#pragma unroll
-  for (int i = 0; i < kProdGroup; ++i) {
-    constexpr int s0 = C + 1 + G * kProdGroup;
-    if (s0 + i < kTile) v[i] = sh.Lc[C][s0 + i];
-  }
-}
-
-template <typename R, int C, int G>
-__device__ __forceinline__ void prod_fma(R (&a)[kTile], R l, const R (&v)[kProdGroup]) {
-#pragma unroll
-  for (int i = 0; i < kProdGroup; ++i) {
-    constexpr int s0 = C + 1 + G * kProdGroup;
-    if (s0 + i < kTile) a[s0 + i] -= l * v[i];
-  }
-}
-
-template <typename R, int C, typename Publish, typename Hook, int... Gs>
-__device__ __forceinline__ void prod_update(R (&a)[kTile], R l, const PanelShared<R>& sh, Publish& publish,
-                                            Hook& while_reading, std::integer_sequence<int, Gs...>) {
-  R v[2][kProdGroup];
-  prod_read<R, C, 0>(sh, v[0]);
-  // the hook (the next column's pivot, which needs no LDS value) runs while the first reads
-  // are in flight; the counter is raised after them, so its release wait also covers the
-  // reads instead of the reads starting one LDS round trip later
-  __builtin_amdgcn_sched_barrier(0);
-  while_reading();
-  __builtin_amdgcn_sched_barrier(0);
-  publish();
-  // group G: issue the reads of G + 1, then the FMAs of G
-  (([&] {
-     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) prod_read<R, C, Gs + 1>(sh, v[(Gs + 1) & 1]);
-     __builtin_amdgcn_sched_barrier(0);
-     prod_fma<R, C, Gs>(a, l, v[Gs & 1]);
-     __builtin_amdgcn_sched_barrier(0);
-   }()),
-   ...);
-}
 
 // Pivot of column C from lane C's value x (non-positive or NaN -> 1, flagged) and its
 // reciprocal square root.
@@ -186,51 +144,133 @@ __device__ __forceinline__ void prod_pivot(R x, R& piv, R& is, bool& bad) {
     bad = true;
     piv = R(1);
   }
-  is = fast_rsqrt(piv);
+  is = pivot_rsqrt(piv);
 }
 
-template <typename R, int C>
-__device__ __forceinline__ void prod_column(R (&a)[kTile], PanelShared<R>& sh, bool& bad, R& piv, R& is) {
-  const int r = threadIdx.x;
-  const R l = (r == C) ? piv * is : (r > C ? a[C] * is : R(0));     // L[r][C]
+// ---- the diagonal factor on kFactorWaves waves -------------------------------------------
+// Factor wave w owns the kFW columns [w kFW, (w + 1) kFW) of the diagonal tile (lane r: row r,
+// a[s] = A_kk[r][w kFW + s]).  It first applies the rank-1 updates of the columns before its own,
+// four published columns at a time, then factors its columns as the single-wave producer above
+// did (pivot by readlane, column and 1 / L_cc published behind the counter, the rank-1 update of
+// its later columns read back from LDS as broadcasts, the next pivot formed while those reads
+// are in flight).  The waves publish in column order, so the consumers are unchanged.  LDS
+// addresses are taken relative to an opaque zero so that each column's reads share one base
+// register and differ only in their immediate offsets.
+constexpr int kFactorWaves = 2;
+constexpr int kFW = kTile / kFactorWaves;
+
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
+// The rest of column C's rank-1 update, a[s] -= L[r][C] L[OWN0 + s][C] for s > c + 1 (column
+// c + 1 was updated on the critical path), with L[.][C] read back from LDS as broadcasts in
+// groups of kProdGroup: the reads of group G + 1 are issued before the FMAs of group G.
+template <typename R, int C, int OWN0, int G>
+__device__ __forceinline__ void fw_read(const PanelShared<R>& sh, int z, R (&v)[kProdGroup]) {
+  constexpr int s0 = C - OWN0 + 2 + G * kProdGroup;      // first register index of the group
+#pragma unroll
+  for (int i = 0; i < kProdGroup; ++i)
+    if (s0 + i < kFW) v[i] = sh.Lc[C][z + OWN0 + s0 + i];
+}
+
+template <typename R, int C, int OWN0, int G>
+__device__ __forceinline__ void fw_fma(R (&a)[kFW], R l, const R (&v)[kProdGroup]) {
+  constexpr int s0 = C - OWN0 + 2 + G * kProdGroup;
+#pragma unroll
+  for (int i = 0; i < kProdGroup; ++i)
+    if (s0 + i < kFW) a[s0 + i] -= l * v[i];
+}
+
+template <typename R, int C, int OWN0, typename Hook, int... Gs>
+__device__ __forceinline__ void fw_bulk(R (&a)[kFW], R l, const PanelShared<R>& sh, int z, Hook& while_reading,
+                                        std::integer_sequence<int, Gs...>) {
+  R v[2][kProdGroup];
+  if constexpr (sizeof...(Gs) > 0) fw_read<R, C, OWN0, 0>(sh, z, v[0]);
+  __builtin_amdgcn_sched_barrier(0);
+  while_reading();
+  (([&] {
+     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) fw_read<R, C, OWN0, Gs + 1>(sh, z, v[(Gs + 1) & 1]);
+     fw_fma<R, C, OWN0, Gs>(a, l, v[Gs & 1]);
+   }()),
+   ...);
+}
+
+// Column C (local index c): L[r][C] from the pivot already formed; the next column's update by
+// L[C + 1][C] through a readlane (no LDS round trip on the critical path); column C and
+// 1 / L_CC go to LDS; the next pivot's reciprocal square root runs while the first group of the
+// rest of the update is being read; the column is published; the rest of the update follows.
+template <typename R, int C, int OWN0>
+__device__ __forceinline__ void fw_column(R (&a)[kFW], PanelShared<R>& sh, int z, bool& bad, R& piv, R& is) {
+  const int r = threadIdx.x % kWave;
+  constexpr int c = C - OWN0;
+  const R l = (r == C) ? piv * is : (r > C ? a[c] * is : R(0));     // L[r][C]
+  if constexpr (c + 1 < kFW) a[c + 1] -= l * lane_bcast<R>(l, C + 1);
   sh.Lc[C][r] = l;                               // column C of L
   sh.dinv[C] = is;
-  auto publish = [&] {
-    __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  auto next_pivot = [&] {
+    if constexpr (c + 1 < kFW) prod_pivot<R, C + 1>(a[c + 1], piv, is, bad);
+    // published in steps of four columns (the consumers and the later factor waves wait for
+    // four at a time): the release store waits for every LDS access in flight
+    if constexpr ((C + 1) % 4 == 0)
+      __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
-  // trailing update a[r][s] -= L[r][C] L[s][C], s > C, with L[s][C] read back from the column
-  // just stored (the same address for every lane: LDS broadcast reads, two values per read).
-  // The next pivot a[C+1][C+1] - L[C+1][C]^2 needs only lane C+1's own values, so it is formed
-  // while those reads are in flight (lane C+1's updated a[C + 1] is not used again: its L entry
-  // is piv * is)
-  if constexpr (C + 1 < kTile) {
-    auto next_pivot = [&] { prod_pivot<R, C + 1>(a[C + 1] - l * l, piv, is, bad); };
-    prod_update<R, C>(a, l, sh, publish, next_pivot,
-                      std::make_integer_sequence<int, (kTile - 1 - C + kProdGroup - 1) / kProdGroup>{});
-  } else {
-    publish();
-  }
+  constexpr int rest = kFW - (c + 2) > 0 ? kFW - (c + 2) : 0;
+  fw_bulk<R, C, OWN0>(a, l, sh, z, next_pivot,
+                      std::make_integer_sequence<int, (rest + kProdGroup - 1) / kProdGroup>{});
 }
 
-template <typename R, int... Cs>
-__device__ __forceinline__ void prod_all(R (&a)[kTile], PanelShared<R>& sh, bool& bad,
-                                         std::integer_sequence<int, Cs...>) {
+template <typename R, int OWN0, int... Cs>
+__device__ __forceinline__ void fw_all(R (&a)[kFW], PanelShared<R>& sh, int z, bool& bad,
+                                       std::integer_sequence<int, Cs...>) {
   R piv, is;
-  prod_pivot<R, 0>(a[0], piv, is, bad);
-  (prod_column<R, Cs>(a, sh, bad, piv, is), ...);
+  prod_pivot<R, OWN0>(a[0], piv, is, bad);
+  (fw_column<R, OWN0 + Cs, OWN0>(a, sh, z, bad, piv, is), ...);
 }
 
-template <typename R>
-__device__ void factor_wave(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh, bool report,
-                            int* __restrict__ info) {
-  const int r = threadIdx.x;                     // 0..63
-  const int64_t base = (int64_t)k * kTile;
-  R a[kTile];
+// a[s] -= sum_{c < OWN0} L[r][c] L[OWN0 + s][c], four published columns per step; false if a
+// wait gave up.
+template <typename R, int OWN0>
+__device__ __forceinline__ bool fw_apply_earlier(R (&a)[kFW], const PanelShared<R>& sh, int z) {
+  const int r = threadIdx.x % kWave;
+  unsigned late = 0;
+  for (int c0 = 0; c0 < OWN0; c0 += 4) {
+    late |= wait_flag(&sh.published, c0 + 4) ? 0u : 1u;
 #pragma unroll
-  for (int c = 0; c < kTile; ++c) a[c] = A[(base + r) + (base + c) * lda];
+    for (int m = 0; m < 4; ++m) {
+      const R l = sh.Lc[c0 + m][r];
+      R v[kFW];
+#pragma unroll
+      for (int q = 0; q < kFW; ++q) v[q] = sh.Lc[c0 + m][z + OWN0 + q];
+#pragma unroll
+      for (int q = 0; q < kFW; ++q) a[q] -= l * v[q];
+    }
+  }
+  return late == 0;
+}
+
+template <typename R, int OWN0>
+__device__ void factor_part(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh, bool report,
+                            int* __restrict__ info) {
+  const int r = threadIdx.x % kWave;
+  const int64_t base = (int64_t)k * kTile;
+  R a[kFW];
+#pragma unroll
+  for (int c = 0; c < kFW; ++c) a[c] = A[(base + r) + (base + OWN0 + c) * lda];
+  const int z = opaque_zero();
   bool bad = false;
-  prod_all<R>(a, sh, bad, std::make_integer_sequence<int, kTile>{});
+  if constexpr (OWN0 > 0) bad = !fw_apply_earlier<R, OWN0>(a, sh, z);
+  fw_all<R, OWN0>(a, sh, z, bad, std::make_integer_sequence<int, kFW>{});
   if (bad && report && r == 0) info[0] = 1;
+}
+
+template <typename R, int... Ws>
+__device__ __forceinline__ void factor_waves(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh,
+                                             bool report, int* __restrict__ info, std::integer_sequence<int, Ws...>) {
+  const int w = threadIdx.x / kWave;
+  ((w == Ws ? factor_part<R, Ws * kFW>(A, lda, k, sh, report, info) : void()), ...);
 }
 
 // Consumer step over columns M .. M+3 of L_kk once they are published: the 4x4 triangle
@@ -304,7 +344,7 @@ __device__ __forceinline__ bool consume_all(R (&x)[kTile], PanelShared<R>& sh, O
 // acc_j -= x_m L[j][m] for j > m.
 template <typename R>
 __device__ bool solve_rows(R* __restrict__ A, int64_t lda, int64_t bi, int64_t bk, PanelShared<R>& sh) {
-  const int q = threadIdx.x - kWave;
+  const int q = threadIdx.x - kFactorWaves * kWave;
   R x[kTile];
 #pragma unroll
   for (int c = 0; c < kTile; ++c) x[c] = A[(bi + q) + (bk + c) * lda];
@@ -316,7 +356,7 @@ __device__ bool solve_rows(R* __restrict__ A, int64_t lda, int64_t bi, int64_t b
 // columns of L arrive; X goes to Xs (row-major) for the caller.
 template <typename R>
 __device__ bool invert_columns(PanelShared<R>& sh, R (*Xs)[kLd]) {
-  const int j = threadIdx.x - kWave;
+  const int j = threadIdx.x - kFactorWaves * kWave;
   R x[kTile];
 #pragma unroll
   for (int m = 0; m < kTile; ++m) x[m] = m == j ? R(1) : R(0);
@@ -324,8 +364,8 @@ __device__ bool invert_columns(PanelShared<R>& sh, R (*Xs)[kLd]) {
   return consume_all<R>(x, sh, out, std::make_integer_sequence<int, kTile / 4>{});
 }
 
-// Panel block for tile row i = k + blk (threads 0..127; the launch's other threads have
-// returned).  Wave 0 factors the diagonal tile and publishes L column by column; wave 1
+// Panel block for tile row i = k + blk (threads 0..191; the launch's other threads have
+// returned).  Waves 0 and 1 factor the diagonal tile and publish L four columns at a time; wave 2
 // consumes the columns as they arrive: block 0 inverts L_kk (kept in Ld for the
 // back-substitution, and y_k = L_kk^-1 b_k), every other block solves its tile L_ik.  The only
 // block barrier is the one after the counter is cleared.
@@ -337,19 +377,19 @@ __device__ void panel_block(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, 
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
   if (t == 0) sh.published = 0;
   __syncthreads();
-  if (t < kWave) {
-    factor_wave<R>(A, lda, k, sh, blk == 0, info);
+  if (t < kFactorWaves * kWave) {
+    factor_waves<R>(A, lda, k, sh, blk == 0, info, std::make_integer_sequence<int, kFactorWaves>{});
     return;
   }
   if (blk > 0) {
-    if (!solve_rows<R>(A, lda, bi, bk, sh) && t == kWave) info[0] = 1;
+    if (!solve_rows<R>(A, lda, bi, bk, sh) && t == kFactorWaves * kWave) info[0] = 1;
     return;
   }
   const bool ok = invert_columns<R>(sh, Xs);
-  if (!ok && t == kWave) info[0] = 1;
-  // wave 1 alone from here (its own LDS writes, in order): L_kk^-1 column-major into Ld, and
+  if (!ok && t == kFactorWaves * kWave) info[0] = 1;
+  // wave 2 alone from here (its own LDS writes, in order): L_kk^-1 column-major into Ld, and
   // y_k = L_kk^-1 b_k
-  const int u = t - kWave;
+  const int u = t - kFactorWaves * kWave;
   R* Lk = Ld + (int64_t)k * kTile * kTile;
 #pragma unroll 8
   for (int c = 0; c < kTile; ++c) Lk[c * kTile + u] = Xs[u][c];
@@ -445,7 +485,7 @@ chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b,
   const int npanel = T - k;
   const int blk = blockIdx.x;
   if (blk < npanel) {
-    if (threadIdx.x >= 2 * kWave) return;
+    if (threadIdx.x >= (kFactorWaves + 1) * kWave) return;
     panel_block<R>(A, lda, Ld, b, k, blk, info, smem.panel.sh, smem.panel.Xs);
     return;
   }
