@@ -22,10 +22,12 @@ Semantics kept on purpose (the drop-in reproduces them):
 
 Parity pins (tests/test_oracle_pgba.py): Jj, Jz and the translation entries of Ji against
 central differences of the projection under a left Sim3 perturbation at general poses;
-all of Ji against them where sij == 1 and tij == 0; the Schur solve against the dense
-joint (pose + depth) solve; the pose-graph block placement against a hand-built system.
-Parity unpinned: the rotation and scale entries of Ji where sij != 1 or tij != 0 follow
-the reference text only.
+all of Ji against them where sij == 1 and tij == 0; at general poses every entry of Ji
+through the identity adjSim3 = D_s Ad^T + N (Ad^T the transposed Sim3 adjoint, so that
+Ji = D_s Ji_true - N Jj with Ji_true and Jj both from central differences), and the map
+itself against a 7x7 block matrix assembled from R, t and s; the Schur solve against the
+dense joint (pose + depth) solve; the pose-graph block placement against a hand-built
+system.
 
 Test infrastructure only.
 """

@@ -229,3 +229,59 @@ def test_argument_contract():
         opg.pgba(args[0][:, :7], *args[1:], 1, 4)
     with pytest.raises(ValueError):
         opg.pgba(*args, 2, 2)
+
+
+def adj_sim3_matrix(t, q, s):
+    """The 7x7 matrix of the reference's adjSim3 (src/droid_kernels.cu:121-146) assembled block by
+    block from R (scipy), t and s -- a second writing of the map, independent of the oracle's
+    vector code: rows 0-2 s R^T X_tau; rows 3-5 s (R^T X_phi + R^T (X_tau x t)) + X_sigma t;
+    row 6 X_sigma."""
+    from scipy.spatial.transform import Rotation
+    R = Rotation.from_quat(q).as_matrix()                         # xyzw, as the reference stores q
+    tx = np.array([[0, -t[2], t[1]], [t[2], 0, -t[0]], [-t[1], t[0], 0]])
+    M = np.zeros((7, 7))
+    M[0:3, 0:3] = s * R.T
+    M[3:6, 0:3] = -s * R.T @ tx                                   # X_tau x t = -[t]x X_tau
+    M[3:6, 3:6] = s * R.T
+    M[3:6, 6] = t
+    M[6, 6] = 1.0
+    return M
+
+
+def test_adj_sim3_equals_its_block_matrix_at_general_poses():
+    rng = np.random.default_rng(11)
+    for _ in range(5):
+        p = random_sim3(rng, rng.uniform(0.5, 2.0))
+        t, q, s = p[:3], p[3:7], p[7]
+        X = rng.normal(size=(4, 7))
+        got = opg.adj_sim3_T(t[None], q[None], np.full(1, s), X)
+        assert np.allclose(got, X @ adj_sim3_matrix(t, q, s).T, atol=1e-12, rtol=1e-12)
+
+
+def test_full_ji_against_central_differences_at_general_poses():
+    """Every entry of the reference-semantics Ji at sij != 1, tij != 0, from central differences
+    alone: with Ad^T the transposed Sim3 adjoint (Ji_true = -Ad^T Jj, both by differences), the
+    reference's map is D_s Ad^T + N, D_s = diag(1, 1, 1, s, s, s, 1) and N with tij in column 6
+    of rows 3-5 and tij^T in row 6, so Ji = D_s Ji_true - N Jj.  A slip in any rotation, scale or
+    coupling entry of the oracle's map changes one of these columns."""
+    rng = np.random.default_rng(12)
+    h, w = 4, 5
+    u, v = oba.pixel_grid(h, w)
+    for trial in range(3):
+        pi, pj = random_sim3(rng, rng.uniform(0.6, 1.6)), random_sim3(rng, rng.uniform(0.6, 1.6))
+        disp = rng.uniform(0.3, 1.0, size=(h, w))
+        t = terms(pi, pj, disp, h, w)
+        tij, _, sij = opg.relative_sim3(pi[None], pj[None])
+        tij, sij = tij[0], float(sij[0])
+        assert abs(sij - 1.0) > 0.05 and np.linalg.norm(tij) > 1e-3
+        Ti, Tj = sim3_mat(pi), sim3_mat(pj)
+        Ds = np.diag([1, 1, 1, sij, sij, sij, 1.0])
+        for p in (0, 8, 19):
+            d = disp.reshape(-1)[p]
+            Ji_true = fd_columns(Ti, Tj, d, u[p], v[p], 'i')      # [2, 7]
+            Jj = fd_columns(Ti, Tj, d, u[p], v[p], 'j')
+            NJj = np.zeros((2, 7))
+            NJj[:, 3:6] = Jj[:, 6:7] * tij[None]
+            NJj[:, 6] = Jj[:, :3] @ tij
+            expect = Ji_true @ Ds - NJj
+            assert np.allclose(t['Ji'][0, :, p], expect, atol=2e-5, rtol=1e-6), (trial, p)
