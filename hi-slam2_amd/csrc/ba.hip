@@ -34,10 +34,13 @@
 namespace hsl2 {
 namespace {
 
+// Sensor-disparity prior weight of the 15-argument ba form (upstream DROID-SLAM's RGB-D mode).
+constexpr float kSensAlpha = 0.05f;
+
 template <typename R>
 int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const float* intr,
-                   const float* targets, const float* weights, const float* eta, int eta_rows,
-                   int iterations, float lm, float ep, float* dx, float* dz, char* ws,
+                   const float* disps_sens, const float* targets, const float* weights, const float* eta,
+                   int eta_rows, int iterations, float lm, float ep, float* dx, float* dz, char* ws,
                    hipStream_t s) {
   const int* blob = reinterpret_cast<const int*>(ws + p->w_blob);
   float* Eall = reinterpret_cast<float*>(ws + p->w_Eall);
@@ -87,8 +90,8 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
     }
     if (schur) {
       launch_frame_reduce<R>(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
-                          blob + p->o_frame_pose, blob + p->o_frame_order, HW, static_cast<int>(F), 0.f, Q, wv,
-                          Eall, s);
+                             blob + p->o_frame_pose, blob + p->o_frame_order, HW, static_cast<int>(F), 0.f, Q, wv,
+                             Eall, s, disps, disps_sens, blob + p->o_frame_kx, kSensAlpha);
       HSL2_LAUNCH_CHECK();
       launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, Spart, s);
       HSL2_LAUNCH_CHECK();
@@ -432,11 +435,11 @@ jdsa_apply(const float* __restrict__ C, const float* __restrict__ w, const float
 
 using namespace hsl2;
 
-extern "C" int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps,
-                           const float* intrinsics, const float* targets, const float* weights,
-                           const float* eta, int64_t eta_rows, int iterations, float lm, float ep,
-                           float* dx, float* dz, void* workspace, size_t workspace_bytes,
-                           int* solve_ok, void* stream) {
+extern "C" int hsl2_ba_run_sens(const hsl2_ba_plan* plan, float* poses, float* disps,
+                                const float* intrinsics, const float* disps_sens, const float* targets,
+                                const float* weights, const float* eta, int64_t eta_rows, int iterations,
+                                float lm, float ep, float* dx, float* dz, void* workspace,
+                                size_t workspace_bytes, int* solve_ok, void* stream) {
   HSL2_REQUIRE(plan, HSL2_ERR_ARGUMENT, "ba: null plan");
   HSL2_REQUIRE(poses && disps && intrinsics && dx, HSL2_ERR_ARGUMENT, "ba: null pointer");
   HSL2_REQUIRE(plan->E == 0 || (targets && weights), HSL2_ERR_ARGUMENT, "ba: null targets/weights");
@@ -458,10 +461,10 @@ extern "C" int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps,
   }
   int st;
   if (plan->precision == HSL2_BA_SPLIT)
-    st = run_iterations<double>(plan, poses, disps, intrinsics, targets, weights, eta,
+    st = run_iterations<double>(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta,
                                 static_cast<int>(eta_rows), iterations, lm, ep, dx, dz, ws, s);
   else
-    st = run_iterations<float>(plan, poses, disps, intrinsics, targets, weights, eta,
+    st = run_iterations<float>(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta,
                                static_cast<int>(eta_rows), iterations, lm, ep, dx, dz, ws, s);
   if (st) return st;
   if (iterations == 0) HSL2_HIP(hipMemsetAsync(dx, 0, sizeof(float) * 6 * plan->P, s));
@@ -524,6 +527,15 @@ extern "C" int hsl2_ba_run_static(const hsl2_ba_plan* plan, float* poses, float*
   if (st) return st;
   return hsl2_ba_run(plan, poses, disps, intrinsics, targets, weights, eta, eta_rows, iterations, lm, ep, dx,
                      dz, ws, HSL2_BA_STATIC_SCRATCH_BYTES, solve_ok, stream);
+}
+
+extern "C" int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps,
+                           const float* intrinsics, const float* targets, const float* weights,
+                           const float* eta, int64_t eta_rows, int iterations, float lm, float ep,
+                           float* dx, float* dz, void* workspace, size_t workspace_bytes,
+                           int* solve_ok, void* stream) {
+  return hsl2_ba_run_sens(plan, poses, disps, intrinsics, nullptr, targets, weights, eta, eta_rows, iterations, lm,
+                          ep, dx, dz, workspace, workspace_bytes, solve_ok, stream);
 }
 
 extern "C" int hsl2_ba_debug_views(const hsl2_ba_plan* plan, void* workspace, double** Hs,

@@ -268,14 +268,19 @@ lin_reduce(const Acc* __restrict__ Hpart, int slices, int E, Acc* __restrict__ H
 }
 
 // ---- 2. per-frame sums: C, w, Q and the pose-i Schur block Ei ----------------
-// Sums in Acc (FP64 in the split mode, FP32 in the all-FP32 mode).
+// Sums in Acc (FP64 in the split mode, FP32 in the all-FP32 mode).  With disps_sens (the
+// sensor-disparity form of upstream DROID-SLAM's ba): a pixel whose sensor disparity is positive
+// takes the prior alpha (d + dz - d_sens)^2 / 2 in place of eta, C + alpha and w - alpha (d - d_sens);
+// the other pixels keep exactly the arithmetic of the call without it.
 template <typename Acc>
 __global__ void __launch_bounds__(256)
 ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
                 const float* __restrict__ Eii, const float* __restrict__ eta, int eta_rows,
                 const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
                 const int* __restrict__ frame_pose, const int* __restrict__ frame_order, int HW, float c_eps,
-                float* __restrict__ Q, float* __restrict__ w, float* __restrict__ Eall) {
+                float* __restrict__ Q, float* __restrict__ w, float* __restrict__ Eall,
+                const float* __restrict__ disps, const float* __restrict__ disps_sens,
+                const int* __restrict__ frame_kx, float alpha) {
   const int f = frame_order[blockIdx.y];          // the frames with most edges first
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= HW) return;
@@ -301,7 +306,15 @@ ba_frame_reduce(const float* __restrict__ Cii, const float* __restrict__ bz,
     c += Cii[(int64_t)e * HW + k];
     b += bz[(int64_t)e * HW + k];
   }
-  const float et = eta[(int64_t)(eta_rows == 1 ? 0 : f) * HW + k];
+  float et = eta[(int64_t)(eta_rows == 1 ? 0 : f) * HW + k];
+  if (disps_sens) {
+    const int64_t px = (int64_t)frame_kx[f] * HW + k;
+    const float ds = disps_sens[px];
+    if (ds > 0.f) {
+      et = alpha;
+      b -= static_cast<Acc>(alpha) * (static_cast<Acc>(disps[px]) - static_cast<Acc>(ds));
+    }
+  }
   Q[(int64_t)f * HW + k] = static_cast<float>(Acc(1) / (c + et + c_eps));
   w[(int64_t)f * HW + k] = static_cast<float>(b);
   const int p = frame_pose[f];
@@ -335,9 +348,12 @@ template <typename Acc = double>
 inline void launch_frame_reduce(const float* Cii, const float* bz, const float* Eii, const float* eta,
                                 int eta_rows, const int* frame_eptr, const int* frame_edges,
                                 const int* frame_pose, const int* frame_order, int HW, int F, float c_eps,
-                                float* Q, float* w, float* Eall, hipStream_t s) {
+                                float* Q, float* w, float* Eall, hipStream_t s, const float* disps = nullptr,
+                                const float* disps_sens = nullptr, const int* frame_kx = nullptr,
+                                float alpha = 0.f) {
   hipLaunchKernelGGL(ba_frame_reduce<Acc>, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii, eta, eta_rows,
-                     frame_eptr, frame_edges, frame_pose, frame_order, HW, c_eps, Q, w, Eall);
+                     frame_eptr, frame_edges, frame_pose, frame_order, HW, c_eps, Q, w, Eall, disps, disps_sens,
+                     frame_kx, alpha);
 }
 
 // ---- 3. Ev rows, one wave per slot -------------------------------------------

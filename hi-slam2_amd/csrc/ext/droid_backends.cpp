@@ -113,7 +113,7 @@ std::vector<torch::Tensor> ba_impl(torch::Tensor poses, torch::Tensor disps, tor
                                    torch::Tensor targets, torch::Tensor weights, torch::Tensor eta,
                                    torch::Tensor ii, torch::Tensor jj, int t0, int t1, int iterations,
                                    double lm, double ep, bool motion_only, int precision,
-                                   bool want_status) {
+                                   bool want_status, const torch::Tensor* disps_sens = nullptr) {
   CHECK_CONTIGUOUS(targets);
   CHECK_CONTIGUOUS(weights);
   CHECK_CONTIGUOUS(poses);
@@ -163,14 +163,22 @@ std::vector<torch::Tensor> ba_impl(torch::Tensor poses, torch::Tensor disps, tor
   void* ws_ptr = reinterpret_cast<void*>((base + 255) & ~uintptr_t(255));
   torch::Tensor dx = torch::zeros({t1 - t0, 6}, opts);
   torch::Tensor dz = motion_only ? torch::Tensor() : torch::zeros({F, ht * wd}, opts);
+  const float* sens_ptr = nullptr;
+  if (disps_sens) {
+    CHECK_CONTIGUOUS((*disps_sens));
+    require_device(*disps_sens, "disps_sens");
+    require(disps_sens->scalar_type() == torch::kFloat32 && disps_sens->sizes() == disps.sizes(),
+            "ba: disps_sens must be float32 shaped like disps [N, ht, wd]");
+    sens_ptr = disps_sens->data_ptr<float>();
+  }
   int ok = 1;
-  raise_status(hsl2_ba_run(plan, poses.data_ptr<float>(), disps.data_ptr<float>(),
-                           intrinsics.data_ptr<float>(), targets.data_ptr<float>(),
-                           weights.data_ptr<float>(), eta_ptr, eta_rows, iterations,
-                           static_cast<float>(lm), static_cast<float>(ep), dx.data_ptr<float>(),
-                           motion_only ? nullptr : dz.data_ptr<float>(),
-                           ws_ptr, static_cast<size_t>(ws.numel() - 256), want_status ? &ok : nullptr,
-                           current_stream()),
+  raise_status(hsl2_ba_run_sens(plan, poses.data_ptr<float>(), disps.data_ptr<float>(),
+                                intrinsics.data_ptr<float>(), sens_ptr, targets.data_ptr<float>(),
+                                weights.data_ptr<float>(), eta_ptr, eta_rows, iterations,
+                                static_cast<float>(lm), static_cast<float>(ep), dx.data_ptr<float>(),
+                                motion_only ? nullptr : dz.data_ptr<float>(),
+                                ws_ptr, static_cast<size_t>(ws.numel() - 256), want_status ? &ok : nullptr,
+                                current_stream()),
                "ba");
   std::vector<torch::Tensor> out = {iterations > 0 ? dx : torch::Tensor(),
                                     iterations > 0 ? dz : torch::Tensor(), torch::Tensor()};
@@ -184,6 +192,16 @@ std::vector<torch::Tensor> ba(torch::Tensor poses, torch::Tensor disps, torch::T
                               double lm, double ep, bool motion_only) {
   return ba_impl(poses, disps, intrinsics, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
                  motion_only, HSL2_BA_SPLIT, false);
+}
+
+// The 15-argument form with the sensor disparities after the intrinsics (upstream DROID-SLAM's
+// binding; see hsl2_ba_run_sens).  An all-zero disps_sens gives the 14-argument results exactly.
+std::vector<torch::Tensor> ba_sens(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
+                                   torch::Tensor disps_sens, torch::Tensor targets, torch::Tensor weights,
+                                   torch::Tensor eta, torch::Tensor ii, torch::Tensor jj, int t0, int t1,
+                                   int iterations, double lm, double ep, bool motion_only) {
+  return ba_impl(poses, disps, intrinsics, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
+                 motion_only, HSL2_BA_SPLIT, false, &disps_sens);
 }
 
 std::vector<torch::Tensor> ba_ex(torch::Tensor poses, torch::Tensor disps, torch::Tensor intrinsics,
@@ -1071,6 +1089,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "MI355X (gfx950) dense-tracking kernels behind the droid_backends interface";
   // bundle adjustment (src/droid.cpp:327)
   m.def("ba", &ba, "dense bundle adjustment (split precision: FP32 linearization, FP64 reduced solve)");
+  m.def("ba", &ba_sens, "dense bundle adjustment with a sensor-disparity prior (15-argument form)");
   m.def("pgba", &pgba, "Sim3 pose-graph bundle adjustment step: [dx, dz], disps updated in place");
   // correlation volume kernels (src/droid.cpp:337-340)
   m.def("altcorr_forward", &altcorr_forward, "on-the-fly correlation lookup");

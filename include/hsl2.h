@@ -227,6 +227,19 @@ int hsl2_ba_run(const hsl2_ba_plan* plan, float* poses, float* disps, const floa
                 int iterations, float lm, float ep, float* dx, float* dz,
                 void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
 
+/* hsl2_ba_run with the sensor-disparity argument of the 15-argument ba form named for this path,
+ * ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, t1, iterations,
+ * lm, ep, motion_only) (upstream DROID-SLAM's RGB-D binding; this reference's ba,
+ * src/droid.cpp:141-168, has no such argument).  disps_sens [num_poses, ht, wd] f32 on the
+ * device, or NULL (then exactly hsl2_ba_run).  A pixel of depth frame kx[f] whose sensor disparity
+ * is positive takes the prior alpha (d + dz - d_sens)^2 / 2, alpha = 0.05, in place of eta:
+ * C = sum Cii + alpha, w = sum bz - alpha (d - d_sens); every other pixel's arithmetic is that of
+ * hsl2_ba_run, so an all-zero disps_sens gives bit-identical results. */
+int hsl2_ba_run_sens(const hsl2_ba_plan* plan, float* poses, float* disps, const float* intrinsics,
+                     const float* disps_sens, const float* targets, const float* weights, const float* eta,
+                     int64_t eta_rows, int iterations, float lm, float ep, float* dx, float* dz,
+                     void* workspace, size_t workspace_bytes, int* solve_ok, void* stream);
+
 /* No counterpart in the reference.  hsl2_ba_run for callers that cannot allocate device
  * memory (code under HIP graph capture, tight tracking loops, harnesses that forbid
  * hipMalloc): the workspace is a static device array of HSL2_BA_STATIC_SCRATCH_BYTES held by

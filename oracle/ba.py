@@ -172,13 +172,23 @@ def schur_pairs(jj_exp, kk_exp, t0, t1):
     return tuple(np.asarray(c, np.int64) for c in zip(*out))
 
 
-def reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only=False):
+SENSOR_ALPHA = 0.05
+
+
+def reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only=False, disps=None, disps_sens=None,
+                   alpha=SENSOR_ALPHA):
     """Assemble the damped-free reduced camera system (A - S, b - bS) in float64.
 
     A, b: src/droid_kernels.cu:1882-1889 with SparseBlock::update_lhs/rhs
     (1598-1640, negative slots dropped).  S, bS: schur_block (1689-1779) with
     C, w, Q, Ei from accum_cuda (1901-1906).  Returns a dict with the dense
     matrix, the rhs and the depth-side terms needed for back-substitution.
+
+    With ``disps_sens`` (the 15-argument form of upstream DROID-SLAM's ba, which this reference
+    does not have): pixels with a sensor disparity (m = disps_sens[kx] > 0) take the prior
+    1/2 alpha (d + dz - d_sens)^2 in place of eta, C = sum Cii + m alpha + (1 - m) eta and
+    w = sum bz - m alpha (d - d_sens) -- the form of HI-SLAM2's own mono prior
+    (hislam2/geom/ba.py:205-206) with a unit Jacobian.
     """
     D = lin['Hs'].shape[-1]                                   # 6 (SE3) or 7 (Sim3)
     P = t1 - t0
@@ -209,7 +219,14 @@ def reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only=False):
     np.add.at(C, kk_exp[P:], lin['Cii'])
     np.add.at(w, kk_exp[P:], lin['bz'])
     eta = np.asarray(eta, np.float64).reshape(-1, HW)
-    Q = 1.0 / (C + eta)
+    if disps_sens is not None:
+        ds = np.asarray(disps_sens, np.float64).reshape(-1, HW)[kx]
+        d = np.asarray(disps, np.float64).reshape(-1, HW)[kx]
+        m = (ds > 0).astype(np.float64)
+        Q = 1.0 / (C + m * alpha + (1.0 - m) * eta)
+        w = w - m * alpha * (d - ds)
+    else:
+        Q = 1.0 / (C + eta)
     Ei = np.zeros((P, D, HW))
     for e in range(E):
         s = ii[e] - t0
@@ -278,12 +295,12 @@ def back_substitute(sys, dx, t0, t1, t0_skip=True):
 
 
 def ba(poses, disps, intr, targets, weights, eta, ii, jj, t0, t1, iterations=2,
-       lm=1e-4, ep=0.1, motion_only=False, t0_skip=True, return_info=False):
+       lm=1e-4, ep=0.1, motion_only=False, t0_skip=True, return_info=False, disps_sens=None):
     """Oracle of ``droid_backends.ba`` (src/droid.cpp:141-168 -> src/droid_kernels.cu:1820-1937).
 
     Works on float64 copies; returns (poses, disps, dx, dz[, info]) where dx
     [P,6] and dz [F,HW] are the last iteration's updates (dz None when
-    motion_only)."""
+    motion_only).  ``disps_sens`` [N, ht, wd]: the sensor-disparity prior of reduced_system."""
     poses = np.array(poses, np.float64)
     disps = np.array(disps, np.float64)
     check_args(poses, disps, ii, jj, t0, t1)
@@ -294,7 +311,7 @@ def ba(poses, disps, intr, targets, weights, eta, ii, jj, t0, t1, iterations=2,
     infos = []
     for _ in range(iterations):
         lin = linearize(poses, disps, intr, targets, weights, ii, jj)
-        sys = reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only)
+        sys = reduced_system(lin, eta, ii, jj, t0, t1, HW, motion_only, disps=disps, disps_sens=disps_sens)
         x, ok = chol_solve(damp(sys['L'], lm, ep), sys['rhs'])
         infos.append(ok)
         dx = x.reshape(P, 6)

@@ -190,3 +190,31 @@ def test_ba_global300_first_iteration(db):
     print('global300 it1', rel_err(dx, rdx), rel_err(dz, rdz))
     assert ok == 1
     assert rel_err(dx, rdx) <= 1e-4 and rel_err(dz, rdz) <= 1e-4
+
+
+def test_ba_sensor_disparity_form(db):
+    """The 15-argument ba(poses, disps, intrinsics, disps_sens, targets, ...): all-zero sensor
+    disparities give the 14-argument results bit for bit; random ones (60 % of the pixels) match the
+    float64 oracle's prior (1e-4 relative).  The prior itself has no counterpart in this reference
+    (parity pinned by the oracle's closed form only)."""
+    p = synth.ba_small()
+    rng = np.random.default_rng(8)
+    sens = np.where(rng.uniform(size=p.disps.shape) < 0.6, rng.uniform(0.2, 1.0, size=p.disps.shape),
+                    0.0).astype(np.float32)
+    poses, disps, intr, tg, wt, eta, ii, jj = to_dev(p)
+    dx, dz, _ = db.ba(poses, disps, intr, tg, wt, eta, ii, jj, p.t0, p.t1, 2, p.lm, p.ep, False)
+    poses2, disps2, _, _, _, _, _, _ = to_dev(p)
+    dx2, dz2, _ = db.ba(poses2, disps2, intr, torch.zeros_like(disps2), tg, wt, eta, ii, jj, p.t0, p.t1, 2,
+                        p.lm, p.ep, False)
+    for a, b in ((poses, poses2), (disps, disps2), (dx, dx2), (dz, dz2)):
+        assert torch.equal(a, b)
+    poses3, disps3, _, _, _, _, _, _ = to_dev(p)
+    dx3, dz3, _ = db.ba(poses3, disps3, intr, torch.from_numpy(sens).cuda(), tg, wt, eta, ii, jj, p.t0, p.t1, 2,
+                        p.lm, p.ep, False)
+    rp, rd, rdx, rdz = oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta, p.ii, p.jj, p.t0,
+                              p.t1, 2, p.lm, p.ep, disps_sens=sens)
+    errs = [rel_err(poses3.cpu().numpy(), rp), rel_err(disps3.cpu().numpy(), rd), rel_err(dx3.cpu().numpy(), rdx),
+            rel_err(dz3.cpu().numpy(), rdz)]
+    print('sensor-prior ba errors (poses, disps, dx, dz):', errs)
+    assert max(errs) <= 1e-4
+    assert not torch.equal(disps3, disps)

@@ -242,3 +242,25 @@ def test_oracle_reproduces_golden_fixture():
                                   f['eta'], f['ii'], f['jj'], int(t0), int(t1), int(it), lm, ep)
     for a, b in ((poses, f['out_poses']), (disps, f['out_disps']), (dx, f['out_dx']), (dz, f['out_dz'])):
         assert np.allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+def test_sensor_prior_closed_form_and_zero_identity():
+    """The 15-argument form's sensor-disparity prior (no counterpart in this reference): with every
+    weight zero the cameras see nothing, dx = 0, and each pixel with a sensor disparity moves exactly
+    onto it (dz = w / C = -alpha (d - d_sens) / alpha) while the others stay put; an all-zero
+    disps_sens is the 14-argument call."""
+    p = synth.ba_small(seed=21, h=6, w=8)
+    rng = np.random.default_rng(4)
+    sens = np.where(rng.uniform(size=p.disps.shape) < 0.6, rng.uniform(0.2, 1.0, size=p.disps.shape), 0.0)
+    zeros = np.zeros_like(p.weights)
+    poses, disps, dx, dz = oba.ba(p.poses, p.disps, p.intrinsics, p.targets, zeros, p.eta, p.ii, p.jj,
+                                  p.t0, p.t1, 1, p.lm, p.ep, disps_sens=sens)
+    assert np.abs(dx).max() == 0.0
+    m = sens > 0
+    assert np.allclose(disps[m], sens[m], rtol=0, atol=1e-12)
+    assert np.array_equal(disps[~m], p.disps.astype(np.float64)[~m])
+    a = oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta, p.ii, p.jj, p.t0, p.t1, 2, p.lm, p.ep)
+    b = oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta, p.ii, p.jj, p.t0, p.t1, 2, p.lm, p.ep,
+               disps_sens=np.zeros_like(p.disps))
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
