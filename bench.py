@@ -657,34 +657,67 @@ def lookup_traffic():
     return None
 
 
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for ln in fh:
+                if ln.startswith('model name'):
+                    return ln.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(p, fm32, coords):
-    """The float64 oracle on the host: one full GN iteration of global300 plus the
-    lookup of 10 of the 3000 edges, scaled to one GN iteration + one full lookup."""
+    """The reference's algorithm on the host (oracle/torch_cpu.py: ba_cuda's semantics in torch
+    float32 with index_add_ segment sums and an FP64 Cholesky; CorrBlock as matmul + avg_pool2d +
+    grid_sample), on all host threads: one warm-up, then the median of 5 GN iterations at each of
+    the 4-keyframe (C1), frontend-window (C2) and global300 (C3) problems, and the lookup of 10 of
+    the 3000 global300 edges (median of 3) scaled to all of them.  value = GN iterations/s of the
+    global300 step (one GN iteration + the lookup of all edges).  The float64 numpy oracle's
+    global300 GN iteration is timed once beside it."""
     import numpy as np
     import torch
+    from hislam2_amd import synth
     sys.path.insert(0, ROOT)
     from oracle import ba as oba
-    from oracle import corr as oc
-    log('cpu baseline: oracle GN iteration')
-    t0 = time.perf_counter()
-    oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta, p.ii, p.jj, p.t0, p.t1, 1,
-           p.lm, p.ep)
-    t_ba = time.perf_counter() - t0
+    from oracle import torch_cpu as tcpu
+    torch.set_num_threads(os.cpu_count() or 1)
+    threads = torch.get_num_threads()
+
+    def median_time(fn, reps):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    ba_s = {}
+    for name in ('small', 'window', 'global300'):
+        q = p if name == 'global300' else synth.PROBLEMS[name]()
+        log('cpu baseline: torch GN iteration,', name)
+        ba_s[name] = median_time(lambda: tcpu.ba(q.poses, q.disps, q.intrinsics, q.targets, q.weights, q.eta, q.ii,
+                                                 q.jj, q.t0, q.t1, 1, q.lm, q.ep), 5)
     n = 10
-    fm = fm32[0].float().cpu().numpy()
-    f1 = fm[p.ii[:n]][None].astype(np.float64)
-    f2 = fm[p.jj[:n]][None].astype(np.float64)
-    pyr = oc.pyramid(oc.corr_allpairs(f1, f2), 4)
-    c = coords[:, :n].cpu().numpy().astype(np.float64)
-    log('cpu baseline: oracle lookup of', n, 'edges')
-    t1 = time.perf_counter()
-    oc.corrblock_lookup(pyr, c, 3)
-    t_lk = time.perf_counter() - t1
-    per_iter = t_ba + t_lk * len(p.ii) / n
-    return {'value': round(1.0 / per_iter, 6), 'unit': 'GN it/s', 'cores': 1, 'kind': 'port',
-            'sample': f'float64 numpy oracle: 1 GN iteration of global300 ({t_ba:.1f} s) + '
-                      f'CorrBlock lookup of {n}/3000 edges ({t_lk:.2f} s, scaled x{len(p.ii) // n}); '
-                      f'host {platform.processor() or platform.machine()}, os.cpu_count()={os.cpu_count()}'}
+    fm = fm32[0].float().cpu()
+    f1, f2 = fm[p.ii[:n]][None], fm[p.jj[:n]][None]
+    c = coords[:, :n].float().cpu()
+    log('cpu baseline: torch lookup of', n, 'edges')
+    lk = median_time(lambda: tcpu.corrblock_lookup(f1, f2, c), 3)
+    per_iter = ba_s['global300'] + lk * len(p.ii) / n
+    log('cpu baseline: float64 oracle GN iteration (global300)')
+    t0 = time.perf_counter()
+    oba.ba(p.poses, p.disps, p.intrinsics, p.targets, p.weights, p.eta, p.ii, p.jj, p.t0, p.t1, 1, p.lm, p.ep)
+    t_or = time.perf_counter() - t0
+    return {'value': round(1.0 / per_iter, 6), 'unit': 'GN it/s', 'cores': threads, 'kind': 'port',
+            'sample': f'torch float32 on {threads} host threads ({_cpu_model()}): median of 5 global300 GN '
+                      f'iterations ({ba_s["global300"]:.3f} s) + CorrBlock lookup of {n}/3000 edges '
+                      f'(median of 3, {lk:.3f} s, scaled x{len(p.ii) // n})',
+            'ba_ms_per_iteration': {k: round(1e3 * v, 3) for k, v in ba_s.items()},
+            'lookup_ms_per_edge': round(1e3 * lk / n, 3),
+            'oracle_float64_global300_iteration_s': round(t_or, 2)}
 
 
 if __name__ == '__main__':
