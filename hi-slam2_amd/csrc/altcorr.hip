@@ -20,8 +20,15 @@ namespace {
 constexpr int kPix = 64;
 constexpr int kThreads = 256;
 
-// ---- layout conversion [rows][C][P] -> [rows][P][C] ----------------------------
-template <typename T>
+// ---- layout conversion [rows][C][P] -> [rows][P][C] (kTiles: fragment tiles) ----------------
+// Offset of (pixel p, channel c) in one row of the converted layout.
+template <typename T, bool kTiles>
+__device__ __forceinline__ int64_t cl_offset(int p, int c, int C) {
+  if constexpr (kTiles) return frag_tile_offset<T>(p, c, C);
+  return (int64_t)p * C + c;
+}
+
+template <typename T, bool kTiles>
 __global__ void __launch_bounds__(256)
 channels_last_kernel(const T* __restrict__ in, T* __restrict__ out, int C, int P) {
   __shared__ T tile[32][33];
@@ -37,14 +44,14 @@ channels_last_kernel(const T* __restrict__ in, T* __restrict__ out, int C, int P
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
     const int p = p0 + k, c = c0 + tx;
-    if (c < C && p < P) dst[(int64_t)p * C + c] = tile[tx][k];
+    if (c < C && p < P) dst[cl_offset<T, kTiles>(p, c, C)] = tile[tx][k];
   }
 }
 
 // The same conversion for P and C multiples of 64 and 16-byte aligned buffers: one block per
 // 64 pixels x 64 channels, 16-byte loads along the pixels of a channel (one 128-byte line per
 // channel row in bf16) and 16-byte stores along the channels of a pixel, transposed in LDS.
-template <typename T>
+template <typename T, bool kTiles>
 __global__ void __launch_bounds__(256)
 channels_last_vec_kernel(const T* __restrict__ in, T* __restrict__ out, int C, int P) {
   constexpr int VE = 16 / sizeof(T);               // elements per 16-byte vector
@@ -72,7 +79,7 @@ channels_last_vec_kernel(const T* __restrict__ in, T* __restrict__ out, int C, i
     T* e = reinterpret_cast<T*>(&v);
 #pragma unroll
     for (int k = 0; k < VE; ++k) e[k] = tile[p][cv + k];
-    *reinterpret_cast<uint4*>(dst + (int64_t)(p0 + p) * C + c0 + cv) = v;
+    *reinterpret_cast<uint4*>(dst + cl_offset<T, kTiles>(p0 + p, c0 + cv, C)) = v;
   }
 }
 
@@ -429,37 +436,58 @@ int check_alt(const void* f1, const void* f2, const float* coords, const int64_t
 
 using namespace hsl2;
 
-extern "C" int hsl2_channels_last(const void* in, void* out, int dtype, int64_t rows,
-                                  int64_t channels, int64_t pixels, void* stream) {
-  HSL2_REQUIRE(in && out, HSL2_ERR_ARGUMENT, "channels_last: null pointer");
-  HSL2_REQUIRE(rows >= 0 && channels > 0 && pixels > 0 && rows < 65536, HSL2_ERR_SHAPE,
-               "channels_last: bad sizes");
+namespace hsl2 {
+namespace {
+
+template <bool kTiles>
+int convert_channel_major(const void* in, void* out, int dtype, int64_t rows, int64_t channels, int64_t pixels,
+                          void* stream, const char* what) {
+  HSL2_REQUIRE(in && out, HSL2_ERR_ARGUMENT, std::string(what) + ": null pointer");
+  HSL2_REQUIRE(rows >= 0 && channels > 0 && pixels > 0 && rows < 65536 && channels * pixels < (1ll << 31),
+               HSL2_ERR_SHAPE, std::string(what) + ": bad sizes");
+  HSL2_REQUIRE(dtype == HSL2_F32 || dtype == HSL2_F16 || dtype == HSL2_BF16, HSL2_ERR_ARGUMENT,
+               std::string(what) + ": unsupported dtype");
+  if (kTiles) {
+    const int G = dtype == HSL2_F32 ? frag_group<float>() : frag_group<__half>();
+    HSL2_REQUIRE(pixels % kFragPixels == 0 && channels % G == 0, HSL2_ERR_UNSUPPORTED,
+                 std::string(what) + ": pixels must be a multiple of 32 and channels of 8 (f32) or 16");
+  }
   if (rows == 0) return HSL2_OK;
-  dim3 grid(static_cast<unsigned>((pixels + 31) / 32), static_cast<unsigned>((channels + 31) / 32),
-            static_cast<unsigned>(rows));
   hipStream_t s = as_stream(stream);
   const int C = static_cast<int>(channels), P = static_cast<int>(pixels);
   const bool vec = channels % 64 == 0 && pixels % 64 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
                    reinterpret_cast<uintptr_t>(out) % 16 == 0;
-  if (vec && dtype != HSL2_F64) {
+  if (vec) {
     const dim3 g(static_cast<unsigned>(pixels / 64), static_cast<unsigned>(channels / 64), static_cast<unsigned>(rows));
     switch (dtype) {
-      case HSL2_F32: hipLaunchKernelGGL(channels_last_vec_kernel<float>, g, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float*>(out), C, P); break;
-      case HSL2_F16: hipLaunchKernelGGL(channels_last_vec_kernel<__half>, g, dim3(256), 0, s, static_cast<const __half*>(in), static_cast<__half*>(out), C, P); break;
-      case HSL2_BF16: hipLaunchKernelGGL(channels_last_vec_kernel<__hip_bfloat16>, g, dim3(256), 0, s, static_cast<const __hip_bfloat16*>(in), static_cast<__hip_bfloat16*>(out), C, P); break;
-      default: set_error("channels_last: unsupported dtype"); return HSL2_ERR_ARGUMENT;
+      case HSL2_F32: hipLaunchKernelGGL((channels_last_vec_kernel<float, kTiles>), g, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float*>(out), C, P); break;
+      case HSL2_F16: hipLaunchKernelGGL((channels_last_vec_kernel<__half, kTiles>), g, dim3(256), 0, s, static_cast<const __half*>(in), static_cast<__half*>(out), C, P); break;
+      default: hipLaunchKernelGGL((channels_last_vec_kernel<__hip_bfloat16, kTiles>), g, dim3(256), 0, s, static_cast<const __hip_bfloat16*>(in), static_cast<__hip_bfloat16*>(out), C, P); break;
     }
-    HSL2_LAUNCH_CHECK();
-    return HSL2_OK;
-  }
-  switch (dtype) {
-    case HSL2_F32: hipLaunchKernelGGL(channels_last_kernel<float>, grid, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float*>(out), C, P); break;
-    case HSL2_F16: hipLaunchKernelGGL(channels_last_kernel<__half>, grid, dim3(256), 0, s, static_cast<const __half*>(in), static_cast<__half*>(out), C, P); break;
-    case HSL2_BF16: hipLaunchKernelGGL(channels_last_kernel<__hip_bfloat16>, grid, dim3(256), 0, s, static_cast<const __hip_bfloat16*>(in), static_cast<__hip_bfloat16*>(out), C, P); break;
-    default: set_error("channels_last: unsupported dtype"); return HSL2_ERR_ARGUMENT;
+  } else {
+    const dim3 g(static_cast<unsigned>((pixels + 31) / 32), static_cast<unsigned>((channels + 31) / 32),
+                 static_cast<unsigned>(rows));
+    switch (dtype) {
+      case HSL2_F32: hipLaunchKernelGGL((channels_last_kernel<float, kTiles>), g, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float*>(out), C, P); break;
+      case HSL2_F16: hipLaunchKernelGGL((channels_last_kernel<__half, kTiles>), g, dim3(256), 0, s, static_cast<const __half*>(in), static_cast<__half*>(out), C, P); break;
+      default: hipLaunchKernelGGL((channels_last_kernel<__hip_bfloat16, kTiles>), g, dim3(256), 0, s, static_cast<const __hip_bfloat16*>(in), static_cast<__hip_bfloat16*>(out), C, P); break;
+    }
   }
   HSL2_LAUNCH_CHECK();
   return HSL2_OK;
+}
+
+}  // namespace
+}  // namespace hsl2
+
+extern "C" int hsl2_channels_last(const void* in, void* out, int dtype, int64_t rows,
+                                  int64_t channels, int64_t pixels, void* stream) {
+  return convert_channel_major<false>(in, out, dtype, rows, channels, pixels, stream, "channels_last");
+}
+
+extern "C" int hsl2_fmap_tiles(const void* in, void* out, int dtype, int64_t rows, int64_t channels,
+                               int64_t pixels, void* stream) {
+  return convert_channel_major<true>(in, out, dtype, rows, channels, pixels, stream, "fmap_tiles");
 }
 
 extern "C" int hsl2_channels_first_f32(const float* in, float* out, int64_t rows,

@@ -123,6 +123,20 @@ __host__ __device__ inline int pyramid_layout(int level, int64_t hw1, int64_t h2
   return (h2 % 8 == 0 && w2 % 8 == 0) ? kPlaneTiles : kPlaneRows;
 }
 
+// ---- feature maps of the correlation build: MFMA fragment tiles -----------------------
+// Per frame, [pixels / 32][channels / G][32 pixels][G channels] with G = 32 / sizeof(T) (two
+// 16-byte pieces of each pixel's channels): the operand fragment of one 32-row MFMA step, 32
+// pixels x G channels, is 1 KB of contiguous memory, so that one wave's 16-byte-per-lane load
+// covers 8 whole cache lines instead of 32 quarter lines.
+template <typename T> __host__ __device__ constexpr int frag_group() { return 32 / static_cast<int>(sizeof(T)); }
+constexpr int kFragPixels = 32;
+
+template <typename T>
+__host__ __device__ inline int64_t frag_tile_offset(int64_t p, int c, int C) {
+  constexpr int G = frag_group<T>();
+  return ((p / kFragPixels) * (C / G) + c / G) * (kFragPixels * G) + (p % kFragPixels) * G + c % G;
+}
+
 __host__ __device__ inline int64_t pyramid_offset(int layout, int64_t pix, int y, int x, int w2, int64_t plane) {
   const int64_t g0 = (pix - pix % kPlaneGroup) * plane, pg = pix % kPlaneGroup;
   if (layout == kPlaneGroups24)

@@ -6,14 +6,16 @@
 // over q's (h, w) grid.
 //
 // One 256-thread block computes a 32 (p) x 8 image rows x WT columns (q) tile with
-// v_mfma_f32_32x32x{8,8,2} (bf16 / f16 / f32 operands, f32 accumulation).  The feature maps
-// arrive channel-last ([pixel][channel]), so every lane fetches its MFMA operand fragment
-// (4 consecutive channels of one pixel: 8 bytes; 1 channel for f32) straight from memory:
-// no LDS staging and no barrier in the channel loop.  Each wave owns four 32x32 output
-// blocks.  Blocks are ordered so that each XCD works through whole edges (the operands of
-// one edge stay in that XCD's L2).  The finished f32 tile is staged in LDS and written as
-// level 0 plus every pooled level it fully contains (8 rows cover 2x2, 4x4 and 8x8 pooling),
-// so the volume is written once and never read back.
+// v_mfma_f32_32x32x16_{bf16,f16} (f32 operands: v_mfma_f32_32x32x2f32), f32 accumulation.  The
+// feature maps arrive in MFMA fragment tiles (frag_tile_offset: 32 pixels x G channels
+// contiguous), so every lane fetches its operand fragment (8 consecutive channels of one pixel,
+// 16 bytes) straight from memory and one wave's fetch is 1 KB of contiguous memory: no LDS
+// staging and no barrier in the channel loop, which for 128 channels keeps three channel groups
+// of fetches in flight.  Each wave owns four 32x32 output blocks.  Blocks are ordered so that
+// each XCD works through whole edges (the operands of one edge stay in that XCD's L2).  The
+// finished f32 tile is staged in LDS and written, with non-temporal stores, as level 0 plus every
+// pooled level it fully contains (8 rows cover 2x2, 4x4 and 8x8 pooling), so the volume is
+// written once and never read back.
 #include "common.hpp"
 
 namespace hsl2 {
@@ -22,46 +24,42 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTP = 32;       // p rows per MFMA operand
 constexpr int kER = 16;       // p rows staged in LDS and emitted per pass (two passes per operand)
-constexpr int kPH = 1;        // 32-row operands per block (2 shares every fmap2 fragment twice but
-                              // halves occupancy; measured no faster at 48x64, 128 channels)
 constexpr int kRows = 8;      // image-2 rows per block
 
 typedef float acc16 __attribute__((ext_vector_type(16)));
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 // Operand fragments are fetched 16 bytes per lane.  The sum over channels is order-free, so
-// within a group of channels the lanes of the two MFMA k-halves take contiguous 16-byte
-// pieces: for bf16 / f16 a group is 16 channels, lane half kg loads [8 kg, 8 kg + 8) and feeds
-// its first 4 values to one MFMA and the other 4 to the next; for f32 a group is 8 channels,
-// lane half kg loads [4 kg, 4 kg + 4) and feeds one value to each of four MFMAs.  A and B use
-// the same assignment, so every channel is multiplied with its own partner exactly once.
+// within a group of channels the lanes of the two MFMA k-halves take contiguous 16-byte pieces:
+// for bf16 / f16 a group is 16 channels, lane half kg loads [8 kg, 8 kg + 8), one
+// v_mfma_f32_32x32x16 per group; for f32 a group is 8 channels, lane half kg loads [4 kg, 4 kg + 4)
+// and feeds one value to each of four v_mfma_f32_32x32x2f32.  A and B use the same assignment,
+// so every channel is multiplied with its own partner exactly once.
 template <typename T> struct Mfma;
 template <> struct Mfma<__hip_bfloat16> {
-  static constexpr int G = 16;                     // channels per 16-byte group
-  static constexpr int S = 2;                      // MFMAs per group
-  struct alignas(16) Frag { bf16x4 v[2]; };
+  static constexpr int G = frag_group<__hip_bfloat16>();   // channels per group
+  static constexpr int S = 1;                               // MFMAs per group
+  typedef __bf16 Frag __attribute__((ext_vector_type(8)));
   static __device__ __forceinline__ Frag load(const __hip_bfloat16* p) { return *reinterpret_cast<const Frag*>(p); }
-  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int s, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a.v[s], b.v[s], c, 0, 0, 0);
+  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
   }
 };
 template <> struct Mfma<__half> {
-  static constexpr int G = 16;
-  static constexpr int S = 2;
-  struct alignas(16) Frag { f16x4 v[2]; };
+  static constexpr int G = frag_group<__half>();
+  static constexpr int S = 1;
+  typedef _Float16 Frag __attribute__((ext_vector_type(8)));
   static __device__ __forceinline__ Frag load(const __half* p) { return *reinterpret_cast<const Frag*>(p); }
-  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int s, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(a.v[s], b.v[s], c, 0, 0, 0);
+  static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int, acc16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
   }
 };
 template <> struct Mfma<float> {
-  static constexpr int G = 8;
+  static constexpr int G = frag_group<float>();
   static constexpr int S = 4;
-  struct alignas(16) Frag { float v[4]; };
+  typedef float Frag __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ Frag load(const float* p) { return *reinterpret_cast<const Frag*>(p); }
   static __device__ __forceinline__ acc16 run(const Frag& a, const Frag& b, int s, acc16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[s], b.v[s], c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], c, 0, 0, 0);
   }
 };
 
@@ -74,6 +72,31 @@ template <typename To, int CW>
 struct alignas(CW * sizeof(To)) OutV {
   To v[CW];
 };
+
+// Non-temporal store of one chunk: the volume is written once and only read later by the
+// lookup, so streaming it past the L2 keeps the feature-map operands resident there.
+template <typename To, int CW>
+__device__ __forceinline__ void store_nt(To* dst, const OutV<To, CW>& o) {
+  constexpr int B = CW * static_cast<int>(sizeof(To));
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  if constexpr (B == 32) {
+    struct Two { u32x4 a, b; };
+    const Two t = __builtin_bit_cast(Two, o);
+    __builtin_nontemporal_store(t.a, reinterpret_cast<u32x4*>(dst));
+    __builtin_nontemporal_store(t.b, reinterpret_cast<u32x4*>(dst) + 1);
+  } else if constexpr (B == 16) {
+    __builtin_nontemporal_store(__builtin_bit_cast(u32x4, o), reinterpret_cast<u32x4*>(dst));
+  } else if constexpr (B == 8) {
+    __builtin_nontemporal_store(__builtin_bit_cast(u32x2, o), reinterpret_cast<u32x2*>(dst));
+  } else if constexpr (B == 4) {
+    __builtin_nontemporal_store(__builtin_bit_cast(unsigned, o), reinterpret_cast<unsigned*>(dst));
+  } else if constexpr (B == 2) {
+    __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, o), reinterpret_cast<unsigned short*>(dst));
+  } else {
+    *reinterpret_cast<OutV<To, CW>*>(dst) = o;
+  }
+}
 
 // Pooled level (hierarchical: each level averages 2x2 of the level above it, as repeated
 // avg_pool2d does).  src holds the level above for the tile ([kER][RS][WS] floats, pitch PS per
@@ -129,14 +152,13 @@ __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* 
             OutV<To, 4> o4;
 #pragma unroll
             for (int e = 0; e < 4; ++e) o4.v[e] = o.v[4 * hseg + e];
-            *reinterpret_cast<OutV<To, 4>*>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x + 4 * hseg,
-                                                                 wl, plane_stride)) = o4;
+            store_nt<To, 4>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x + 4 * hseg, wl, plane_stride),
+                            o4);
           }
         }
       } else {
         // a kPlaneGroups level has CW == 8 and gx a multiple of 8: the chunk is one segment
-        *reinterpret_cast<OutV<To, CW>*>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl,
-                                                              plane_stride)) = o;
+        store_nt<To, CW>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl, plane_stride), o);
       }
     }
   }
@@ -184,7 +206,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
     OutV<To, 8> o;
 #pragma unroll
     for (int e = 0; e < 8; ++e) o.v[e] = from_f<To>(src[e]);
-    *reinterpret_cast<OutV<To, 8>*>(l0 + pyramid_offset(lay0, pl0 + p, y0 + y, x0 + x, w, HW)) = o;
+    store_nt<To, 8>(l0 + pyramid_offset(lay0, pl0 + p, y0 + y, x0 + x, w, HW), o);
   }
   if (levels < 2) {
     __syncthreads();
@@ -226,10 +248,13 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
 }
 
 // 1-D grid over (p tile, q tile, edge), p tile fastest; see the block order note above.
-// A block covers kPH x 32 p rows (kPH MFMA operands sharing every fmap2 fragment) x 8 image
-// rows x WT columns; WT is a compile-time constant so that every index split is a shift or a
-// constant division.
-template <typename Ti, typename To, int WT>
+// A block covers 32 p rows x 8 image rows x WT columns; WT is a compile-time constant so that
+// every index split is a shift or a constant division.  NG > 0: the channel-group count C / G
+// as a compile-time constant (the channel loop is unrolled with kPipe groups of fetches in
+// flight); NG == 0: any multiple of G, one group at a time.
+constexpr int kPipe = 3;
+
+template <typename Ti, typename To, int WT, int NG>
 __global__ void __launch_bounds__(kThreads)
 corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __restrict__ l0,
                   To* __restrict__ l1, To* __restrict__ l2, To* __restrict__ l3, int C, int h,
@@ -237,6 +262,10 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   constexpr int NQ = kRows * WT;                   // columns of the tile
   constexpr int SQ = NQ + 4;                       // padded LDS row of the f32 tile
   constexpr int PW = NQ / 32 / 4;                  // 32-wide MFMA blocks per wave
+  using M = Mfma<Ti>;
+  using Frag = typename M::Frag;
+  constexpr int G = M::G;
+  constexpr int FT = kFragPixels * G;              // elements of one fragment tile
   extern __shared__ __align__(16) unsigned char smem[];
   float* sout = reinterpret_cast<float*>(smem);    // [kER][SQ]
 
@@ -250,80 +279,95 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   const int n = static_cast<int>(lb / ((int64_t)ptiles * qtiles));
 
   const int HW = h * w;
-  const int p0 = pt * kPH * kTP;
+  const int p0 = pt * kTP;
   const int qtiles_x = w / WT;
   const int y0 = (qt / qtiles_x) * kRows;
   const int x0 = (qt % qtiles_x) * WT;
   const int t = threadIdx.x;
   const int wave = t / kWave, lane = t % kWave;
-  constexpr int G = Mfma<Ti>::G;
-  // lane i of the 32-row MFMA operand; lanes 32-63 carry the other half of each channel group
+  // lane (i, kg): pixel i of a 32-pixel fragment tile, channels [kg G / 2, (kg + 1) G / 2) of
+  // each group; group g of the tile starting at pixel P (a multiple of 32) is at P C + g FT
   const int i = lane % 32, kg = lane / 32;
-  const Ti* ap0 = f1 + ((int64_t)n * HW + p0 + i) * C + kg * (G / 2);
-  const Ti* ap1 = ap0 + (int64_t)(kPH - 1) * kTP * C;
+  const int lane_off = i * G + kg * (G / 2);
+  const Ti* ap = f1 + ((int64_t)n * HW + p0) * C + lane_off;
   const Ti* bp[PW];
 #pragma unroll
   for (int k = 0; k < PW; ++k) {
-    const int q = (wave * PW + k) * 32 + i;
-    const int qpix = (y0 + q / WT) * w + x0 + q % WT;
-    bp[k] = f2 + ((int64_t)n * HW + qpix) * C + kg * (G / 2);
+    // the 32 columns of MFMA block k are 32 consecutive pixels starting at a multiple of 32
+    // (WT is 64 or the whole width)
+    const int q0 = (wave * PW + k) * 32;
+    const int qpix0 = (y0 + q0 / WT) * w + x0 + q0 % WT;
+    bp[k] = f2 + ((int64_t)n * HW + qpix0) * C + lane_off;
   }
 
-  acc16 acc0[PW], acc1[PW];
+  acc16 acc[PW];
 #pragma unroll
   for (int k = 0; k < PW; ++k)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      acc0[k][v] = 0.f;
-      acc1[k][v] = 0.f;
+    for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
+  if constexpr (NG > 0) {
+    constexpr int D = NG < kPipe ? NG : kPipe;
+    Frag fa[D], fb[D][PW];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      fa[d] = M::load(ap + d * FT);
+#pragma unroll
+      for (int k = 0; k < PW; ++k) fb[d][k] = M::load(bp[k] + d * FT);
     }
-#pragma unroll 2
-  for (int kk = 0; kk < C; kk += G) {
-    const typename Mfma<Ti>::Frag a0 = Mfma<Ti>::load(ap0 + kk);
-    typename Mfma<Ti>::Frag a1;
-    if constexpr (kPH == 2) a1 = Mfma<Ti>::load(ap1 + kk);
-    typename Mfma<Ti>::Frag bf[PW];
 #pragma unroll
-    for (int k = 0; k < PW; ++k) bf[k] = Mfma<Ti>::load(bp[k] + kk);
+    for (int g = 0; g < NG; ++g) {
+      const int sl = g % D;
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int st = 0; st < Mfma<Ti>::S; ++st)
+      for (int st = 0; st < M::S; ++st)
 #pragma unroll
-      for (int k = 0; k < PW; ++k) {
-        acc0[k] = Mfma<Ti>::run(a0, bf[k], st, acc0[k]);
-        if constexpr (kPH == 2) acc1[k] = Mfma<Ti>::run(a1, bf[k], st, acc1[k]);
+        for (int k = 0; k < PW; ++k) acc[k] = M::run(fa[sl], fb[sl][k], st, acc[k]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (g + D < NG) {
+        fa[sl] = M::load(ap + (g + D) * FT);
+#pragma unroll
+        for (int k = 0; k < PW; ++k) fb[sl][k] = M::load(bp[k] + (g + D) * FT);
       }
+    }
+  } else {
+    for (int g = 0; g < C / G; ++g) {
+      const Frag a = M::load(ap + g * FT);
+      Frag bf[PW];
+#pragma unroll
+      for (int k = 0; k < PW; ++k) bf[k] = M::load(bp[k] + g * FT);
+#pragma unroll
+      for (int st = 0; st < M::S; ++st)
+#pragma unroll
+        for (int k = 0; k < PW; ++k) acc[k] = M::run(a, bf[k], st, acc[k]);
+    }
   }
 
-  // each group of 32 p rows through the same LDS tile; MFMA output layout: value v of lane l
-  // is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32
+  // the 32 p rows through the LDS tile 16 at a time; MFMA output layout: value v of lane l
+  // is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32 (rows 0-15 are values 0-7 of each
+  // lane, rows 16-31 values 8-15)
 #pragma unroll
-  for (int half = 0; half < kPH; ++half) {
-    // rows 0-15 of the operand are values 0-7 of each lane, rows 16-31 values 8-15
+  for (int part = 0; part < kTP / kER; ++part) {
 #pragma unroll
-    for (int part = 0; part < kTP / kER; ++part) {
+    for (int k = 0; k < PW; ++k) {
+      const int q = (wave * PW + k) * 32 + lane % 32;
 #pragma unroll
-      for (int k = 0; k < PW; ++k) {
-        const int q = (wave * PW + k) * 32 + lane % 32;
-#pragma unroll
-        for (int v = 8 * part; v < 8 * part + 8; ++v) {
-          const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4) - kER * part;
-          sout[p * SQ + q] = (half ? acc1[k][v] : acc0[k][v]) * 0.0625f;
-        }
+      for (int v = 8 * part; v < 8 * part + 8; ++v) {
+        const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4) - kER * part;
+        sout[p * SQ + q] = acc[k][v] * 0.0625f;
       }
-      __syncthreads();
-      emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + half * kTP + part * kER, h, w, y0,
-                        x0, levels, tiled);
     }
+    __syncthreads();
+    emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + part * kER, h, w, y0, x0, levels, tiled);
   }
 }
 
-template <typename Ti, typename To, int WT>
+template <typename Ti, typename To, int WT, int NG>
 int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
               int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
   constexpr int NQ = kRows * WT, SQ = NQ + 4;
   const size_t lds = sizeof(float) * kER * SQ;
   HSL2_REQUIRE(lds <= 160 * 1024, HSL2_ERR_UNSUPPORTED, "corr build: LDS budget exceeded");
-  const int ptiles = static_cast<int>(h * w / (kPH * kTP));
+  const int ptiles = static_cast<int>(h * w / kTP);
   const int qtiles = static_cast<int>((h / kRows) * (w / WT));
   const int64_t nblocks = (int64_t)ptiles * qtiles * batch;
   if (nblocks == 0) return HSL2_OK;
@@ -338,7 +382,7 @@ int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch
   }
   To* l[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < L; ++i) l[i] = static_cast<To*>(levels[i]);
-  hipLaunchKernelGGL((corr_build_kernel<Ti, To, WT>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
+  hipLaunchKernelGGL((corr_build_kernel<Ti, To, WT, NG>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
                      static_cast<const Ti*>(f1), static_cast<const Ti*>(f2), l[0], l[1], l[2], l[3],
                      static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), L, tiled, ptiles, qtiles,
                      nblocks);
@@ -347,15 +391,25 @@ int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch
 }
 
 // tile widths: 64 for widths that are multiples of 64, else the width itself (16, 32 or 48)
+template <typename Ti, typename To, int NG>
+int launch_ng(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
+              int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
+  if (w % 64 == 0) return launch_wt<Ti, To, 64, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 48) return launch_wt<Ti, To, 48, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 32) return launch_wt<Ti, To, 32, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 16) return launch_wt<Ti, To, 16, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  set_error("corr build: width must be a multiple of 64, or 16, 32 or 48");
+  return HSL2_ERR_UNSUPPORTED;
+}
+
+// 128 channels (the update network's feature maps) get the unrolled, pipelined channel loop
 template <typename Ti, typename To>
 int launch(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
            int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
-  if (w % 64 == 0) return launch_wt<Ti, To, 64>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  if (w == 48) return launch_wt<Ti, To, 48>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  if (w == 32) return launch_wt<Ti, To, 32>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  if (w == 16) return launch_wt<Ti, To, 16>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  set_error("corr build: width must be a multiple of 64, or 16, 32 or 48");
-  return HSL2_ERR_UNSUPPORTED;
+  HSL2_REQUIRE(C % Mfma<Ti>::G == 0, HSL2_ERR_UNSUPPORTED, "corr build: channel count not a multiple of the group");
+  constexpr int kNG128 = 128 / Mfma<Ti>::G;
+  if (C == 128) return launch_ng<Ti, To, kNG128>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  return launch_ng<Ti, To, 0>(f1, f2, levels, batch, C, h, w, L, tiled, s);
 }
 
 }  // namespace
@@ -374,7 +428,7 @@ extern "C" int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtyp
   HSL2_REQUIRE(channels % 16 == 0, HSL2_ERR_UNSUPPORTED, "corr build: channels must be a multiple of 16");
   HSL2_REQUIRE(h % kRows == 0 && w % 8 == 0, HSL2_ERR_UNSUPPORTED,
                "corr build: h and w must be multiples of 8");
-  HSL2_REQUIRE(h * w < (1ll << 31) && (h * w) % (kPH * kTP) == 0, HSL2_ERR_SHAPE, "corr build: bad plane size");
+  HSL2_REQUIRE(h * w < (1ll << 31) && (h * w) % kTP == 0, HSL2_ERR_SHAPE, "corr build: bad plane size");
   hipStream_t s = as_stream(stream);
   if (out_dtype == HSL2_BF16) {
     if (in_dtype == HSL2_BF16) return launch<__hip_bfloat16, __hip_bfloat16>(fmap1, fmap2, levels, batch, channels, h, w, num_levels, tiled, s);

@@ -520,13 +520,13 @@ std::vector<torch::Tensor> corr_build(torch::Tensor fmap1, torch::Tensor fmap2, 
     ptrs.push_back(levels.back().data_ptr());
   }
   int odc = od == torch::kBFloat16 ? HSL2_BF16 : od == torch::kFloat32 ? HSL2_F32 : od == torch::kFloat16 ? HSL2_F16 : -1;
-  // the kernel reads channel-last maps: [B*N, C, h*w] -> [B*N, h*w, C]
+  // the kernel reads the maps in MFMA fragment tiles: [B*N, C, h*w] -> [B*N, h*w/32, C/G, 32, G]
   torch::Tensor c1 = torch::empty_like(fmap1), c2 = torch::empty_like(fmap2);
-  raise_status(hsl2_channels_last(fmap1.data_ptr(), c1.data_ptr(), dtype_code(fmap1), B * N, C, h * w,
-                                  current_stream()),
+  raise_status(hsl2_fmap_tiles(fmap1.data_ptr(), c1.data_ptr(), dtype_code(fmap1), B * N, C, h * w,
+                               current_stream()),
                "corr_build");
-  raise_status(hsl2_channels_last(fmap2.data_ptr(), c2.data_ptr(), dtype_code(fmap2), B * N, C, h * w,
-                                  current_stream()),
+  raise_status(hsl2_fmap_tiles(fmap2.data_ptr(), c2.data_ptr(), dtype_code(fmap2), B * N, C, h * w,
+                               current_stream()),
                "corr_build");
   raise_status(hsl2_corr_build(c1.data_ptr(), c2.data_ptr(), dtype_code(fmap1), ptrs.data(), odc, B * N, C,
                                h, w, num_levels, tiled ? 1 : 0, current_stream()),

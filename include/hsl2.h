@@ -100,8 +100,9 @@ int hsl2_corr_lookup_conv(const void* const* levels, int num_levels, const float
 
 /* Replaces CorrBlock.__init__ + CorrBlock.corr (hislam2/modules/corr.py:24-38, 63-71):
  * all-pairs correlation (f1/4)^T (f2/4) of fmap1/fmap2 followed by `num_levels` 2x2
- * average-pool levels.  fmap1/fmap2 are CHANNEL-LAST [batch, h * w, C] (`in_dtype`
- * F32/BF16/F16; hsl2_channels_last converts the reference's [batch, C, h, w]).  levels[i]
+ * average-pool levels.  fmap1/fmap2 are in MFMA FRAGMENT TILES: per batch item
+ * [h w / 32][C / G][32 pixels][G channels], G = 16 for BF16/F16 and 8 for F32 (`in_dtype`;
+ * hsl2_fmap_tiles converts the reference's [batch, C, h, w]).  levels[i]
  * points at [batch, h, w, h >> i, w >> i] of `out_dtype` (F32/BF16).  Requires h, w multiples
  * of 8 and C a multiple of 16.  tiled == 0 gives the reference's row-major planes
  * (element (pix, y, x) at pix * h2 w2 + y w2 + x, pix = item * h w + pixel).  tiled != 0 (read
@@ -122,6 +123,11 @@ int hsl2_corr_build(const void* fmap1, const void* fmap2, int in_dtype, void* co
  * kernels read channel-last feature maps so that a tap's channel vector is contiguous. */
 int hsl2_channels_last(const void* in, void* out, int dtype, int64_t rows, int64_t channels,
                        int64_t pixels, void* stream);
+/* Layout helper for hsl2_corr_build: [rows, C, P] -> MFMA fragment tiles
+ * [rows, P / 32, C / G, 32, G] (G = 32 / element size: 16 for F16/BF16, 8 for F32), so that one
+ * MFMA operand fragment (32 pixels x G channels) is contiguous.  P a multiple of 32, C of G. */
+int hsl2_fmap_tiles(const void* in, void* out, int dtype, int64_t rows, int64_t channels,
+                    int64_t pixels, void* stream);
 /* Layout helper: [rows, P, C] -> [rows, C, P], f32. */
 int hsl2_channels_first_f32(const float* in, float* out, int64_t rows, int64_t channels,
                             int64_t pixels, void* stream);

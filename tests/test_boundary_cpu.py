@@ -45,6 +45,19 @@ def test_status_strings(lib):
     assert lib.hsl2_version() >= 1
 
 
+def test_fmap_tiles_rejects_unaligned_shapes(lib):
+    """The fragment-tile conversion needs whole 32-pixel tiles and whole channel groups (16 for
+    f16/bf16, 8 for f32); other shapes are refused before any GPU work."""
+    f = lib.hsl2_fmap_tiles
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                  ctypes.c_int64, ctypes.c_void_p]
+    buf = ctypes.c_void_p(16)                   # never dereferenced: every call below is refused
+    assert f(buf, buf, 2, 2, 128, 100, None) != 0          # 100 pixels: not whole tiles
+    assert f(buf, buf, 2, 2, 40, 96, None) != 0            # bf16, 40 channels: not whole groups
+    assert f(buf, buf, 5, 2, 128, 96, None) != 0           # unknown dtype
+    assert f(None, buf, 2, 2, 128, 96, None) != 0          # null pointer
+
+
 @pytest.fixture(scope='module')
 def db():
     from hislam2_amd import backend
