@@ -41,8 +41,7 @@ template <typename R>
 int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const float* intr,
                    const float* disps_sens, const float* targets, const float* weights, const float* eta,
                    int eta_rows, int iterations, float lm, float ep, float* dx, float* dz, char* ws,
-                   hipStream_t s) {
-  const int* blob = reinterpret_cast<const int*>(ws + p->w_blob);
+                   const int* blob, hipStream_t s) {
   float* Eall = reinterpret_cast<float*>(ws + p->w_Eall);
   float* Eii = reinterpret_cast<float*>(ws + p->w_Eii);
   float* Cii = reinterpret_cast<float*>(ws + p->w_Cii);
@@ -454,18 +453,23 @@ extern "C" int hsl2_ba_run_sens(const hsl2_ba_plan* plan, float* poses, float* d
   }
   hipStream_t s = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
-  HSL2_HIP(hipMemcpyAsync(ws + plan->w_blob, plan->blob.data(), plan->blob.size() * sizeof(int32_t),
-                          hipMemcpyHostToDevice, s));
-  if (plan->NPAIR == 0 && plan->NS == 0) {
-    // motion-only (or empty) plans read S/Ev through the assembly lists only when present
-  }
+  // the plan's tables: the registered device copy when it lives on this device, else uploaded
+  // into the workspace
+  const int* blob = reinterpret_cast<const int*>(ws + plan->w_blob);
+  int dev = -1;
+  if (plan->dev_blob) HSL2_HIP(hipGetDevice(&dev));
+  if (plan->dev_blob && dev == plan->dev_blob_device)
+    blob = plan->dev_blob;
+  else
+    HSL2_HIP(hipMemcpyAsync(ws + plan->w_blob, plan->blob.data(), plan->blob.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, s));
   int st;
   if (plan->precision == HSL2_BA_SPLIT)
     st = run_iterations<double>(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta,
-                                static_cast<int>(eta_rows), iterations, lm, ep, dx, dz, ws, s);
+                                static_cast<int>(eta_rows), iterations, lm, ep, dx, dz, ws, blob, s);
   else
     st = run_iterations<float>(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta,
-                               static_cast<int>(eta_rows), iterations, lm, ep, dx, dz, ws, s);
+                               static_cast<int>(eta_rows), iterations, lm, ep, dx, dz, ws, blob, s);
   if (st) return st;
   if (iterations == 0) HSL2_HIP(hipMemsetAsync(dx, 0, sizeof(float) * 6 * plan->P, s));
   if (solve_ok) {
@@ -643,5 +647,23 @@ extern "C" int hsl2_jdsa_solve(const float* C, const float* w, const float* disp
                      static_cast<int>(wd), static_cast<int>(hs), static_cast<int>(ws), alpha, disps_out, dscales_out,
                      dzcov);
   HSL2_LAUNCH_CHECK();
+  return HSL2_OK;
+}
+
+extern "C" size_t hsl2_ba_plan_table_bytes(const hsl2_ba_plan* plan) {
+  return plan ? plan->blob.size() * sizeof(int32_t) : 0;
+}
+
+extern "C" int hsl2_ba_plan_upload_tables(hsl2_ba_plan* plan, void* device_buffer, void* stream) {
+  HSL2_REQUIRE(plan && device_buffer, HSL2_ERR_ARGUMENT, "ba_plan_upload_tables: null pointer");
+  HSL2_REQUIRE(reinterpret_cast<uintptr_t>(device_buffer) % 256 == 0, HSL2_ERR_ARGUMENT,
+               "ba_plan_upload_tables: the buffer must be 256-byte aligned");
+  int dev = -1;
+  HSL2_HIP(hipGetDevice(&dev));
+  hipStream_t s = as_stream(stream);
+  HSL2_HIP(hipMemcpyAsync(device_buffer, plan->blob.data(), plan->blob.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+  plan->dev_blob = static_cast<const int32_t*>(device_buffer);
+  plan->dev_blob_device = dev;
   return HSL2_OK;
 }

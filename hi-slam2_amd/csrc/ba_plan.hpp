@@ -27,6 +27,10 @@ struct hsl2_ba_plan {
 
   // ---- int32 tables uploaded to the workspace (all in one blob) ----
   std::vector<int32_t> blob;
+  // a device copy of blob registered by hsl2_ba_plan_upload_tables and the device it lives on;
+  // runs on that device read the tables there instead of uploading blob into the workspace
+  const int32_t* dev_blob = nullptr;
+  int dev_blob_device = -1;
   // offsets (in int32 elements) of each table inside blob
   int64_t o_edge_ii = 0, o_edge_jj = 0;        // [E]
   int64_t o_edge_frame = 0;                    // [E] depth-frame index of ii[e]

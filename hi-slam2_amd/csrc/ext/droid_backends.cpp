@@ -60,35 +60,67 @@ int dtype_code(const torch::Tensor& t) {
 }
 
 // ---- ba (src/droid.cpp:141-168) ------------------------------------------------
-// The last few ba plans, keyed by the complete edge list and every planning argument, so
-// that repeated calls on one factor graph (each call of a global BA over the same graph)
-// skip the host planning.  A hit compares the whole key; entries are shared so an evicted
-// plan stays valid for a call still using it.
+// The last few ba plans, keyed by the complete edge list and every planning argument (device
+// included), so that repeated calls on one factor graph (each call of a global BA over the same
+// graph) skip the host planning and the upload of the plan's tables, which stay resident on the
+// device.  Lookup first by identity: the same ii / jj storage (held by the entry, so its address
+// cannot be taken by another tensor) at the same version counter, which torch advances on every
+// in-place write, holds the same edges, and the call needs no host copy of the list and no
+// stream synchronisation.  Otherwise the list is copied to the host and compared in full.
+// Entries are shared, so an evicted plan (and its device tables) stays valid for a call still
+// using it.
 class PlanCache {
  public:
-  using Args = std::array<int64_t, 8>;   // num_poses, ht, wd, t0, t1, motion_only, precision, semantics
+  using Args = std::array<int64_t, 9>;   // num_poses, ht, wd, t0, t1, motion_only, precision, semantics, device
   using Ptr = std::shared_ptr<const hsl2_ba_plan>;
 
-  Ptr get(const int64_t* ii, const int64_t* jj, int64_t E, const Args& a) {
+  Ptr get(const torch::Tensor& ii, const torch::Tensor& jj, const Args& a, const torch::TensorOptions& dev) {
+    const int64_t E = ii.size(0);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (auto it = entries_.begin(); it != entries_.end(); ++it) {
+        if (it->args == a && same_edges(it->ii_ref, it->ii_version, ii) && same_edges(it->jj_ref, it->jj_version, jj)) {
+          entries_.splice(entries_.begin(), entries_, it);
+          return entries_.front().plan;
+        }
+      }
+    }
+    torch::Tensor ii_h = ii.to(torch::kCPU), jj_h = jj.to(torch::kCPU);
+    const int64_t* iip = ii_h.data_ptr<int64_t>();
+    const int64_t* jjp = jj_h.data_ptr<int64_t>();
     {
       std::lock_guard<std::mutex> lock(mu_);
       for (auto it = entries_.begin(); it != entries_.end(); ++it) {
         if (it->args == a && static_cast<int64_t>(it->ii.size()) == E &&
-            std::memcmp(it->ii.data(), ii, E * sizeof(int64_t)) == 0 &&
-            std::memcmp(it->jj.data(), jj, E * sizeof(int64_t)) == 0) {
+            std::memcmp(it->ii.data(), iip, E * sizeof(int64_t)) == 0 &&
+            std::memcmp(it->jj.data(), jjp, E * sizeof(int64_t)) == 0) {
+          it->remember(ii, jj);
           entries_.splice(entries_.begin(), entries_, it);
           return entries_.front().plan;
         }
       }
     }
     hsl2_ba_plan* raw = nullptr;
-    raise_status(hsl2_ba_plan_create_ex(ii, jj, E, a[0], a[1], a[2], static_cast<int>(a[3]),
+    raise_status(hsl2_ba_plan_create_ex(iip, jjp, E, a[0], a[1], a[2], static_cast<int>(a[3]),
                                         static_cast<int>(a[4]), static_cast<int>(a[5]),
                                         static_cast<int>(a[6]), static_cast<int>(a[7]), &raw),
                  "ba");
-    Ptr plan(raw, hsl2_ba_plan_destroy);
+    std::unique_ptr<hsl2_ba_plan, void (*)(hsl2_ba_plan*)> owned(raw, hsl2_ba_plan_destroy);
+    // the plan's tables on the device, owned by the plan's deleter
+    torch::Tensor tables = torch::empty({static_cast<int64_t>(hsl2_ba_plan_table_bytes(raw)) + 256},
+                                        dev.dtype(torch::kUInt8));
+    const uintptr_t tb = reinterpret_cast<uintptr_t>(tables.data_ptr());
+    raise_status(hsl2_ba_plan_upload_tables(raw, reinterpret_cast<void*>((tb + 255) & ~uintptr_t(255)),
+                                            current_stream()),
+                 "ba");
+    Ptr plan(owned.release(), [tables](hsl2_ba_plan* q) mutable {
+      hsl2_ba_plan_destroy(q);
+      tables = torch::Tensor();
+    });
     std::lock_guard<std::mutex> lock(mu_);
-    entries_.push_front(Entry{a, std::vector<int64_t>(ii, ii + E), std::vector<int64_t>(jj, jj + E), plan});
+    Entry e{a, std::vector<int64_t>(iip, iip + E), std::vector<int64_t>(jjp, jjp + E), {}, 0, {}, 0, plan};
+    e.remember(ii, jj);
+    entries_.push_front(std::move(e));
     if (entries_.size() > kCapacity) entries_.pop_back();
     return plan;
   }
@@ -97,8 +129,22 @@ class PlanCache {
   struct Entry {
     Args args;
     std::vector<int64_t> ii, jj;
+    torch::Tensor ii_ref;     // the caller's tensors of the last hit, and their versions then
+    int64_t ii_version;
+    torch::Tensor jj_ref;
+    int64_t jj_version;
     Ptr plan;
+    void remember(const torch::Tensor& i, const torch::Tensor& j) {
+      ii_ref = i;
+      ii_version = i._version();
+      jj_ref = j;
+      jj_version = j._version();
+    }
   };
+  static bool same_edges(const torch::Tensor& held, int64_t version, const torch::Tensor& t) {
+    return held.defined() && held.data_ptr() == t.data_ptr() && held.numel() == t.numel() &&
+           held.device() == t.device() && t._version() == version;
+  }
   static constexpr size_t kCapacity = 4;
   std::mutex mu_;
   std::list<Entry> entries_;
@@ -135,11 +181,12 @@ std::vector<torch::Tensor> ba_impl(torch::Tensor poses, torch::Tensor disps, tor
           "ba: targets and weights must be [E, 2, ht, wd]");
   c10::hip::HIPGuardMasqueradingAsCUDA guard(poses.device());
 
-  // one host copy of the edge list per call (the reference copies it 16+ times per iteration)
-  torch::Tensor ii_h = ii.to(torch::kCPU), jj_h = jj.to(torch::kCPU);
+  // at most one host copy of the edge list per call, none when the same ii / jj tensors repeat
+  // (the reference copies the list 16+ times per iteration)
   const PlanCache::Ptr plan_ref = plan_cache().get(
-      ii_h.data_ptr<int64_t>(), jj_h.data_ptr<int64_t>(), E,
-      {poses.size(0), ht, wd, t0, t1, motion_only ? 1 : 0, precision, HSL2_SEM_CUDA});
+      ii, jj, {poses.size(0), ht, wd, t0, t1, motion_only ? 1 : 0, precision, HSL2_SEM_CUDA,
+               static_cast<int64_t>(poses.device().index())},
+      poses.options());
   const hsl2_ba_plan* plan = plan_ref.get();
   const int64_t F = hsl2_ba_plan_num_frames(plan);
   int64_t eta_rows = 1;

@@ -219,6 +219,14 @@ int hsl2_ba_plan_frames(const hsl2_ba_plan* plan, int64_t* kx);
 int hsl2_ba_plan_inverse(const hsl2_ba_plan* plan, int64_t* kk_exp);
 /* bytes of device workspace hsl2_ba_run needs (256-byte aligned base required) */
 size_t hsl2_ba_plan_workspace_bytes(const hsl2_ba_plan* plan);
+/* No counterpart in the reference (its ba re-derives the index structures on the host every
+ * iteration, src/droid_kernels.cu:1837-1849).  A plan's int32 tables are constant: their size in
+ * bytes, and registering a device copy.  hsl2_ba_plan_upload_tables copies them on `stream` into
+ * device_buffer (hsl2_ba_plan_table_bytes bytes, 256-byte aligned, on the current device) and
+ * records it in the plan; later runs on that device read the tables there instead of uploading
+ * them into the workspace on every call.  The buffer must outlive every run of the plan. */
+size_t hsl2_ba_plan_table_bytes(const hsl2_ba_plan* plan);
+int hsl2_ba_plan_upload_tables(hsl2_ba_plan* plan, void* device_buffer, void* stream);
 
 /* Replaces ba (src/droid.cpp:141-168 -> src/droid_kernels.cu:1820-1937).
  * Device arrays: poses [num_poses, 7] f32 and disps [num_poses, ht, wd] f32 (updated in
