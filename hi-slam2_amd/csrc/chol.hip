@@ -148,16 +148,24 @@ __device__ __forceinline__ void prod_pivot(R x, R& piv, R& is, bool& bad) {
 }
 
 // ---- the diagonal factor on kFactorWaves waves -------------------------------------------
-// Factor wave w owns the kFW columns [w kFW, (w + 1) kFW) of the diagonal tile (lane r: row r,
-// a[s] = A_kk[r][w kFW + s]).  It first applies the rank-1 updates of the columns before its own,
+// Factor wave w owns the W columns [OWN0, OWN0 + W) = [kFactorCol[w], kFactorCol[w + 1]) of the
+// diagonal tile (lane r: row r, a[s] = A_kk[r][OWN0 + s]); the ranges start at multiples of four,
+// the publication granule, and shrink towards the end of the tile, where the waves start late.
+// It first applies the rank-1 updates of the columns before its own,
 // four published columns at a time, then factors its columns as the single-wave producer above
 // did (pivot by readlane, column and 1 / L_cc published behind the counter, the rank-1 update of
 // its later columns read back from LDS as broadcasts, the next pivot formed while those reads
 // are in flight).  The waves publish in column order, so the consumers are unchanged.  LDS
 // addresses are taken relative to an opaque zero so that each column's reads share one base
 // register and differ only in their immediate offsets.
+// Two waves of 32 columns: three (24 / 20 / 20) measured no faster, the per-column cost being
+// the pivot chain and the LDS round trip rather than the width of each wave's update.
 constexpr int kFactorWaves = 2;
-constexpr int kFW = kTile / kFactorWaves;
+constexpr int kFactorCol[kFactorWaves + 1] = {0, 32, kTile};
+// chol_step blocks: the factor waves and one consumer wave, at most one wave per SIMD so that
+// each may use the whole register file; the update tasks use all 256 threads
+constexpr int kStepThreads = 256;
+static_assert((kFactorWaves + 1) * kWave <= kStepThreads, "factor and consumer waves fit the block");
 
 __device__ __forceinline__ int opaque_zero() {
   int z;
@@ -168,32 +176,32 @@ __device__ __forceinline__ int opaque_zero() {
 // The rest of column C's rank-1 update, a[s] -= L[r][C] L[OWN0 + s][C] for s > c + 1 (column
 // c + 1 was updated on the critical path), with L[.][C] read back from LDS as broadcasts in
 // groups of kProdGroup: the reads of group G + 1 are issued before the FMAs of group G.
-template <typename R, int C, int OWN0, int G>
+template <typename R, int C, int OWN0, int W, int G>
 __device__ __forceinline__ void fw_read(const PanelShared<R>& sh, int z, R (&v)[kProdGroup]) {
   constexpr int s0 = C - OWN0 + 2 + G * kProdGroup;      // first register index of the group
 #pragma unroll
   for (int i = 0; i < kProdGroup; ++i)
-    if (s0 + i < kFW) v[i] = sh.Lc[C][z + OWN0 + s0 + i];
+    if (s0 + i < W) v[i] = sh.Lc[C][z + OWN0 + s0 + i];
 }
 
-template <typename R, int C, int OWN0, int G>
-__device__ __forceinline__ void fw_fma(R (&a)[kFW], R l, const R (&v)[kProdGroup]) {
+template <typename R, int C, int OWN0, int W, int G>
+__device__ __forceinline__ void fw_fma(R (&a)[W], R l, const R (&v)[kProdGroup]) {
   constexpr int s0 = C - OWN0 + 2 + G * kProdGroup;
 #pragma unroll
   for (int i = 0; i < kProdGroup; ++i)
-    if (s0 + i < kFW) a[s0 + i] -= l * v[i];
+    if (s0 + i < W) a[s0 + i] -= l * v[i];
 }
 
-template <typename R, int C, int OWN0, typename Hook, int... Gs>
-__device__ __forceinline__ void fw_bulk(R (&a)[kFW], R l, const PanelShared<R>& sh, int z, Hook& while_reading,
+template <typename R, int C, int OWN0, int W, typename Hook, int... Gs>
+__device__ __forceinline__ void fw_bulk(R (&a)[W], R l, const PanelShared<R>& sh, int z, Hook& while_reading,
                                         std::integer_sequence<int, Gs...>) {
   R v[2][kProdGroup];
-  if constexpr (sizeof...(Gs) > 0) fw_read<R, C, OWN0, 0>(sh, z, v[0]);
+  if constexpr (sizeof...(Gs) > 0) fw_read<R, C, OWN0, W, 0>(sh, z, v[0]);
   __builtin_amdgcn_sched_barrier(0);
   while_reading();
   (([&] {
-     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) fw_read<R, C, OWN0, Gs + 1>(sh, z, v[(Gs + 1) & 1]);
-     fw_fma<R, C, OWN0, Gs>(a, l, v[Gs & 1]);
+     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) fw_read<R, C, OWN0, W, Gs + 1>(sh, z, v[(Gs + 1) & 1]);
+     fw_fma<R, C, OWN0, W, Gs>(a, l, v[Gs & 1]);
    }()),
    ...);
 }
@@ -202,38 +210,38 @@ __device__ __forceinline__ void fw_bulk(R (&a)[kFW], R l, const PanelShared<R>& 
 // L[C + 1][C] through a readlane (no LDS round trip on the critical path); column C and
 // 1 / L_CC go to LDS; the next pivot's reciprocal square root runs while the first group of the
 // rest of the update is being read; the column is published; the rest of the update follows.
-template <typename R, int C, int OWN0>
-__device__ __forceinline__ void fw_column(R (&a)[kFW], PanelShared<R>& sh, int z, bool& bad, R& piv, R& is) {
+template <typename R, int C, int OWN0, int W>
+__device__ __forceinline__ void fw_column(R (&a)[W], PanelShared<R>& sh, int z, bool& bad, R& piv, R& is) {
   const int r = threadIdx.x % kWave;
   constexpr int c = C - OWN0;
   const R l = (r == C) ? piv * is : (r > C ? a[c] * is : R(0));     // L[r][C]
-  if constexpr (c + 1 < kFW) a[c + 1] -= l * lane_bcast<R>(l, C + 1);
+  if constexpr (c + 1 < W) a[c + 1] -= l * lane_bcast<R>(l, C + 1);
   sh.Lc[C][r] = l;                               // column C of L
   sh.dinv[C] = is;
   auto next_pivot = [&] {
-    if constexpr (c + 1 < kFW) prod_pivot<R, C + 1>(a[c + 1], piv, is, bad);
+    if constexpr (c + 1 < W) prod_pivot<R, C + 1>(a[c + 1], piv, is, bad);
     // published in steps of four columns (the consumers and the later factor waves wait for
     // four at a time): the release store waits for every LDS access in flight
     if constexpr ((C + 1) % 4 == 0)
       __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
-  constexpr int rest = kFW - (c + 2) > 0 ? kFW - (c + 2) : 0;
-  fw_bulk<R, C, OWN0>(a, l, sh, z, next_pivot,
+  constexpr int rest = W - (c + 2) > 0 ? W - (c + 2) : 0;
+  fw_bulk<R, C, OWN0, W>(a, l, sh, z, next_pivot,
                       std::make_integer_sequence<int, (rest + kProdGroup - 1) / kProdGroup>{});
 }
 
-template <typename R, int OWN0, int... Cs>
-__device__ __forceinline__ void fw_all(R (&a)[kFW], PanelShared<R>& sh, int z, bool& bad,
+template <typename R, int OWN0, int W, int... Cs>
+__device__ __forceinline__ void fw_all(R (&a)[W], PanelShared<R>& sh, int z, bool& bad,
                                        std::integer_sequence<int, Cs...>) {
   R piv, is;
   prod_pivot<R, OWN0>(a[0], piv, is, bad);
-  (fw_column<R, OWN0 + Cs, OWN0>(a, sh, z, bad, piv, is), ...);
+  (fw_column<R, OWN0 + Cs, OWN0, W>(a, sh, z, bad, piv, is), ...);
 }
 
 // a[s] -= sum_{c < OWN0} L[r][c] L[OWN0 + s][c], four published columns per step; false if a
 // wait gave up.
-template <typename R, int OWN0>
-__device__ __forceinline__ bool fw_apply_earlier(R (&a)[kFW], const PanelShared<R>& sh, int z) {
+template <typename R, int OWN0, int W>
+__device__ __forceinline__ bool fw_apply_earlier(R (&a)[W], const PanelShared<R>& sh, int z) {
   const int r = threadIdx.x % kWave;
   unsigned late = 0;
   for (int c0 = 0; c0 < OWN0; c0 += 4) {
@@ -241,28 +249,29 @@ __device__ __forceinline__ bool fw_apply_earlier(R (&a)[kFW], const PanelShared<
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const R l = sh.Lc[c0 + m][r];
-      R v[kFW];
+      R v[W];
 #pragma unroll
-      for (int q = 0; q < kFW; ++q) v[q] = sh.Lc[c0 + m][z + OWN0 + q];
+      for (int q = 0; q < W; ++q) v[q] = sh.Lc[c0 + m][z + OWN0 + q];
 #pragma unroll
-      for (int q = 0; q < kFW; ++q) a[q] -= l * v[q];
+      for (int q = 0; q < W; ++q) a[q] -= l * v[q];
     }
   }
   return late == 0;
 }
 
-template <typename R, int OWN0>
+template <typename R, int OWN0, int W>
 __device__ void factor_part(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh, bool report,
                             int* __restrict__ info) {
+  static_assert(OWN0 % 4 == 0 && W > 0, "factor ranges start at multiples of the publication granule");
   const int r = threadIdx.x % kWave;
   const int64_t base = (int64_t)k * kTile;
-  R a[kFW];
+  R a[W];
 #pragma unroll
-  for (int c = 0; c < kFW; ++c) a[c] = A[(base + r) + (base + OWN0 + c) * lda];
+  for (int c = 0; c < W; ++c) a[c] = A[(base + r) + (base + OWN0 + c) * lda];
   const int z = opaque_zero();
   bool bad = false;
-  if constexpr (OWN0 > 0) bad = !fw_apply_earlier<R, OWN0>(a, sh, z);
-  fw_all<R, OWN0>(a, sh, z, bad, std::make_integer_sequence<int, kFW>{});
+  if constexpr (OWN0 > 0) bad = !fw_apply_earlier<R, OWN0, W>(a, sh, z);
+  fw_all<R, OWN0, W>(a, sh, z, bad, std::make_integer_sequence<int, W>{});
   if (bad && report && r == 0) info[0] = 1;
 }
 
@@ -270,7 +279,9 @@ template <typename R, int... Ws>
 __device__ __forceinline__ void factor_waves(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh,
                                              bool report, int* __restrict__ info, std::integer_sequence<int, Ws...>) {
   const int w = threadIdx.x / kWave;
-  ((w == Ws ? factor_part<R, Ws * kFW>(A, lda, k, sh, report, info) : void()), ...);
+  ((w == Ws ? factor_part<R, kFactorCol[Ws], kFactorCol[Ws + 1] - kFactorCol[Ws]>(A, lda, k, sh, report, info)
+            : void()),
+   ...);
 }
 
 // Consumer step over columns M .. M+3 of L_kk once they are published: the 4x4 triangle
@@ -476,7 +487,7 @@ template <typename R> union StepShared {
 // m = T - k, tiles (ii, jj) of step k - 1 with 1 <= jj <= ii < m, then rhs rows 1 .. m - 1.
 // Neither part touches what the other reads or writes.
 template <typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStepThreads)
 chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k, int T,
           int* __restrict__ info) {
   __shared__ StepShared<R> smem;
@@ -636,7 +647,7 @@ int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_
     // panel k, with the rest of step k - 1's trailing update alongside
     const int m1 = k > 0 ? T - k - 1 : 0;
     const int rest = k > 0 ? m1 * (m1 + 1) / 2 + m1 : 0;
-    hipLaunchKernelGGL(chol_step<R>, dim3(T - k + rest), dim3(256), 0, s, A, n_pad, Ld, b, k, T, info);
+    hipLaunchKernelGGL(chol_step<R>, dim3(T - k + rest), dim3(kStepThreads), 0, s, A, n_pad, Ld, b, k, T, info);
     HSL2_LAUNCH_CHECK();
     const int m = T - k - 1;
     if (m > 0) {
