@@ -570,8 +570,13 @@ chol_update_next(R* __restrict__ A, int64_t lda, R* __restrict__ b, int k, int m
 }
 
 // Back-substitution x = L^-T y in one launch, one block per tile row j (block b takes
-// j = T - 1 - b, so the blocks without dependencies start first; all T blocks are resident
-// together at the sizes used here, T <= 1024 blocks of 256 threads).  Block j keeps y_j in
+// j = T - 1 - b, so the blocks without dependencies start first).  Forward progress: block b
+// waits only for x_k with k > j, which blocks of lower index publish, so the lowest-index block
+// still running never waits for one that has not started, as long as blocks are dispatched in
+// index order (as this hardware does; the bounded wait turns a violation into info = 1, not a
+// hang).  Up to T = 512 all blocks are resident together as well (68 KB of LDS each in FP64: two
+// per CU on 256 CUs), which is why hsl2_spd_solve accepts n <= 32768; the BA's reduced systems
+// (6 P unknowns) stay within that for P <= 5461 poses.  Block j keeps y_j in
 // LDS and, for k = T-1 down to j+1, stages tile (k, j) of L, waits until x_k is published
 // (agent-scope acquire of flags[k]; bounded, a timeout sets info[0] = 1) and applies
 // y_j -= L_kj^T x_k; then x_j = (L_jj^-1)^T y_j is written and flags[j] raised with an
@@ -700,7 +705,7 @@ extern "C" size_t hsl2_spd_solve_workspace_bytes(int dtype, int64_t n) {
 extern "C" int hsl2_spd_solve(int dtype, const void* A, const void* b, void* x, int64_t n,
                               void* workspace, size_t workspace_bytes, int* ok, void* stream) {
   HSL2_REQUIRE(A && b && x && workspace, HSL2_ERR_ARGUMENT, "spd_solve: null pointer");
-  HSL2_REQUIRE(n > 0 && n < (1 << 16), HSL2_ERR_SHAPE, "spd_solve: n must be in [1, 65535]");
+  HSL2_REQUIRE(n > 0 && n <= (1 << 15), HSL2_ERR_SHAPE, "spd_solve: n must be in [1, 32768]");
   HSL2_REQUIRE(dtype == HSL2_F64 || dtype == HSL2_F32, HSL2_ERR_ARGUMENT, "spd_solve: dtype must be F32 or F64");
   HSL2_REQUIRE(workspace_bytes >= hsl2_spd_solve_workspace_bytes(dtype, n), HSL2_ERR_WORKSPACE,
                "spd_solve: workspace too small");

@@ -402,7 +402,8 @@ int hsl2_jdsa_solve(const float* C, const float* w, const float* disps, const fl
  * tiled Cholesky that ba uses for its reduced camera system (replaces the host Eigen
  * SimplicialLLT of SparseBlock::solve, src/droid_kernels.cu:1659-1680).  A is not modified;
  * x = 0 and *ok = 0 (host, may be NULL; reading it synchronizes) when a pivot is not
- * positive.  The workspace (256-byte aligned) holds the padded copy and the factors. */
+ * positive.  1 <= n <= 32768.  The workspace (256-byte aligned) holds the padded copy and the
+ * factors. */
 size_t hsl2_spd_solve_workspace_bytes(int dtype, int64_t n);
 int hsl2_spd_solve(int dtype, const void* A, const void* b, void* x, int64_t n, void* workspace,
                    size_t workspace_bytes, int* ok, void* stream);

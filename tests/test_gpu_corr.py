@@ -57,7 +57,7 @@ def test_lookup_forward(db, dtype, r, shape):
     assert np.all(got[1, :, :, 2, 3] == 0)
 
 
-@pytest.mark.parametrize('dtype', [torch.float32, torch.float64, torch.bfloat16])
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64, torch.float16, torch.bfloat16])
 def test_lookup_backward(db, dtype):
     rng = np.random.default_rng(20)
     BN, h1, w1, h2, w2, r = 2, 8, 12, 9, 11, 3
