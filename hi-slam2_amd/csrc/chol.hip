@@ -129,37 +129,31 @@ __device__ __forceinline__ bool wait_flag(const int* flag, int v) {
 template <typename R> struct PanelShared {
   R Lc[kTile][kLd];        // Lc[c][r] = L_kk[r][c]
   R dinv[kTile];           // 1 / L_kk[c][c]
+  R bk[kTile];             // b_k (block 0), staged before the factor for y_k = L_kk^-1 b_k
   int published;           // columns of L_kk published so far
 };
-
-// Factor waves broadcast LDS reads in groups of kProdGroup columns.
-constexpr int kProdGroup = 16;
-
-// Pivot of column C from lane C's value x (non-positive or NaN -> 1, flagged) and its
-// reciprocal square root.
-template <typename R, int C>
-__device__ __forceinline__ void prod_pivot(R x, R& piv, R& is, bool& bad) {
-  piv = lane_bcast<R>(x, C);
-  if (!(piv > R(0))) {
-    bad = true;
-    piv = R(1);
-  }
-  is = pivot_rsqrt(piv);
-}
 
 // ---- the diagonal factor on kFactorWaves waves -------------------------------------------
 // Factor wave w owns the W columns [OWN0, OWN0 + W) = [kFactorCol[w], kFactorCol[w + 1]) of the
 // diagonal tile (lane r: row r, a[s] = A_kk[r][OWN0 + s]); the ranges start at multiples of four,
-// the publication granule, and shrink towards the end of the tile, where the waves start late.
-// It first applies the rank-1 updates of the columns before its own,
-// four published columns at a time, then factors its columns as the single-wave producer above
-// did (pivot by readlane, column and 1 / L_cc published behind the counter, the rank-1 update of
-// its later columns read back from LDS as broadcasts, the next pivot formed while those reads
-// are in flight).  The waves publish in column order, so the consumers are unchanged.  LDS
-// addresses are taken relative to an opaque zero so that each column's reads share one base
-// register and differ only in their immediate offsets.
-// Two waves of 32 columns: three (24 / 20 / 20) measured no faster, the per-column cost being
-// the pivot chain and the LDS round trip rather than the width of each wave's update.
+// the publication granule.  It first applies the rank-1 updates of the columns before its own,
+// four published columns at a time, then factors its columns four at a time:
+//   * the 4 x 4 diagonal block is read from its four lanes by v_readlane (ten values) and
+//     factored in every lane alike: pivots e_j, their reciprocal square roots, and the block's
+//     off-diagonal entries l_ij;
+//   * each lane forms its row's four entries of L by the same updates the column-by-column
+//     algorithm applies (a[c+j] -= L_m l_jm for m < j, in column order, then times 1 / L_jj), so
+//     every value, the pivots included, is the one that algorithm computes, operation for
+//     operation;
+//   * the four columns and 1 / L_jj go to LDS and are published together (workgroup-scope
+//     release of the counter), and the wave's later columns take the rank-4 update, reading the
+//     published columns back from LDS as broadcasts four columns at a time, the next group's
+//     reads in flight; the first group is the next block's four columns.
+// So the cross-lane round trips and the pivot latencies are paid once per four columns.  The
+// waves publish in column order, so the consumers are unchanged.  LDS addresses are taken
+// relative to an opaque zero so that each group's reads share one base register.
+// Two waves of 32 columns: three (24 / 20 / 20) measured no faster with the column-by-column
+// step, the per-column cost being the pivot chain and the LDS round trip.
 constexpr int kFactorWaves = 2;
 constexpr int kFactorCol[kFactorWaves + 1] = {0, 32, kTile};
 // chol_step blocks: the factor waves and one consumer wave, at most one wave per SIMD so that
@@ -173,69 +167,124 @@ __device__ __forceinline__ int opaque_zero() {
   return z;
 }
 
-// The rest of column C's rank-1 update, a[s] -= L[r][C] L[OWN0 + s][C] for s > c + 1 (column
-// c + 1 was updated on the critical path), with L[.][C] read back from LDS as broadcasts in
-// groups of kProdGroup: the reads of group G + 1 are issued before the FMAs of group G.
-template <typename R, int C, int OWN0, int W, int G>
-__device__ __forceinline__ void fw_read(const PanelShared<R>& sh, int z, R (&v)[kProdGroup]) {
-  constexpr int s0 = C - OWN0 + 2 + G * kProdGroup;      // first register index of the group
-#pragma unroll
-  for (int i = 0; i < kProdGroup; ++i)
-    if (s0 + i < W) v[i] = sh.Lc[C][z + OWN0 + s0 + i];
+// A non-positive (or NaN) pivot is replaced by 1 and flagged, as the failed-LLT contract needs.
+template <typename R>
+__device__ __forceinline__ R checked_pivot(R e, bool& bad) {
+  if (!(e > R(0))) {
+    bad = true;
+    e = R(1);
+  }
+  return e;
 }
 
-template <typename R, int C, int OWN0, int W, int G>
-__device__ __forceinline__ void fw_fma(R (&a)[W], R l, const R (&v)[kProdGroup]) {
-  constexpr int s0 = C - OWN0 + 2 + G * kProdGroup;
+// Rank-4 update of local columns s = c + 4 + 4 G .. + 3 by the block's columns C4 .. C4 + 3:
+// a[s] -= L_j L[OWN0 + s][C4 + j] for j = 0..3 in order; v[j][i] = L[OWN0 + s0 + i][C4 + j].
+template <typename R, int C4, int OWN0, int W, int G>
+__device__ __forceinline__ void fb_read(const PanelShared<R>& sh, int z, R (&v)[4][4]) {
+  constexpr int s0 = C4 - OWN0 + 4 + 4 * G;
 #pragma unroll
-  for (int i = 0; i < kProdGroup; ++i)
-    if (s0 + i < W) a[s0 + i] -= l * v[i];
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (s0 + i < W) v[j][i] = sh.Lc[C4 + j][z + OWN0 + s0 + i];
 }
 
-template <typename R, int C, int OWN0, int W, typename Hook, int... Gs>
-__device__ __forceinline__ void fw_bulk(R (&a)[W], R l, const PanelShared<R>& sh, int z, Hook& while_reading,
+template <typename R, int C4, int OWN0, int W, int G>
+__device__ __forceinline__ void fb_fma(R (&a)[W], const R (&L)[4], const R (&v)[4][4]) {
+  constexpr int s0 = C4 - OWN0 + 4 + 4 * G;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (s0 + i < W) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[s0 + i] -= L[j] * v[j][i];
+    }
+}
+
+template <typename R, int C4, int OWN0, int W, int... Gs>
+__device__ __forceinline__ void fb_bulk(R (&a)[W], const R (&L)[4], const PanelShared<R>& sh, int z,
                                         std::integer_sequence<int, Gs...>) {
-  R v[2][kProdGroup];
-  if constexpr (sizeof...(Gs) > 0) fw_read<R, C, OWN0, W, 0>(sh, z, v[0]);
-  __builtin_amdgcn_sched_barrier(0);
-  while_reading();
+  R v[2][4][4];
+  if constexpr (sizeof...(Gs) > 0) fb_read<R, C4, OWN0, W, 0>(sh, z, v[0]);
   (([&] {
-     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) fw_read<R, C, OWN0, W, Gs + 1>(sh, z, v[(Gs + 1) & 1]);
-     fw_fma<R, C, OWN0, W, Gs>(a, l, v[Gs & 1]);
+     if constexpr (Gs + 1 < static_cast<int>(sizeof...(Gs))) fb_read<R, C4, OWN0, W, Gs + 1>(sh, z, v[(Gs + 1) & 1]);
+     fb_fma<R, C4, OWN0, W, Gs>(a, L, v[Gs & 1]);
    }()),
    ...);
 }
 
-// Column C (local index c): L[r][C] from the pivot already formed; the next column's update by
-// L[C + 1][C] through a readlane (no LDS round trip on the critical path); column C and
-// 1 / L_CC go to LDS; the next pivot's reciprocal square root runs while the first group of the
-// rest of the update is being read; the column is published; the rest of the update follows.
-template <typename R, int C, int OWN0, int W>
-__device__ __forceinline__ void fw_column(R (&a)[W], PanelShared<R>& sh, int z, bool& bad, R& piv, R& is) {
+// Columns C4 .. C4 + 3 (local c .. c + 3).  Every expression below repeats, in the same order,
+// the operation that the column-by-column step performs on the lane that owns the value.
+template <typename R, int C4, int OWN0, int W>
+__device__ __forceinline__ void fw_block4(R (&a)[W], PanelShared<R>& sh, int z, bool& bad) {
+  static_assert((C4 - OWN0) % 4 == 0 && C4 - OWN0 + 4 <= W, "blocks of four columns inside the wave's range");
   const int r = threadIdx.x % kWave;
-  constexpr int c = C - OWN0;
-  const R l = (r == C) ? piv * is : (r > C ? a[c] * is : R(0));     // L[r][C]
-  if constexpr (c + 1 < W) a[c + 1] -= l * lane_bcast<R>(l, C + 1);
-  sh.Lc[C][r] = l;                               // column C of L
-  sh.dinv[C] = is;
-  auto next_pivot = [&] {
-    if constexpr (c + 1 < W) prod_pivot<R, C + 1>(a[c + 1], piv, is, bad);
-    // published in steps of four columns (the consumers and the later factor waves wait for
-    // four at a time): the release store waits for every LDS access in flight
-    if constexpr ((C + 1) % 4 == 0)
-      __hip_atomic_store(&sh.published, C + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-  };
-  constexpr int rest = W - (c + 2) > 0 ? W - (c + 2) : 0;
-  fw_bulk<R, C, OWN0, W>(a, l, sh, z, next_pivot,
-                      std::make_integer_sequence<int, (rest + kProdGroup - 1) / kProdGroup>{});
+  constexpr int c = C4 - OWN0;
+  // the diagonal block, lower triangle
+  const R d00 = lane_bcast<R>(a[c], C4);
+  const R d10 = lane_bcast<R>(a[c], C4 + 1), d11 = lane_bcast<R>(a[c + 1], C4 + 1);
+  const R d20 = lane_bcast<R>(a[c], C4 + 2), d21 = lane_bcast<R>(a[c + 1], C4 + 2);
+  const R d22 = lane_bcast<R>(a[c + 2], C4 + 2);
+  const R d30 = lane_bcast<R>(a[c], C4 + 3), d31 = lane_bcast<R>(a[c + 1], C4 + 3);
+  const R d32 = lane_bcast<R>(a[c + 2], C4 + 3), d33 = lane_bcast<R>(a[c + 3], C4 + 3);
+  R L[4];
+  // column C4
+  const R e0 = checked_pivot<R>(d00, bad);
+  const R is0 = pivot_rsqrt(e0);
+  const R l10 = d10 * is0, l20 = d20 * is0, l30 = d30 * is0;
+  L[0] = (r == C4) ? e0 * is0 : (r > C4 ? a[c] * is0 : R(0));
+  a[c + 1] -= L[0] * l10;
+  a[c + 2] -= L[0] * l20;
+  a[c + 3] -= L[0] * l30;
+  // column C4 + 1
+  R t11 = d11;
+  t11 -= l10 * l10;
+  const R e1 = checked_pivot<R>(t11, bad);
+  const R is1 = pivot_rsqrt(e1);
+  R t21 = d21;
+  t21 -= l20 * l10;
+  R t31 = d31;
+  t31 -= l30 * l10;
+  const R l21 = t21 * is1, l31 = t31 * is1;
+  L[1] = (r == C4 + 1) ? e1 * is1 : (r > C4 + 1 ? a[c + 1] * is1 : R(0));
+  a[c + 2] -= L[1] * l21;
+  a[c + 3] -= L[1] * l31;
+  // column C4 + 2
+  R t22 = d22;
+  t22 -= l20 * l20;
+  t22 -= l21 * l21;
+  const R e2 = checked_pivot<R>(t22, bad);
+  const R is2 = pivot_rsqrt(e2);
+  R t32 = d32;
+  t32 -= l30 * l20;
+  t32 -= l31 * l21;
+  const R l32 = t32 * is2;
+  L[2] = (r == C4 + 2) ? e2 * is2 : (r > C4 + 2 ? a[c + 2] * is2 : R(0));
+  a[c + 3] -= L[2] * l32;
+  // column C4 + 3
+  R t33 = d33;
+  t33 -= l30 * l30;
+  t33 -= l31 * l31;
+  t33 -= l32 * l32;
+  const R e3 = checked_pivot<R>(t33, bad);
+  const R is3 = pivot_rsqrt(e3);
+  L[3] = (r == C4 + 3) ? e3 * is3 : (r > C4 + 3 ? a[c + 3] * is3 : R(0));
+  // the four columns and their 1 / L_jj, published together (the release store waits for
+  // every LDS access in flight)
+#pragma unroll
+  for (int j = 0; j < 4; ++j) sh.Lc[C4 + j][r] = L[j];
+  sh.dinv[C4] = is0;
+  sh.dinv[C4 + 1] = is1;
+  sh.dinv[C4 + 2] = is2;
+  sh.dinv[C4 + 3] = is3;
+  __hip_atomic_store(&sh.published, C4 + 4, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  constexpr int rest = W - (c + 4);
+  fb_bulk<R, C4, OWN0, W>(a, L, sh, z, std::make_integer_sequence<int, (rest + 3) / 4>{});
 }
 
-template <typename R, int OWN0, int W, int... Cs>
+template <typename R, int OWN0, int W, int... Bs>
 __device__ __forceinline__ void fw_all(R (&a)[W], PanelShared<R>& sh, int z, bool& bad,
-                                       std::integer_sequence<int, Cs...>) {
-  R piv, is;
-  prod_pivot<R, OWN0>(a[0], piv, is, bad);
-  (fw_column<R, OWN0 + Cs, OWN0, W>(a, sh, z, bad, piv, is), ...);
+                                       std::integer_sequence<int, Bs...>) {
+  (fw_block4<R, OWN0 + 4 * Bs, OWN0, W>(a, sh, z, bad), ...);
 }
 
 // a[s] -= sum_{c < OWN0} L[r][c] L[OWN0 + s][c], four published columns per step; false if a
@@ -262,7 +311,7 @@ __device__ __forceinline__ bool fw_apply_earlier(R (&a)[W], const PanelShared<R>
 template <typename R, int OWN0, int W>
 __device__ void factor_part(const R* __restrict__ A, int64_t lda, int k, PanelShared<R>& sh, bool report,
                             int* __restrict__ info) {
-  static_assert(OWN0 % 4 == 0 && W > 0, "factor ranges start at multiples of the publication granule");
+  static_assert(OWN0 % 4 == 0 && W % 4 == 0 && W > 0, "factor ranges are whole blocks of four columns");
   const int r = threadIdx.x % kWave;
   const int64_t base = (int64_t)k * kTile;
   R a[W];
@@ -271,7 +320,7 @@ __device__ void factor_part(const R* __restrict__ A, int64_t lda, int k, PanelSh
   const int z = opaque_zero();
   bool bad = false;
   if constexpr (OWN0 > 0) bad = !fw_apply_earlier<R, OWN0, W>(a, sh, z);
-  fw_all<R, OWN0, W>(a, sh, z, bad, std::make_integer_sequence<int, W>{});
+  fw_all<R, OWN0, W>(a, sh, z, bad, std::make_integer_sequence<int, W / 4>{});
   if (bad && report && r == 0) info[0] = 1;
 }
 
@@ -396,17 +445,20 @@ __device__ void panel_block(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, 
     if (!solve_rows<R>(A, lda, bi, bk, sh) && t == kFactorWaves * kWave) info[0] = 1;
     return;
   }
-  const bool ok = invert_columns<R>(sh, Xs);
-  if (!ok && t == kFactorWaves * kWave) info[0] = 1;
-  // wave 2 alone from here (its own LDS writes, in order): L_kk^-1 column-major into Ld, and
+  // the consumer wave of block 0 (its own LDS writes and reads, in order): b_k staged in LDS
+  // now, its load hidden behind the factor (b_k is final: the previous launch updated it, this
+  // one does not touch it); then L_kk^-1 by columns, written column-major into Ld, and
   // y_k = L_kk^-1 b_k
   const int u = t - kFactorWaves * kWave;
+  sh.bk[u] = b[bk + u];
+  const bool ok = invert_columns<R>(sh, Xs);
+  if (!ok && t == kFactorWaves * kWave) info[0] = 1;
   R* Lk = Ld + (int64_t)k * kTile * kTile;
 #pragma unroll 8
   for (int c = 0; c < kTile; ++c) Lk[c * kTile + u] = Xs[u][c];
   R acc = 0;
 #pragma unroll 8
-  for (int c = 0; c < kTile; ++c) acc += Xs[u][c] * b[bk + c];
+  for (int c = 0; c < kTile; ++c) acc += Xs[u][c] * sh.bk[c];
   b[bk + u] = acc;
 }
 
