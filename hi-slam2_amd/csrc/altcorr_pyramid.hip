@@ -114,7 +114,7 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
   constexpr int ROWB = kApC * sizeof(T) + 16;             // padded LDS bytes per staged position
   constexpr int CHUNK = ap_chunk<T>(), TPC = CHUNK / 16;  // positions and tiles per chunk
   constexpr int SPT = CHUNK * PP / kApThreads;            // staged pieces per thread
-  constexpr int TAPB = (kApPix * TS * 4 + 15) / 16 * 16;
+  constexpr int TAPB = ((kApPix * TS + kApPix) * 4 + 15) / 16 * 16;   // taps + one spare slot per lane
   extern __shared__ __align__(16) unsigned char smem[];
   float* s_tap = reinterpret_cast<float*>(smem);
   unsigned char* s_b = smem + TAPB;
@@ -166,6 +166,15 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
   }
   const bool mine = ((lane & 15) >> 2) == wave;        // lane's block pixel lies in this wave's sub-patch
 
+  // scatter geometry of this lane's MFMA results: column `col` is map position (ycol, xcol) of a
+  // tile, rows 4g + i are the block pixels bpi[i]; results outside a pixel's window go to this
+  // lane's own spare slot behind the taps (never read), so the stores need no branch
+  const int ycol = col >> 3, xcol = col & 7;
+  const int trash = kApPix * TS + lane;
+  int tbase[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) tbase[i] = bpi[i] * TS;
+
   for (int lvl = 0; lvl < lv.L; ++lvl) {
     const int Hl = lv.h[lvl], Wl = lv.w[lvl];
     const int ox = pin ? window_origin<D>(cx, lvl) : kFar;
@@ -173,62 +182,86 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
     const bool hit = ox != kFar && oy != kFar && ox + D > 0 && ox < Wl && oy + D > 0 && oy < Hl;
     const int y_lo = hit ? max(oy, 0) : INT_MAX, y_hi = hit ? min(oy + D - 1, Hl - 1) : INT_MIN;
     const int x_lo = hit ? max(ox, 0) : INT_MAX, x_hi = hit ? min(ox + D - 1, Wl - 1) : INT_MIN;
-    const int Y0 = wave_min_i(y_lo), Y1 = wave_max_i(y_hi);
-    const int X0 = wave_min_i(x_lo), X1 = wave_max_i(x_hi);
-    const int sy0 = wave_min_i(mine ? y_lo : INT_MAX), sy1 = wave_max_i(mine ? y_hi : INT_MIN);
-    const int sx0 = wave_min_i(mine ? x_lo : INT_MAX), sx1 = wave_max_i(mine ? x_hi : INT_MIN);
-    int oyi[4], oxi[4];
+    // the block's box and this wave's part of it are the same in every lane: scalar registers
+    const int Y0 = __builtin_amdgcn_readfirstlane(wave_min_i(y_lo));
+    const int Y1 = __builtin_amdgcn_readfirstlane(wave_max_i(y_hi));
+    const int X0 = __builtin_amdgcn_readfirstlane(wave_min_i(x_lo));
+    const int X1 = __builtin_amdgcn_readfirstlane(wave_max_i(x_hi));
+    const int sy0 = __builtin_amdgcn_readfirstlane(wave_min_i(mine ? y_lo : INT_MAX));
+    const int sy1 = __builtin_amdgcn_readfirstlane(wave_max_i(mine ? y_hi : INT_MIN));
+    const int sx0 = __builtin_amdgcn_readfirstlane(wave_min_i(mine ? x_lo : INT_MAX));
+    const int sx1 = __builtin_amdgcn_readfirstlane(wave_max_i(mine ? x_hi : INT_MIN));
+    int ry[4], rx[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      oyi[i] = __shfl(oy, bpi[i], kWave);
-      oxi[i] = __shfl(ox, bpi[i], kWave);
+      ry[i] = ycol - __shfl(oy, bpi[i], kWave);
+      rx[i] = xcol - __shfl(ox, bpi[i], kWave);
     }
     if (wave == 0) {
       const float sc = 1.f / static_cast<float>(1 << lvl);
       const float x = cx * sc, y = cy * sc;
       s_frac[lane] = make_float2(x - floorf(x), y - floorf(y));
     }
-    for (int k = t; k < kApPix * TS; k += kApThreads) s_tap[k] = 0.f;
+    {
+      f32x4* tz = reinterpret_cast<f32x4*>(s_tap);
+      for (int k = t; k < kApPix * TS / 4; k += kApThreads) tz[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
     __syncthreads();
 
     const bool any = Y0 <= Y1;
-    const int TC = any ? (X1 - X0 + 8) / 8 : 0;          // tile columns (8 positions each)
-    const int NT = any ? ((Y1 - Y0 + 2) / 2) * TC : 0;   // tiles (2 rows each)
-    const int nch = (NT + TPC - 1) / TPC;
+    const int TC = any ? (X1 - X0 + 8) >> 3 : 0;         // tile columns (8 positions each)
+    const int TR = any ? (Y1 - Y0 + 2) >> 1 : 0;         // tile rows (2 positions each)
+    const int nch = (TR * TC + TPC - 1) / TPC;
     const T* lp = lv.p[lvl] + (b * N + (edge_ok ? fj : 0)) * (int64_t)Hl * Wl * kApC;
 
+    // staging: the k-th piece of thread t belongs to tile k / (SPT / TPC) of the chunk, position
+    // j of that tile and 16-byte piece pc of the position's channel vector
+    constexpr int PPT = SPT / TPC;                        // pieces per thread and tile
+    const int pj = ((t / PP) & 15), pc = t & (PP - 1);
     u32x4 st[SPT];
-    auto fetch = [&](int c) {
+    int fr = 0, fc = 0;                                   // tile cursor of the next chunk to fetch
+    auto fetch = [&]() {
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
-        const int piece = k * kApThreads + t;
-        const int q = piece / PP, pc = piece - q * PP;
-        const int tt = TPC * c + (q >> 4), j = q & 15;
-        const int tr = TC ? tt / TC : 0, tcl = tt - tr * TC;
-        const int Y = Y0 + 2 * tr + (j >> 3), X = X0 + 8 * tcl + (j & 7);
-        if (tt < NT && Y < Hl && X < Wl)
-          st[k] = reinterpret_cast<const u32x4*>(lp + ((int64_t)Y * Wl + X) * kApC)[pc];
+        int r = fr, c = fc + k / PPT;
+        while (c >= TC) {
+          c -= TC;
+          ++r;
+        }
+        const int j = PPT == 1 ? pj : ((k % PPT) * (kApThreads / PP) + t / PP) & 15;
+        const int Y = Y0 + 2 * r + (j >> 3), X = X0 + 8 * c + (j & 7);
+        if (r < TR && Y < Hl && X < Wl)
+          st[k] = reinterpret_cast<const u32x4*>(lp + (Y * Wl + X) * kApC)[pc];
         else
           st[k] = u32x4{0u, 0u, 0u, 0u};
       }
+      fc += TPC;
+      while (fc >= TC && TC > 0) {
+        fc -= TC;
+        ++fr;
+      }
     };
-    if (nch > 0) fetch(0);
+    if (nch > 0) fetch();
+    int mr = 0, mc = 0;                                   // tile cursor of the chunk in compute
     for (int c = 0; c < nch; ++c) {
       unsigned char* buf = s_b + (c & 1) * CHUNK * ROWB;
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         const int piece = k * kApThreads + t;
-        const int q = piece / PP, pc = piece - q * PP;
+        const int q = piece / PP;
         *reinterpret_cast<u32x4*>(buf + q * ROWB + pc * 16) = st[k];
       }
       __syncthreads();
-      if (c + 1 < nch) fetch(c + 1);
+      if (c + 1 < nch) fetch();
 #pragma unroll
       for (int t4 = 0; t4 < TPC; ++t4) {
-        const int tt = TPC * c + t4;
-        if (tt >= NT) break;
-        const int tr = tt / TC;
-        const int Ty = Y0 + 2 * tr, Tx = X0 + 8 * (tt - tr * TC);
+        int r = mr, cc = mc + t4;
+        while (cc >= TC) {
+          cc -= TC;
+          ++r;
+        }
+        if (r >= TR) break;
+        const int Ty = Y0 + 2 * r, Tx = X0 + 8 * cc;
         if (Ty > sy1 || Ty + 1 < sy0 || Tx > sx1 || Tx + 7 < sx0) continue;   // wave-uniform
         u32x4 bb[NV];
         const u32x4* src = reinterpret_cast<const u32x4*>(buf + (t4 * 16 + col) * ROWB + g * SLICE * sizeof(T));
@@ -237,30 +270,48 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < ApMma<T>::kSteps; ++s) acc = ApMma<T>::step(a, bb, s, acc);
-        const int Y = Ty + (col >> 3), X = Tx + (col & 7);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int dy = Y - oyi[i], dx = X - oxi[i];
-          if (static_cast<unsigned>(dy) < static_cast<unsigned>(D) && static_cast<unsigned>(dx) < static_cast<unsigned>(D))
-            s_tap[bpi[i] * TS + dy * D + dx] = acc[i];
+          const int dy = Ty + ry[i], dx = Tx + rx[i];
+          const bool in = static_cast<unsigned>(dy) < static_cast<unsigned>(D) &&
+                          static_cast<unsigned>(dx) < static_cast<unsigned>(D);
+          s_tap[in ? tbase[i] + dy * D + dx : trash] = acc[i];
         }
+      }
+      mc += TPC;
+      while (mc >= TC && TC > 0) {
+        mc -= TC;
+        ++mr;
       }
     }
     __syncthreads();
 
-    // bilinear combine: each wave writes one output channel of the 64 pixels per pass
-    T* ob = out + (bm * lv.L + lvl) * (int64_t)(R * R) * HW;
-    for (int idx = t; idx < R * R * kApPix; idx += kApThreads) {
-      const int o = idx >> 6, P = idx & 63;
+    // bilinear combine: thread t takes block pixel P = t % 64 and the tap rows by = t / 64 + 4 n;
+    // each pass reads two rows of taps once and forms the R outputs (ax = 0 .. R-1) of row by
+    {
+      const int P = t & 63;
       const int y = ty0 + (P >> 4), x = tx0 + (P & 15);
-      if (y >= H || x >= W) continue;
-      const int ax = o / R, by = o - ax * R;
-      const float2 fr = s_frac[P];
-      const float* tp = s_tap + P * TS + by * D + ax;
-      const float w00 = (1.f - fr.x) * (1.f - fr.y) * 0.0625f, w01 = fr.x * (1.f - fr.y) * 0.0625f;
-      const float w10 = (1.f - fr.x) * fr.y * 0.0625f, w11 = fr.x * fr.y * 0.0625f;
-      const float v = w00 * tp[0] + w01 * tp[1] + w10 * tp[D] + w11 * tp[D + 1];
-      ob[(int64_t)o * HW + y * W + x] = from_f<T>(v);
+      const bool pok = y < H && x < W;
+      const float2 frc = s_frac[P];
+      const float w00 = (1.f - frc.x) * (1.f - frc.y) * 0.0625f, w01 = frc.x * (1.f - frc.y) * 0.0625f;
+      const float w10 = (1.f - frc.x) * frc.y * 0.0625f, w11 = frc.x * frc.y * 0.0625f;
+      T* ob = out + (bm * lv.L + lvl) * (int64_t)(R * R) * HW + (pok ? y * W + x : 0);
+      for (int by = t >> 6; by < R; by += kApThreads / kApPix) {
+        const float* tp = s_tap + P * TS + by * D;
+        float r0[D], r1[D];
+#pragma unroll
+        for (int e = 0; e < D; ++e) {
+          r0[e] = tp[e];
+          r1[e] = tp[D + e];
+        }
+        if (pok) {
+#pragma unroll
+          for (int ax = 0; ax < R; ++ax) {
+            const float v = w00 * r0[ax] + w01 * r0[ax + 1] + w10 * r1[ax] + w11 * r1[ax + 1];
+            ob[(int64_t)(ax * R + by) * HW] = from_f<T>(v);
+          }
+        }
+      }
     }
     __syncthreads();
   }
@@ -322,7 +373,7 @@ int launch_altcorr_pyramid(const void* const* levels, int L, const float* coords
   const int64_t nb = B * M * tiles;
   HSL2_REQUIRE(nb < (1ll << 31), HSL2_ERR_SHAPE, "altcorr pyramid: too many blocks");
   constexpr int TS = D * D + 1;
-  const size_t lds = (kApPix * TS * 4 + 15) / 16 * 16 + 2 * ap_chunk<T>() * (kApC * sizeof(T) + 16);
+  const size_t lds = ((kApPix * TS + kApPix) * 4 + 15) / 16 * 16 + 2 * ap_chunk<T>() * (kApC * sizeof(T) + 16);
   hipLaunchKernelGGL((altcorr_pyramid_kernel<T, D>), dim3(static_cast<unsigned>(nb)), dim3(kApThreads), lds, s, lv,
                      coords, ii, jj, static_cast<T*>(corr), static_cast<int>(N), static_cast<int>(H),
                      static_cast<int>(W), static_cast<int>(M), B * M, tiles_x, tiles);
@@ -386,7 +437,7 @@ extern "C" int hsl2_altcorr_pyramid_forward(const void* const* levels, int num_l
   HSL2_REQUIRE(radius == 3 || radius == 4, HSL2_ERR_UNSUPPORTED, "altcorr pyramid: radius 3 or 4");
   HSL2_REQUIRE(C == kApC, HSL2_ERR_UNSUPPORTED, "altcorr pyramid: 128 channels");
   HSL2_REQUIRE(B >= 0 && N > 0 && M >= 0 && H > 0 && W > 0 && (H >> (num_levels - 1)) > 0 &&
-                   (W >> (num_levels - 1)) > 0 && H * W < (1 << 30),
+                   (W >> (num_levels - 1)) > 0 && H * W * C < (1ll << 31),
                HSL2_ERR_SHAPE, "altcorr pyramid: bad sizes");
   for (int l = 0; l < num_levels; ++l) {
     HSL2_REQUIRE(levels[l] && reinterpret_cast<uintptr_t>(levels[l]) % 16 == 0, HSL2_ERR_ARGUMENT,
