@@ -167,13 +167,13 @@ __device__ __forceinline__ int opaque_zero() {
   return z;
 }
 
-// A non-positive (or NaN) pivot is replaced by 1 and flagged, as the failed-LLT contract needs.
+// A non-positive (or NaN) pivot is flagged (info = 1, as Eigen's LLT reports a NumericalIssue)
+// and left in place: its reciprocal square root is then NaN or infinite, which only poisons
+// values that every consumer of the solve discards once info is set (the solution counts as
+// zero), and the pivot chain carries no compare-and-select.
 template <typename R>
 __device__ __forceinline__ R checked_pivot(R e, bool& bad) {
-  if (!(e > R(0))) {
-    bad = true;
-    e = R(1);
-  }
+  bad |= !(e > R(0));
   return e;
 }
 
