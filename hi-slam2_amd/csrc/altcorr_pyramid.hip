@@ -218,9 +218,8 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
     // j of that tile and 16-byte piece pc of the position's channel vector
     constexpr int PPT = SPT / TPC;                        // pieces per thread and tile
     const int pj = ((t / PP) & 15), pc = t & (PP - 1);
-    u32x4 st[SPT];
     int fr = 0, fc = 0;                                   // tile cursor of the next chunk to fetch
-    auto fetch = [&]() {
+    auto fetch = [&](u32x4 (&st)[SPT]) {
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
         int r = fr, c = fc + k / PPT;
@@ -241,9 +240,10 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
         ++fr;
       }
     };
-    if (nch > 0) fetch();
     int mr = 0, mc = 0;                                   // tile cursor of the chunk in compute
-    for (int c = 0; c < nch; ++c) {
+    // chunk c: its staged pieces (fetched two chunks earlier) to LDS buffer c % 2, one barrier,
+    // the fetch of chunk c + 2 into the freed registers, then the chunk's tiles
+    auto chunk = [&](int c, u32x4 (&st)[SPT]) {
       unsigned char* buf = s_b + (c & 1) * CHUNK * ROWB;
 #pragma unroll
       for (int k = 0; k < SPT; ++k) {
@@ -252,7 +252,7 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
         *reinterpret_cast<u32x4*>(buf + q * ROWB + pc * 16) = st[k];
       }
       __syncthreads();
-      if (c + 1 < nch) fetch();
+      if (c + 2 < nch) fetch(st);
 #pragma unroll
       for (int t4 = 0; t4 < TPC; ++t4) {
         int r = mr, cc = mc + t4;
@@ -283,6 +283,13 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
         mc -= TC;
         ++mr;
       }
+    };
+    u32x4 st0[SPT], st1[SPT];
+    if (nch > 0) fetch(st0);
+    if (nch > 1) fetch(st1);
+    for (int c = 0; c < nch; c += 2) {
+      chunk(c, st0);
+      if (c + 1 < nch) chunk(c + 1, st1);
     }
     __syncthreads();
 
