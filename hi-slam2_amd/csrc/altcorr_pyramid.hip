@@ -107,14 +107,16 @@ __global__ void __launch_bounds__(kApThreads)
 altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const int64_t* __restrict__ ii,
                        const int64_t* __restrict__ jj, T* __restrict__ out, int N, int H, int W, int M,
                        int64_t BM, int tiles_x, int tiles) {
-  constexpr int R = D - 1, D2 = D * D, TS = D2 + 1;
+  // taps of a pixel: D x D floats, tap (dy, dx) at dy D + dx; TS odd, so that the combine's
+  // per-pixel reads hit distinct banks
+  constexpr int R = D - 1, TS = D * D + 1;
   constexpr int SLICE = kApC / 4;                         // channels per lane group
   constexpr int NV = SLICE * sizeof(T) / 16;              // 16-byte pieces per lane slice
   constexpr int PP = kApC * sizeof(T) / 16;               // 16-byte pieces per map position
   constexpr int ROWB = kApC * sizeof(T) + 16;             // padded LDS bytes per staged position
   constexpr int CHUNK = ap_chunk<T>(), TPC = CHUNK / 16;  // positions and tiles per chunk
   constexpr int SPT = CHUNK * PP / kApThreads;            // staged pieces per thread
-  constexpr int TAPB = ((kApPix * TS + kApPix) * 4 + 15) / 16 * 16;   // taps + one spare slot per lane
+  constexpr int TAPB = ((kApPix * TS + 4 * kApPix) * 4 + 15) / 16 * 16;   // taps + four spare floats per lane
   extern __shared__ __align__(16) unsigned char smem[];
   float* s_tap = reinterpret_cast<float*>(smem);
   unsigned char* s_b = smem + TAPB;
@@ -157,23 +159,17 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
     cx = c[0];
     cy = c[1];
   }
-  // the block pixels of the MFMA result rows 4g + i
-  int bpi[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int p = 4 * g + i;
-    bpi[i] = (p >> 2) * kApCols + wave * 4 + (p & 3);
-  }
   const bool mine = ((lane & 15) >> 2) == wave;        // lane's block pixel lies in this wave's sub-patch
 
-  // scatter geometry of this lane's MFMA results: column `col` is map position (ycol, xcol) of a
-  // tile, rows 4g + i are the block pixels bpi[i]; results outside a pixel's window go to this
-  // lane's own spare slot behind the taps (never read), so the stores need no branch
-  const int ycol = col >> 3, xcol = col & 7;
-  const int trash = kApPix * TS + lane;
-  int tbase[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) tbase[i] = bpi[i] * TS;
+  // The MFMA runs with the staged map positions as its rows (A) and the wave's 16 pixels as its
+  // columns (B), so lane (g, col) receives the results of pixel `col` (block pixel bp) at the
+  // tile's positions 4g .. 4g + 3: row g / 2 of the tile, columns 4 (g % 2) .. 4 (g % 2) + 3,
+  // four consecutive taps of one window row sharing one address computation.  A result outside
+  // the pixel's window goes to one of this lane's four spare floats behind the taps (never
+  // read), so the stores need no branch.
+  const int bp = (col >> 2) * kApCols + wave * 4 + (col & 3);
+  const int trash = kApPix * TS + 4 * lane;
+  const int tb = bp * TS;
 
   for (int lvl = 0; lvl < lv.L; ++lvl) {
     const int Hl = lv.h[lvl], Wl = lv.w[lvl];
@@ -191,12 +187,8 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
     const int sy1 = __builtin_amdgcn_readfirstlane(wave_max_i(mine ? y_hi : INT_MIN));
     const int sx0 = __builtin_amdgcn_readfirstlane(wave_min_i(mine ? x_lo : INT_MAX));
     const int sx1 = __builtin_amdgcn_readfirstlane(wave_max_i(mine ? x_hi : INT_MIN));
-    int ry[4], rx[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      ry[i] = ycol - __shfl(oy, bpi[i], kWave);
-      rx[i] = xcol - __shfl(ox, bpi[i], kWave);
-    }
+    const int ry = (g >> 1) - __shfl(oy, bp, kWave);
+    const int rx = 4 * (g & 1) - __shfl(ox, bp, kWave);
     if (wave == 0) {
       const float sc = 1.f / static_cast<float>(1 << lvl);
       const float x = cx * sc, y = cy * sc;
@@ -204,7 +196,7 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
     }
     {
       f32x4* tz = reinterpret_cast<f32x4*>(s_tap);
-      for (int k = t; k < kApPix * TS / 4; k += kApThreads) tz[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int k = t; k < (kApPix * TS + 3) / 4; k += kApThreads) tz[k] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
     __syncthreads();
 
@@ -269,13 +261,14 @@ altcorr_pyramid_kernel(ApLevels<T> lv, const float* __restrict__ coords, const i
         for (int k = 0; k < NV; ++k) bb[k] = src[k];
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < ApMma<T>::kSteps; ++s) acc = ApMma<T>::step(a, bb, s, acc);
+        for (int s = 0; s < ApMma<T>::kSteps; ++s) acc = ApMma<T>::step(bb, a, s, acc);
+        const int dy = Ty + ry, dx = Tx + rx;
+        const bool row = static_cast<unsigned>(dy) < static_cast<unsigned>(D);
+        const int at = tb + dy * D + dx;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int dy = Ty + ry[i], dx = Tx + rx[i];
-          const bool in = static_cast<unsigned>(dy) < static_cast<unsigned>(D) &&
-                          static_cast<unsigned>(dx) < static_cast<unsigned>(D);
-          s_tap[in ? tbase[i] + dy * D + dx : trash] = acc[i];
+        for (int v = 0; v < 4; ++v) {
+          const bool in = row && static_cast<unsigned>(dx + v) < static_cast<unsigned>(D);
+          s_tap[in ? at + v : trash + v] = acc[v];
         }
       }
       mc += TPC;
@@ -380,7 +373,7 @@ int launch_altcorr_pyramid(const void* const* levels, int L, const float* coords
   const int64_t nb = B * M * tiles;
   HSL2_REQUIRE(nb < (1ll << 31), HSL2_ERR_SHAPE, "altcorr pyramid: too many blocks");
   constexpr int TS = D * D + 1;
-  const size_t lds = ((kApPix * TS + kApPix) * 4 + 15) / 16 * 16 + 2 * ap_chunk<T>() * (kApC * sizeof(T) + 16);
+  const size_t lds = ((kApPix * TS + 4 * kApPix) * 4 + 15) / 16 * 16 + 2 * ap_chunk<T>() * (kApC * sizeof(T) + 16);
   hipLaunchKernelGGL((altcorr_pyramid_kernel<T, D>), dim3(static_cast<unsigned>(nb)), dim3(kApThreads), lds, s, lv,
                      coords, ii, jj, static_cast<T*>(corr), static_cast<int>(N), static_cast<int>(H),
                      static_cast<int>(W), static_cast<int>(M), B * M, tiles_x, tiles);
