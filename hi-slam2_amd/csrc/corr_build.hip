@@ -254,7 +254,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
 // flight); NG == 0: any multiple of G, one group at a time.
 constexpr int kPipe = 3;
 
-template <typename Ti, typename To, int WT, int NG, int PB>
+template <typename Ti, typename To, int WT, int NG>
 __global__ void __launch_bounds__(kThreads)
 corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __restrict__ l0,
                   To* __restrict__ l1, To* __restrict__ l2, To* __restrict__ l3, int C, int h,
@@ -279,7 +279,7 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   const int n = static_cast<int>(lb / ((int64_t)ptiles * qtiles));
 
   const int HW = h * w;
-  const int p0 = pt * kTP * PB;
+  const int p0 = pt * kTP;
   const int qtiles_x = w / WT;
   const int y0 = (qt / qtiles_x) * kRows;
   const int x0 = (qt % qtiles_x) * WT;
@@ -289,9 +289,7 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   // each group; group g of the tile starting at pixel P (a multiple of 32) is at P C + g FT
   const int i = lane % 32, kg = lane / 32;
   const int lane_off = i * G + kg * (G / 2);
-  const Ti* ap[PB];
-#pragma unroll
-  for (int j = 0; j < PB; ++j) ap[j] = f1 + ((int64_t)n * HW + p0 + j * kTP) * C + lane_off;
+  const Ti* ap = f1 + ((int64_t)n * HW + p0) * C + lane_off;
   const Ti* bp[PW];
 #pragma unroll
   for (int k = 0; k < PW; ++k) {
@@ -302,20 +300,17 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
     bp[k] = f2 + ((int64_t)n * HW + qpix0) * C + lane_off;
   }
 
-  acc16 acc[PB][PW];
+  acc16 acc[PW];
 #pragma unroll
-  for (int j = 0; j < PB; ++j)
+  for (int k = 0; k < PW; ++k)
 #pragma unroll
-    for (int k = 0; k < PW; ++k)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) acc[j][k][v] = 0.f;
+    for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
   if constexpr (NG > 0) {
     constexpr int D = NG < kPipe ? NG : kPipe;
-    Frag fa[D][PB], fb[D][PW];
+    Frag fa[D], fb[D][PW];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-#pragma unroll
-      for (int j = 0; j < PB; ++j) fa[d][j] = M::load(ap[j] + d * FT);
+      fa[d] = M::load(ap + d * FT);
 #pragma unroll
       for (int k = 0; k < PW; ++k) fb[d][k] = M::load(bp[k] + d * FT);
     }
@@ -326,30 +321,24 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
 #pragma unroll
       for (int st = 0; st < M::S; ++st)
 #pragma unroll
-        for (int j = 0; j < PB; ++j)
-#pragma unroll
-          for (int k = 0; k < PW; ++k) acc[j][k] = M::run(fa[sl][j], fb[sl][k], st, acc[j][k]);
+        for (int k = 0; k < PW; ++k) acc[k] = M::run(fa[sl], fb[sl][k], st, acc[k]);
       __builtin_amdgcn_sched_barrier(0);
       if (g + D < NG) {
-#pragma unroll
-        for (int j = 0; j < PB; ++j) fa[sl][j] = M::load(ap[j] + (g + D) * FT);
+        fa[sl] = M::load(ap + (g + D) * FT);
 #pragma unroll
         for (int k = 0; k < PW; ++k) fb[sl][k] = M::load(bp[k] + (g + D) * FT);
       }
     }
   } else {
     for (int g = 0; g < C / G; ++g) {
-      Frag af[PB], bf[PW];
-#pragma unroll
-      for (int j = 0; j < PB; ++j) af[j] = M::load(ap[j] + g * FT);
+      const Frag a = M::load(ap + g * FT);
+      Frag bf[PW];
 #pragma unroll
       for (int k = 0; k < PW; ++k) bf[k] = M::load(bp[k] + g * FT);
 #pragma unroll
       for (int st = 0; st < M::S; ++st)
 #pragma unroll
-        for (int j = 0; j < PB; ++j)
-#pragma unroll
-          for (int k = 0; k < PW; ++k) acc[j][k] = M::run(af[j], bf[k], st, acc[j][k]);
+        for (int k = 0; k < PW; ++k) acc[k] = M::run(a, bf[k], st, acc[k]);
     }
   }
 
@@ -357,31 +346,28 @@ corr_build_kernel(const Ti* __restrict__ f1, const Ti* __restrict__ f2, To* __re
   // is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32 (rows 0-15 are values 0-7 of each
   // lane, rows 16-31 values 8-15)
 #pragma unroll
-  for (int j = 0; j < PB; ++j)
+  for (int part = 0; part < kTP / kER; ++part) {
 #pragma unroll
-    for (int part = 0; part < kTP / kER; ++part) {
+    for (int k = 0; k < PW; ++k) {
+      const int q = (wave * PW + k) * 32 + lane % 32;
 #pragma unroll
-      for (int k = 0; k < PW; ++k) {
-        const int q = (wave * PW + k) * 32 + lane % 32;
-#pragma unroll
-        for (int v = 8 * part; v < 8 * part + 8; ++v) {
-          const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4) - kER * part;
-          sout[p * SQ + q] = acc[j][k][v] * 0.0625f;
-        }
+      for (int v = 8 * part; v < 8 * part + 8; ++v) {
+        const int p = 8 * (v / 4) + 4 * (lane / 32) + (v % 4) - kER * part;
+        sout[p * SQ + q] = acc[k][v] * 0.0625f;
       }
-      __syncthreads();
-      emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + j * kTP + part * kER, h, w, y0, x0,
-                        levels, tiled);
     }
+    __syncthreads();
+    emit_rows<To, WT>(sout, sout, l0, l1, l2, l3, (int64_t)n * HW + p0 + part * kER, h, w, y0, x0, levels, tiled);
+  }
 }
 
-template <typename Ti, typename To, int WT, int NG, int PB>
+template <typename Ti, typename To, int WT, int NG>
 int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
               int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
   constexpr int NQ = kRows * WT, SQ = NQ + 4;
   const size_t lds = sizeof(float) * kER * SQ;
   HSL2_REQUIRE(lds <= 160 * 1024, HSL2_ERR_UNSUPPORTED, "corr build: LDS budget exceeded");
-  const int ptiles = static_cast<int>(h * w / (kTP * PB));
+  const int ptiles = static_cast<int>(h * w / kTP);
   const int qtiles = static_cast<int>((h / kRows) * (w / WT));
   const int64_t nblocks = (int64_t)ptiles * qtiles * batch;
   if (nblocks == 0) return HSL2_OK;
@@ -396,7 +382,7 @@ int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch
   }
   To* l[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < L; ++i) l[i] = static_cast<To*>(levels[i]);
-  hipLaunchKernelGGL((corr_build_kernel<Ti, To, WT, NG, PB>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
+  hipLaunchKernelGGL((corr_build_kernel<Ti, To, WT, NG>), dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), lds, s,
                      static_cast<const Ti*>(f1), static_cast<const Ti*>(f2), l[0], l[1], l[2], l[3],
                      static_cast<int>(C), static_cast<int>(h), static_cast<int>(w), L, tiled, ptiles, qtiles,
                      nblocks);
@@ -408,15 +394,10 @@ int launch_wt(const void* f1, const void* f2, void* const* levels, int64_t batch
 template <typename Ti, typename To, int NG>
 int launch_ng(const void* f1, const void* f2, void* const* levels, int64_t batch, int64_t C,
               int64_t h, int64_t w, int L, int tiled, hipStream_t s) {
-  // two 32-row p blocks per wave where the plane allows it: each fetched fmap2 fragment then
-  // feeds twice the MFMA work
-  const bool pb2 = NG > 0 && (h * w) % (2 * kTP) == 0;
-  if (w % 64 == 0)
-    return pb2 ? launch_wt<Ti, To, 64, NG, 2>(f1, f2, levels, batch, C, h, w, L, tiled, s)
-               : launch_wt<Ti, To, 64, NG, 1>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  if (w == 48) return launch_wt<Ti, To, 48, NG, 1>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  if (w == 32) return launch_wt<Ti, To, 32, NG, 1>(f1, f2, levels, batch, C, h, w, L, tiled, s);
-  if (w == 16) return launch_wt<Ti, To, 16, NG, 1>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w % 64 == 0) return launch_wt<Ti, To, 64, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 48) return launch_wt<Ti, To, 48, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 32) return launch_wt<Ti, To, 32, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
+  if (w == 16) return launch_wt<Ti, To, 16, NG>(f1, f2, levels, batch, C, h, w, L, tiled, s);
   set_error("corr build: width must be a multiple of 64, or 16, 32 or 48");
   return HSL2_ERR_UNSUPPORTED;
 }
