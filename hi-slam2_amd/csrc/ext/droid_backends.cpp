@@ -1134,22 +1134,27 @@ std::vector<torch::Tensor> pgba(torch::Tensor poses, torch::Tensor disps, torch:
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "MI355X (gfx950) dense-tracking kernels behind the droid_backends interface";
+  // The calls run without the GIL (they touch no Python object; the plan caches are guarded by
+  // their own mutexes and the library's error message is per thread), so that a tracking thread
+  // in ba does not stall the other Python threads.  corr_build and altcorr_pyramid read a
+  // Python dtype argument and keep the GIL.
+  const auto kNoGil = py::call_guard<py::gil_scoped_release>();
   // bundle adjustment (src/droid.cpp:327)
-  m.def("ba", &ba, "dense bundle adjustment (split precision: FP32 linearization, FP64 reduced solve)");
-  m.def("ba", &ba_sens, "dense bundle adjustment with a sensor-disparity prior (15-argument form)");
-  m.def("pgba", &pgba, "Sim3 pose-graph bundle adjustment step: [dx, dz], disps updated in place");
+  m.def("ba", &ba, "dense bundle adjustment (split precision: FP32 linearization, FP64 reduced solve)", kNoGil);
+  m.def("ba", &ba_sens, "dense bundle adjustment with a sensor-disparity prior (15-argument form)", kNoGil);
+  m.def("pgba", &pgba, "Sim3 pose-graph bundle adjustment step: [dx, dz], disps updated in place", kNoGil);
   // correlation volume kernels (src/droid.cpp:337-340)
-  m.def("altcorr_forward", &altcorr_forward, "on-the-fly correlation lookup");
-  m.def("altcorr_backward", &altcorr_backward, "on-the-fly correlation lookup adjoint");
-  m.def("corr_index_forward", &corr_index_forward, "radius-r lookup into a correlation volume");
-  m.def("corr_index_backward", &corr_index_backward, "adjoint of corr_index_forward");
+  m.def("altcorr_forward", &altcorr_forward, "on-the-fly correlation lookup", kNoGil);
+  m.def("altcorr_backward", &altcorr_backward, "on-the-fly correlation lookup adjoint", kNoGil);
+  m.def("corr_index_forward", &corr_index_forward, "radius-r lookup into a correlation volume", kNoGil);
+  m.def("corr_index_backward", &corr_index_backward, "adjoint of corr_index_forward", kNoGil);
   // additions of this build
-  m.def("ba_ex", &ba_ex, "ba with an explicit precision mode (0 split, 1 all-FP32); returns [dx, dz, None, ok]");
-  m.def("ba_index", &ba_index, "kx and kk_exp of a ba call (host-side planner)");
-  m.def("spd_solve", &spd_solve, "dense SPD solve with the tiled GPU Cholesky; returns [x, ok]");
-  m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch",
+  m.def("ba_ex", &ba_ex, "ba with an explicit precision mode (0 split, 1 all-FP32); returns [dx, dz, None, ok]", kNoGil);
+  m.def("ba_index", &ba_index, "kx and kk_exp of a ba call (host-side planner)", kNoGil);
+  m.def("spd_solve", &spd_solve, "dense SPD solve with the tiled GPU Cholesky; returns [x, ok]", kNoGil);
+  m.def("corr_pyramid_lookup", &corr_pyramid_lookup, "CorrBlock lookup over all levels in one launch", kNoGil,
         py::arg("levels"), py::arg("coords"), py::arg("radius"), py::arg("tiled") = false);
-  m.def("corr_lookup_conv", &corr_lookup_conv, "CorrBlock lookup fused with a 1x1 convolution (+ ReLU)",
+  m.def("corr_lookup_conv", &corr_lookup_conv, "CorrBlock lookup fused with a 1x1 convolution (+ ReLU)", kNoGil,
         py::arg("levels"), py::arg("coords"), py::arg("radius"), py::arg("tiled"), py::arg("weight"),
         py::arg("bias"), py::arg("relu") = true);
   m.def("corr_build", &corr_build, "all-pairs correlation + average-pool pyramid", py::arg("fmap1"),
@@ -1158,42 +1163,42 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("altcorr_pyramid", &altcorr_pyramid, "channel-last feature pyramid of AltCorrBlock", py::arg("fmaps"),
         py::arg("num_levels") = 4, py::arg("dtype") = py::none());
   m.def("altcorr_pyramid_forward", &altcorr_pyramid_forward,
-        "AltCorrBlock lookup over all levels in one launch on the matrix cores", py::arg("levels"), py::arg("coords"),
+        "AltCorrBlock lookup over all levels in one launch on the matrix cores", kNoGil, py::arg("levels"), py::arg("coords"),
         py::arg("ii"), py::arg("jj"), py::arg("radius") = 3);
-  m.def("fp8_quantize", &fp8_quantize, "per-channel-scaled e4m3fn quantisation: [q (channel-last uint8), scales]");
-  m.def("altcorr_forward_fp8", &altcorr_forward_fp8, "altcorr lookup on fp8 feature maps", py::arg("q1"),
+  m.def("fp8_quantize", &fp8_quantize, "per-channel-scaled e4m3fn quantisation: [q (channel-last uint8), scales]", kNoGil);
+  m.def("altcorr_forward_fp8", &altcorr_forward_fp8, "altcorr lookup on fp8 feature maps", kNoGil, py::arg("q1"),
         py::arg("s1"), py::arg("q2"), py::arg("s2"), py::arg("coords"), py::arg("ii"), py::arg("jj"),
         py::arg("radius"), py::arg("out_float") = false);
-  m.def("proj_trans", &proj_trans, "depth-only terms summed per ii frame: [C, w]", py::arg("poses"),
+  m.def("proj_trans", &proj_trans, "depth-only terms summed per ii frame: [C, w]", kNoGil, py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("targets"), py::arg("weights"), py::arg("ii"),
         py::arg("jj"));
-  m.def("frame_distance", &frame_distance, "beta-weighted mean flow of each edge", py::arg("poses"),
+  m.def("frame_distance", &frame_distance, "beta-weighted mean flow of each edge", kNoGil, py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("ii"), py::arg("jj"), py::arg("beta"));
-  m.def("covis_distance", &covis_distance, "fraction of pixels leaving both neighbours", py::arg("poses"),
+  m.def("covis_distance", &covis_distance, "fraction of pixels leaving both neighbours", kNoGil, py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("ii"));
-  m.def("depth_filter", &depth_filter, "multi-view depth consistency counts", py::arg("poses"), py::arg("disps"),
+  m.def("depth_filter", &depth_filter, "multi-view depth consistency counts", kNoGil, py::arg("poses"), py::arg("disps"),
         py::arg("intrinsics"), py::arg("ix"), py::arg("thresh"));
-  m.def("iproj", &iproj, "pixels lifted to 3-D points", py::arg("poses"), py::arg("disps"), py::arg("intrinsics"));
-  m.def("jdsa_solve", &jdsa_solve, "JDSA reduced system per frame slot and update: [disps, dzcov]",
+  m.def("iproj", &iproj, "pixels lifted to 3-D points", kNoGil, py::arg("poses"), py::arg("disps"), py::arg("intrinsics"));
+  m.def("jdsa_solve", &jdsa_solve, "JDSA reduced system per frame slot and update: [disps, dzcov]", kNoGil,
         py::arg("C"), py::arg("w"), py::arg("disps"), py::arg("prior"), py::arg("dscales"), py::arg("eta"),
         py::arg("kx"), py::arg("slot_src"), py::arg("gx"), py::arg("gy"), py::arg("alpha"), py::arg("ep"),
         py::arg("lm"));
-  m.def("bi_inter", &bi_inter, "bilinear interpolation of scale grids: [outputs, Jacobis]", py::arg("scales"),
+  m.def("bi_inter", &bi_inter, "bilinear interpolation of scale grids: [outputs, Jacobis]", kNoGil, py::arg("scales"),
         py::arg("grids"));
   m.def("projective_transform", &projective_transform,
-        "torch-formulation projective transform: [coords, valid, Ji, Jj, Jz]", py::arg("poses"),
+        "torch-formulation projective transform: [coords, valid, Ji, Jj, Jz]", kNoGil, py::arg("poses"),
         py::arg("disps"), py::arg("intrinsics"), py::arg("ii"), py::arg("jj"), py::arg("jacobian") = false,
         py::arg("return_depth") = false);
   m.def("projective_transform_backward", &projective_transform_backward,
-        "adjoint of projective_transform: [grad_poses (tangent), grad_disps]");
+        "adjoint of projective_transform: [grad_poses (tangent), grad_disps]", kNoGil);
   py::class_<GNStep>(m, "GNStep", "one Gauss-Newton step of the torch formulation (hislam2/geom/ba.py) "
                                   "with its hand-written adjoint")
       .def(py::init<torch::Tensor, torch::Tensor, int64_t, int64_t, int64_t, int, bool>(), py::arg("ii"),
            py::arg("jj"), py::arg("num_poses"), py::arg("ht"), py::arg("wd"), py::arg("fixedp"),
            py::arg("motion_only") = false)
       .def("frames", &GNStep::frames)
-      .def("forward", &GNStep::forward)
-      .def("backward", &GNStep::backward);
+      .def("forward", &GNStep::forward, kNoGil)
+      .def("backward", &GNStep::backward, kNoGil);
   m.attr("PRECISION_SPLIT") = py::int_(static_cast<int>(HSL2_BA_SPLIT));
   m.attr("PRECISION_FP32") = py::int_(static_cast<int>(HSL2_BA_FP32));
 }
