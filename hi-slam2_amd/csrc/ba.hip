@@ -4,10 +4,11 @@
 // Per Gauss-Newton iteration:
 //   1. ba_linearize      one block per edge: relative pose, per-pixel residuals and
 //                        Jacobians, the 12x12 Hessian and gradient of the edge
-//                        reduced through LDS, and the per-pixel Schur terms
+//                        reduced through LDS, and the pose-j Schur rows E_ij
 //                        (src/droid_kernels.cu:364-606).
-//   2. ba_frame_reduce   per depth frame: C, w, Q = 1/(C+eta) and the pose-i block
-//                        Ei of the frame's edges (accum_cuda, 1413-1463, 1901-1906).
+//   2. ba_frame_sums     per depth frame, from the pixels (each edge's rows recomputed):
+//                        C, w, Q = 1/(C+eta) and the pose-i block Ei of the frame's
+//                        edges (accum_cuda, 1413-1463, 1901-1906).
 //   3. ba_schur_gram_lds per depth frame, 16x16 tiles of the slots' Gram matrix
 //                        S = sum_px E_a Q E_b^T on the matrix cores, one block per group
 //                        pair of row tiles staged in LDS by 32-pixel chunks (FP32 tile
@@ -43,9 +44,6 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
                    int eta_rows, int iterations, float lm, float ep, float* dx, float* dz, char* ws,
                    const int* blob, hipStream_t s) {
   float* Eall = reinterpret_cast<float*>(ws + p->w_Eall);
-  float* Eii = reinterpret_cast<float*>(ws + p->w_Eii);
-  float* Cii = reinterpret_cast<float*>(ws + p->w_Cii);
-  float* bz = reinterpret_cast<float*>(ws + p->w_bz);
   float* Q = reinterpret_cast<float*>(ws + p->w_Q);
   float* wv = reinterpret_cast<float*>(ws + p->w_w);
   // sums in the solve type: FP64 in the split mode, FP32 in the all-FP32 mode
@@ -75,7 +73,7 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
       if (schur)
         hipLaunchKernelGGL((ba_linearize<true, kSemCuda, R>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
-                           Hs, vs, Eall + (int64_t)P * 6 * HW, Eii, Cii, bz, zl, Hpart);
+                           Hs, vs, Eall + (int64_t)P * 6 * HW, nullptr, nullptr, nullptr, zl, Hpart);
       else
         hipLaunchKernelGGL((ba_linearize<false, kSemCuda, R>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
@@ -88,9 +86,11 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
       }
     }
     if (schur) {
-      launch_frame_reduce<R>(Cii, bz, Eii, eta, eta_rows, blob + p->o_frame_eptr, blob + p->o_frame_edges,
-                             blob + p->o_frame_pose, blob + p->o_frame_order, HW, static_cast<int>(F), 0.f, Q, wv,
-                             Eall, s, disps, disps_sens, blob + p->o_frame_kx, kSensAlpha);
+      hipLaunchKernelGGL((ba_frame_sums<kSemCuda, R>), dim3((HW + 255) / 256, F), dim3(256), 0, s, poses, disps,
+                         intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, eta, eta_rows,
+                         blob + p->o_frame_eptr, blob + p->o_frame_edges, blob + p->o_frame_pose,
+                         blob + p->o_frame_order, blob + p->o_frame_kx, HW, wd, 0.f, Q, wv, Eall, disps_sens,
+                         kSensAlpha);
       HSL2_LAUNCH_CHECK();
       launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, Spart, s);
       HSL2_LAUNCH_CHECK();

@@ -82,11 +82,14 @@ template <int kSem>
 __device__ __forceinline__ bool pixel_rows(const EdgeGeom& g, float u, float v, float d,
                                            float tu, float tv, float wtu, float wtv,
                                            RowJac& ru, RowJac& rv) {
+  // explicit multiply-adds and no implicit contraction: the rows are formed alike wherever they
+  // are inlined
+#pragma clang fp contract(off)
   const float X0 = (u - g.cxi) / g.fxi;
   const float Y0 = (v - g.cyi) / g.fyi;
-  const float x = g.R[0] * X0 + g.R[1] * Y0 + g.R[2] + d * g.t[0];
-  const float y = g.R[3] * X0 + g.R[4] * Y0 + g.R[5] + d * g.t[1];
-  const float zr = g.R[6] * X0 + g.R[7] * Y0 + g.R[8] + d * g.t[2];
+  const float x = fmaf(d, g.t[0], fmaf(g.R[1], Y0, fmaf(g.R[0], X0, g.R[2])));
+  const float y = fmaf(d, g.t[1], fmaf(g.R[4], Y0, fmaf(g.R[3], X0, g.R[5])));
+  const float zr = fmaf(d, g.t[2], fmaf(g.R[7], Y0, fmaf(g.R[6], X0, g.R[8])));
   if (kSem == kSemCuda) {
     if (!(zr >= kMinDepth)) return false;
   } else {
@@ -97,37 +100,64 @@ __device__ __forceinline__ bool pixel_rows(const EdgeGeom& g, float u, float v, 
   const float d2 = dinv * dinv;
   ru.w = 0.001f * wtu;
   rv.w = 0.001f * wtv;
-  ru.r = tu - (g.fx * (x * dinv) + g.cx);
-  rv.r = tv - (g.fy * (y * dinv) + g.cy);
+  ru.r = tu - fmaf(g.fx, x * dinv, g.cx);
+  rv.r = tv - fmaf(g.fy, y * dinv, g.cy);
   const float P0 = g.fx * dinv, P2 = -g.fx * x * d2;   // u row: [P0, 0, P2]
   const float Q1 = g.fy * dinv, Q2 = -g.fy * y * d2;   // v row: [0, Q1, Q2]
   ru.Jj[0] = P0 * d;  ru.Jj[1] = 0.f;    ru.Jj[2] = P2 * d;
-  ru.Jj[3] = P2 * y;  ru.Jj[4] = P0 * z - P2 * x;  ru.Jj[5] = -P0 * y;
+  ru.Jj[3] = P2 * y;  ru.Jj[4] = fmaf(P0, z, -(P2 * x));  ru.Jj[5] = -P0 * y;
   rv.Jj[0] = 0.f;     rv.Jj[1] = Q1 * d; rv.Jj[2] = Q2 * d;
-  rv.Jj[3] = -Q1 * z + Q2 * y;  rv.Jj[4] = -Q2 * x;  rv.Jj[5] = Q1 * x;
-  ru.Jz = P0 * g.t[0] + P2 * g.t[2];
-  rv.Jz = Q1 * g.t[1] + Q2 * g.t[2];
+  rv.Jj[3] = fmaf(-Q1, z, Q2 * y);  rv.Jj[4] = -Q2 * x;  rv.Jj[5] = Q1 * x;
+  ru.Jz = fmaf(P0, g.t[0], P2 * g.t[2]);
+  rv.Jz = fmaf(Q1, g.t[1], Q2 * g.t[2]);
   // Ji = -Adj(G_ij)^T Jj: tau part -R^T Jt, phi part -R^T (Jr - t x Jt) (src/droid_kernels.cu:104-119)
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     RowJac& J = k == 0 ? ru : rv;
     const float a0 = J.Jj[0], a1 = J.Jj[1], a2 = J.Jj[2];
-    const float c0 = J.Jj[3] - (g.t[1] * a2 - g.t[2] * a1);
-    const float c1 = J.Jj[4] - (g.t[2] * a0 - g.t[0] * a2);
-    const float c2 = J.Jj[5] - (g.t[0] * a1 - g.t[1] * a0);
-    J.Ji[0] = -(g.R[0] * a0 + g.R[3] * a1 + g.R[6] * a2);
-    J.Ji[1] = -(g.R[1] * a0 + g.R[4] * a1 + g.R[7] * a2);
-    J.Ji[2] = -(g.R[2] * a0 + g.R[5] * a1 + g.R[8] * a2);
-    J.Ji[3] = -(g.R[0] * c0 + g.R[3] * c1 + g.R[6] * c2);
-    J.Ji[4] = -(g.R[1] * c0 + g.R[4] * c1 + g.R[7] * c2);
-    J.Ji[5] = -(g.R[2] * c0 + g.R[5] * c1 + g.R[8] * c2);
+    const float c0 = J.Jj[3] - fmaf(g.t[1], a2, -(g.t[2] * a1));
+    const float c1 = J.Jj[4] - fmaf(g.t[2], a0, -(g.t[0] * a2));
+    const float c2 = J.Jj[5] - fmaf(g.t[0], a1, -(g.t[1] * a0));
+    J.Ji[0] = -fmaf(g.R[6], a2, fmaf(g.R[3], a1, g.R[0] * a0));
+    J.Ji[1] = -fmaf(g.R[7], a2, fmaf(g.R[4], a1, g.R[1] * a0));
+    J.Ji[2] = -fmaf(g.R[8], a2, fmaf(g.R[5], a1, g.R[2] * a0));
+    J.Ji[3] = -fmaf(g.R[6], c2, fmaf(g.R[3], c1, g.R[0] * c0));
+    J.Ji[4] = -fmaf(g.R[7], c2, fmaf(g.R[4], c1, g.R[1] * c0));
+    J.Ji[5] = -fmaf(g.R[8], c2, fmaf(g.R[5], c1, g.R[2] * c0));
   }
   return true;
 }
 
 __device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
 
+// A valid pixel's terms of one edge in its depth frame's sums (FP32): the depth diagonal
+// c = w Jz^2, the depth gradient b = w r Jz and the pose-i coupling ei = w Jz Ji (zero for an
+// edge from a frame to itself).  The one definition serves ba_linearize's per-edge planes and
+// ba_frame_sums, so both form the same values.
+__device__ __forceinline__ void depth_terms(const RowJac& ru, const RowJac& rv, bool self, float& c, float& b,
+                                            float (&ei)[6]) {
+#pragma clang fp contract(off)
+  const float wzu = ru.w * ru.Jz, wzv = rv.w * rv.Jz;
+  c = fmaf(wzv, rv.Jz, wzu * ru.Jz);
+  b = fmaf(wzv, rv.r, wzu * ru.r);
+  if (!self) {
+#pragma unroll
+    for (int n = 0; n < 6; ++n) ei[n] = fmaf(wzv, rv.Ji[n], wzu * ru.Ji[n]);
+  }
+}
+
+// The pose-j coupling ej = w Jz Jj of a valid pixel of an edge between two frames, formed as
+// depth_terms forms ei.
+__device__ __forceinline__ void pose_j_terms(const RowJac& ru, const RowJac& rv, float (&ej)[6]) {
+#pragma clang fp contract(off)
+  const float wzu = ru.w * ru.Jz, wzv = rv.w * rv.Jz;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) ej[n] = fmaf(wzv, rv.Jj[n], wzu * ru.Jj[n]);
+}
+
 // ---- 1. per-edge linearization ----------------------------------------------
+// With Cii == nullptr the per-edge planes Cii, bz and Eii are not written (ba_frame_sums forms
+// the frame sums from the pixels instead); E_ij goes to Eij either way.
 template <bool kSchur, int kSem, typename Acc = double>
 __global__ void __launch_bounds__(kLinThreads)
 ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
@@ -176,24 +206,17 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
     const bool ok = pixel_rows<kSem>(g, static_cast<float>(col), static_cast<float>(row), d, tu, tv, wu,
                                wv, ru, rv);
     if (kSchur) {
-      float c = 0.f, b = 0.f, ei[6] = {0, 0, 0, 0, 0, 0}, ej[6] = {0, 0, 0, 0, 0, 0};
-      if (ok) {
-        c = ru.w * ru.Jz * ru.Jz + rv.w * rv.Jz * rv.Jz;
-        b = ru.w * ru.r * ru.Jz + rv.w * rv.r * rv.Jz;
-        if (!self) {
+      float ej[6] = {0, 0, 0, 0, 0, 0};
+      if (ok && !self) pose_j_terms(ru, rv, ej);
 #pragma unroll
-          for (int n = 0; n < 6; ++n) {
-            ei[n] = ru.w * ru.Jz * ru.Ji[n] + rv.w * rv.Jz * rv.Ji[n];
-            ej[n] = ru.w * ru.Jz * ru.Jj[n] + rv.w * rv.Jz * rv.Jj[n];
-          }
-        }
-      }
-      Cii[(int64_t)e * HW + k] = c;
-      bz[(int64_t)e * HW + k] = b;
+      for (int n = 0; n < 6; ++n) Eij[((int64_t)e * 6 + n) * HW + k] = ej[n];
+      if (Cii) {                                  // the per-edge planes of ba_frame_reduce
+        float c = 0.f, b = 0.f, ei[6] = {0, 0, 0, 0, 0, 0};
+        if (ok) depth_terms(ru, rv, self, c, b, ei);
+        Cii[(int64_t)e * HW + k] = c;
+        bz[(int64_t)e * HW + k] = b;
 #pragma unroll
-      for (int n = 0; n < 6; ++n) {
-        Eii[((int64_t)e * 6 + n) * HW + k] = ei[n];
-        Eij[((int64_t)e * 6 + n) * HW + k] = ej[n];
+        for (int n = 0; n < 6; ++n) Eii[((int64_t)e * 6 + n) * HW + k] = ei[n];
       }
     }
     if (ok && !self) {
@@ -354,6 +377,112 @@ inline void launch_frame_reduce(const float* Cii, const float* bz, const float* 
   hipLaunchKernelGGL(ba_frame_reduce<Acc>, dim3((HW + 255) / 256, F), dim3(256), 0, s, Cii, bz, Eii, eta, eta_rows,
                      frame_eptr, frame_edges, frame_pose, frame_order, HW, c_eps, Q, w, Eall, disps, disps_sens,
                      frame_kx, alpha);
+}
+
+// ---- 2'. per-frame sums straight from the pixels --------------------------------
+// The results of ba_frame_reduce (Q, w and the pose-i block Ei of each depth frame) without the
+// per-edge planes: thread k takes pixel k of frame f and, edge by edge in the frame's edge order,
+// recomputes the pixel's rows (pixel_rows, as ba_linearize does), forms the edge's depth_terms
+// in FP32 and adds them in Acc in the order in which ba_frame_reduce adds the stored planes
+// (an invalid pixel's or a self edge's zeros leave a sum unchanged), so the sums are the same.
+// Per pixel and edge it reads the targets and weights, 16 B, where ba_linearize wrote and
+// ba_frame_reduce read back Cii, bz and Eii, 2 x 32 B.  The edges' geometry is staged in LDS
+// kFsEdges at a time; the targets and weights of the next four edges are in flight while four
+// are summed.  The sensor-disparity prior is applied as in ba_frame_reduce.
+constexpr int kFsEdges = 64;
+
+template <int kSem, typename Acc>
+__global__ void __launch_bounds__(256)
+ba_frame_sums(const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
+              int intr_stride, const float* __restrict__ targets, const float* __restrict__ weights,
+              const int* __restrict__ edge_ii, const int* __restrict__ edge_jj, const float* __restrict__ eta,
+              int eta_rows, const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
+              const int* __restrict__ frame_pose, const int* __restrict__ frame_order,
+              const int* __restrict__ frame_kx, int HW, int wd, float c_eps, float* __restrict__ Q,
+              float* __restrict__ w, float* __restrict__ Eall, const float* __restrict__ disps_sens, float alpha) {
+  __shared__ EdgeGeom geo[kFsEdges];
+  __shared__ int eid[kFsEdges];
+  __shared__ int eself[kFsEdges];
+  const int f = frame_order[blockIdx.y];          // the frames with most edges first
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = k < HW;
+  const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
+  const int p = frame_pose[f];
+  const int64_t kx = frame_kx[f];
+  const int row = k / wd;
+  const float pu = static_cast<float>(k - row * wd), pv = static_cast<float>(row);
+  const float d = in ? disps[kx * HW + k] : 0.f;
+  Acc c = 0, b = 0, ei[6] = {0, 0, 0, 0, 0, 0};
+  for (int base = e0; base < e1; base += kFsEdges) {
+    const int cnt = min(kFsEdges, e1 - base);
+    __syncthreads();                              // the previous chunk's geometry is consumed
+    if (static_cast<int>(threadIdx.x) < cnt) {
+      const int e = frame_edges[base + threadIdx.x];
+      const int ix = edge_ii[e], jx = edge_jj[e];
+      eid[threadIdx.x] = e;
+      eself[threadIdx.x] = ix == jx;
+      load_edge_geom(poses, intr, intr_stride, ix, jx, geo[threadIdx.x]);
+    }
+    __syncthreads();
+    if (!in) continue;
+    // targets and weights (tu, tv, wu, wv) of edges q0 .. q0 + 3 of the chunk
+    auto fetch = [&](int q0, float (&o)[4][4]) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (q0 + j < cnt) {
+          const int64_t e = eid[q0 + j];
+          const float* tg = targets + e * 2 * HW;
+          const float* wt = weights + e * 2 * HW;
+          o[j][0] = tg[k];
+          o[j][1] = tg[HW + k];
+          o[j][2] = wt[k];
+          o[j][3] = wt[HW + k];
+        }
+      }
+    };
+    float cur[4][4], nxt[4][4];
+    fetch(0, cur);
+    for (int q0 = 0; q0 < cnt; q0 += 4) {
+      if (q0 + 4 < cnt) fetch(q0 + 4, nxt);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (q0 + j < cnt) {
+          RowJac ru, rv;
+          if (pixel_rows<kSem>(geo[q0 + j], pu, pv, d, cur[j][0], cur[j][1], cur[j][2], cur[j][3], ru, rv)) {
+            float cc, bb, ee[6] = {0, 0, 0, 0, 0, 0};
+            depth_terms(ru, rv, eself[q0 + j] != 0, cc, bb, ee);
+            c += cc;
+            b += bb;
+            if (p >= 0) {
+#pragma unroll
+              for (int n = 0; n < 6; ++n) ei[n] += ee[n];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) cur[j][n] = nxt[j][n];
+    }
+  }
+  if (!in) return;
+  float et = eta[(int64_t)(eta_rows == 1 ? 0 : f) * HW + k];
+  if (disps_sens) {
+    const int64_t px = kx * HW + k;
+    const float ds = disps_sens[px];
+    if (ds > 0.f) {
+      et = alpha;
+      b -= static_cast<Acc>(alpha) * (static_cast<Acc>(disps[px]) - static_cast<Acc>(ds));
+    }
+  }
+  Q[(int64_t)f * HW + k] = static_cast<float>(Acc(1) / (c + et + c_eps));
+  w[(int64_t)f * HW + k] = static_cast<float>(b);
+  if (p >= 0) {
+#pragma unroll
+    for (int n = 0; n < 6; ++n) Eall[((int64_t)p * 6 + n) * HW + k] = static_cast<float>(ei[n]);
+  }
 }
 
 // ---- 3. Ev rows, one wave per slot -------------------------------------------

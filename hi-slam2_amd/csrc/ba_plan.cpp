@@ -298,9 +298,12 @@ extern "C" int hsl2_ba_plan_create_ex(const int64_t* ii, const int64_t* jj, int6
   const size_t real = (p->precision == HSL2_BA_SPLIT) ? sizeof(double) : sizeof(float);
   p->w_blob = take(B.size() * sizeof(int32_t));
   p->w_Eall = take(schur ? (P + E) * 6 * HW * f4 : 0);
-  p->w_Eii = take(schur ? E * 6 * HW * f4 : 0);
-  p->w_Cii = take(schur ? E * HW * f4 : 0);
-  p->w_bz = take(schur ? E * HW * f4 : 0);
+  // the per-edge planes of ba_frame_reduce: the torch-semantics step only (the ba_cuda loop
+  // forms the frame sums from the pixels)
+  const bool planes = schur && semantics == HSL2_SEM_TORCH;
+  p->w_Eii = take(planes ? E * 6 * HW * f4 : 0);
+  p->w_Cii = take(planes ? E * HW * f4 : 0);
+  p->w_bz = take(planes ? E * HW * f4 : 0);
   p->w_Q = take(schur ? F * HW * f4 : 0);
   p->w_w = take(schur ? F * HW * f4 : 0);
   const size_t f8 = sizeof(double);

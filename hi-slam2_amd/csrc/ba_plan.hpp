@@ -52,9 +52,9 @@ struct hsl2_ba_plan {
   // ---- workspace layout (byte offsets) ----
   size_t w_blob = 0;
   size_t w_Eall = 0;     // f32 [P+E][6][HW]
-  size_t w_Eii = 0;      // f32 [E][6][HW]
-  size_t w_Cii = 0;      // f32 [E][HW]
-  size_t w_bz = 0;       // f32 [E][HW]
+  size_t w_Eii = 0;      // f32 [E][6][HW] (torch semantics only)
+  size_t w_Cii = 0;      // f32 [E][HW] (torch semantics only)
+  size_t w_bz = 0;       // f32 [E][HW] (torch semantics only)
   size_t w_Q = 0;        // f32 [F][HW]
   size_t w_w = 0;        // f32 [F][HW]
   size_t w_Hs = 0;       // f64 [4][E][6][6]
