@@ -260,7 +260,7 @@ int hsl2_ba_run_sens(const hsl2_ba_plan* plan, float* poses, float* disps, const
  * the library on each GPU it runs on until the process ends, so the caller passes inputs and
  * outputs only.  Plans needing more workspace (hsl2_ba_plan_workspace_bytes) are rejected with
  * HSL2_ERR_WORKSPACE; frontend windows fit (12 keyframes, 70 edges at 48x64 need
- * 12.7 MiB).  All calls on one GPU share that array and keep the last call's contents: calls
+ * 7.2 MiB).  All calls on one GPU share that array and keep the last call's contents: calls
  * that overlap on one GPU race.  Allocates nothing; otherwise identical to hsl2_ba_run. */
 #define HSL2_BA_STATIC_SCRATCH_BYTES (64u << 20)
 int hsl2_ba_run_static(const hsl2_ba_plan* plan, float* poses, float* disps, const float* intrinsics,
