@@ -316,8 +316,8 @@ def main():
         'oracle_err_lookup_bf16': split['lookup_err'],
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': traffic,
-                     'kernel': 'corr_lookup_vec_kernel<bf16> (4 levels, r=3, 9.216M lookups/launch, '
-                               '912 B/lookup algorithmic)'},
+                     'kernel': 'corr_lookup_vec4_kernel<bf16> (all 4 levels per block, r=3, 9.216M '
+                               'lookups/launch, 912 B/lookup algorithmic)'},
         'cpu_baseline': cpu,
     }
     if plain is not None:
