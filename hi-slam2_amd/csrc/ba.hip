@@ -2,29 +2,32 @@
 // (src/droid_kernels.cu:1820-1937) with every step on the device.
 //
 // Per Gauss-Newton iteration:
-//   1. ba_linearize      one block per edge: relative pose, per-pixel residuals and
-//                        Jacobians, the 12x12 Hessian and gradient of the edge
-//                        reduced through LDS, and the pose-j Schur rows E_ij
-//                        (src/droid_kernels.cu:364-606).
-//   2. ba_frame_sums     per depth frame, from the pixels (each edge's rows recomputed):
-//                        C, w, Q = 1/(C+eta) and the pose-i block Ei of the frame's
-//                        edges (accum_cuda, 1413-1463, 1901-1906).
-//   3. ba_schur_gram_lds per depth frame, 16x16 tiles of the slots' Gram matrix
+//   1. ba_lin_frame      the linearisation and the frame sums in one launch (small graphs;
+//                        ba_linearize then ba_frame_sums for large ones):
+//                        linearize_block, one block per edge (and pixel slice): relative pose,
+//                        per-pixel residuals and Jacobians, the 12x12 Hessian and gradient of
+//                        the edge reduced through LDS, and the pose-j Schur rows E_ij
+//                        (src/droid_kernels.cu:364-606); frame_sums_block, per depth frame
+//                        from the pixels (each edge's rows recomputed): C, w, Q = 1/(C+eta)
+//                        and the pose-i block Ei of the frame's edges (accum_cuda,
+//                        1413-1463, 1901-1906).
+//   2. ba_schur_gram_lds per depth frame, 16x16 tiles of the slots' Gram matrix
 //                        S = sum_px E_a Q E_b^T on the matrix cores, one block per group
 //                        pair of row tiles staged in LDS by 32-pixel chunks (FP32 tile
 //                        sums per 256 pixels, totals in the solve type); the blocks of diagonal
 //                        groups also form Ev = sum_px Q w E (EEt6x6 / Ev6x1, 1466-1560)
-//                        (ba_schur_ev, one wave per slot, when motion-only).
-//   4. ba_assemble       the damped reduced camera system A - S, b - bS in FP64
+//                        (+ gram_reduce for sliced items; a sliced linearisation's
+//                        lin_reduce runs in the same launch).
+//   3. ba_assemble_sys   the damped reduced camera system A - S and its rhs in FP64
 //                        (FP32 in the all-FP32 mode, whose per-edge Hessian blocks, frame
 //                        sums and Gram totals are FP32 as well), written densely in column-major
 //                        order, padded to a multiple of 64 with an identity block
 //                        (SparseBlock, 1584-1686).
-//   5. cholesky_solve    tiled Cholesky factorisation and solve (chol.hip); a non-positive
+//   4. cholesky_solve    tiled Cholesky factorisation and solve (chol.hip); a non-positive
 //                        pivot gives dx = 0 (1669-1677).
-//   6. ba_backsub        dz = Q (w - sum E^T dx) with the reference's t0 skip, and
-//                        the disparity update (EvT6x1, disp_retr, 1562-1582, 1398-1411).
-//   7. ba_pose_retr      T <- Exp(dx) T (1342-1396).
+//   5. ba_update         dz = Q (w - sum E^T dx) with the reference's t0 skip and the
+//                        disparity update (EvT6x1, disp_retr, 1562-1582, 1398-1411), and
+//                        T <- Exp(dx) T (1342-1396), in one launch.
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -37,6 +40,10 @@ namespace {
 
 // Sensor-disparity prior weight of the 15-argument ba form (upstream DROID-SLAM's RGB-D mode).
 constexpr float kSensAlpha = 0.05f;
+// Graphs whose linearisation and frame sums take at most this many blocks together run them in
+// one launch (a frontend window: 70 edges x 4 slices + 12 frames x 12 pixel blocks = 424); larger
+// ones launch them apart, each at its own occupancy.
+constexpr int kFuseBlocks = 1024;
 
 template <typename R>
 int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const float* intr,
@@ -67,56 +74,63 @@ int run_iterations(const hsl2_ba_plan* p, float* poses, float* disps, const floa
 
   hipLaunchKernelGGL(ba_clear_flags, dim3(1), dim3(64), 0, s, info);
   HSL2_LAUNCH_CHECK();
+  const int nx = (HW + 255) / 256;                // pixel blocks per frame
   for (int it = 0; it < iterations; ++it) {
     const bool last = it == iterations - 1;
     if (E > 0) {
-      if (schur)
+      const int nlin = E * zl;
+      if (schur && nlin + nx * F <= kFuseBlocks) {
+        // a small graph: linearisation and frame sums (independent) in one launch
+        hipLaunchKernelGGL((ba_lin_frame<kSemCuda, R>), dim3(static_cast<unsigned>(nlin + nx * F)), dim3(256), 0, s,
+                           poses, disps, intr, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
+                           Hs, vs, Eall + (int64_t)P * 6 * HW, zl, Hpart, nlin, eta, eta_rows,
+                           blob + p->o_frame_eptr, blob + p->o_frame_edges, blob + p->o_frame_pose,
+                           blob + p->o_frame_order, blob + p->o_frame_kx, Q, wv, Eall, disps_sens, kSensAlpha, nx);
+        HSL2_LAUNCH_CHECK();
+      } else if (schur) {
         hipLaunchKernelGGL((ba_linearize<true, kSemCuda, R>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
                            Hs, vs, Eall + (int64_t)P * 6 * HW, nullptr, nullptr, nullptr, zl, Hpart);
-      else
+        HSL2_LAUNCH_CHECK();
+        hipLaunchKernelGGL((ba_frame_sums<kSemCuda, R>), dim3(nx, F), dim3(256), 0, s, poses, disps, intr, targets,
+                           weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, eta, eta_rows,
+                           blob + p->o_frame_eptr, blob + p->o_frame_edges, blob + p->o_frame_pose,
+                           blob + p->o_frame_order, blob + p->o_frame_kx, Q, wv, Eall, disps_sens, kSensAlpha);
+        HSL2_LAUNCH_CHECK();
+      } else {
         hipLaunchKernelGGL((ba_linearize<false, kSemCuda, R>), dim3(E, zl), dim3(kLinThreads), 0, s, poses, disps,
                            intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, HW, wd, E,
                            Hs, vs, nullptr, nullptr, nullptr, nullptr, zl, Hpart);
-      HSL2_LAUNCH_CHECK();
-      if (zl > 1) {
-        hipLaunchKernelGGL(lin_reduce<R>, dim3(static_cast<unsigned>((E * 156 + 255) / 256)), dim3(256), 0, s, Hpart,
-                           zl, E, Hs, vs);
         HSL2_LAUNCH_CHECK();
+        if (zl > 1) {
+          hipLaunchKernelGGL(lin_reduce<R>, dim3(static_cast<unsigned>((E * 156 + 255) / 256)), dim3(256), 0, s,
+                             Hpart, zl, E, Hs, vs);
+          HSL2_LAUNCH_CHECK();
+        }
       }
     }
     if (schur) {
-      hipLaunchKernelGGL((ba_frame_sums<kSemCuda, R>), dim3((HW + 255) / 256, F), dim3(256), 0, s, poses, disps,
-                         intr, 0, targets, weights, blob + p->o_edge_ii, blob + p->o_edge_jj, eta, eta_rows,
-                         blob + p->o_frame_eptr, blob + p->o_frame_edges, blob + p->o_frame_pose,
-                         blob + p->o_frame_order, blob + p->o_frame_kx, HW, wd, 0.f, Q, wv, Eall, disps_sens,
-                         kSensAlpha);
-      HSL2_LAUNCH_CHECK();
-      launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, Spart, s);
+      // the Gram (with the slices' lin_reduce alongside when the linearisation was sliced)
+      launch_schur_gram(p, blob, Eall, Q, wv, S, Ev, Spart, s, E > 0 && zl > 1 ? Hpart : nullptr, Hs, vs);
       HSL2_LAUNCH_CHECK();
     }
-    const int64_t nn = n_pad * n_pad;
-    hipLaunchKernelGGL(ba_assemble<R>, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, s,
-                       Hs, S, blob + p->o_blk_ptr, blob + p->o_blk_src, E, P, n_pad, lm, ep, 0,
-                       A, info);
-    HSL2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ba_assemble_rhs<R>, dim3(static_cast<unsigned>((n_pad / 6 + 4) / 4)), dim3(256), 0, s,
-                       vs, Ev, blob + p->o_rhs_ptr, blob + p->o_rhs_src, E, P,
-                       static_cast<int>(n_pad), bvec);
+    // the reduced system and its rhs
+    const int nA = static_cast<int>((n_pad * n_pad + 255) / 256);
+    const int nR = static_cast<int>((n_pad / 6 + 4) / 4);
+    hipLaunchKernelGGL(ba_assemble_sys<R>, dim3(static_cast<unsigned>(nA + nR)), dim3(256), 0, s, Hs, S,
+                       blob + p->o_blk_ptr, blob + p->o_blk_src, vs, Ev, blob + p->o_rhs_ptr, blob + p->o_rhs_src, E,
+                       P, n_pad, lm, ep, A, bvec, info, nA);
     HSL2_LAUNCH_CHECK();
     {
       const int st = cholesky_solve<R>(A, n_pad, Ld, bvec, xvec, info, s);
       if (st) return st;
     }
-    if (schur) {
-      hipLaunchKernelGGL(ba_backsub<R>, dim3((HW + 255) / 256, F), dim3(256), 0, s, Eall, Q, wv,
-                         xvec, info, blob + p->o_slot_ptr, blob + p->o_slot_row,
-                         blob + p->o_slot_pose, blob + p->o_frame_kx, HW, 1, disps,
-                         last ? dz : nullptr);
-      HSL2_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(ba_pose_retr<R>, dim3((P + 63) / 64), dim3(64), 0, s, poses, xvec, info,
-                       p->t0, P, 0, last ? dx : nullptr);
+    // depth back-substitution and pose retraction in one launch
+    const int nbs = schur ? nx * F : 0;
+    hipLaunchKernelGGL(ba_update<R>, dim3(static_cast<unsigned>(nbs + (P + 255) / 256)), dim3(256), 0, s, Eall, Q,
+                       wv, xvec, info, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_slot_pose,
+                       blob + p->o_frame_kx, HW, disps, last ? dz : nullptr, nx, nbs, poses, p->t0, P,
+                       last ? dx : nullptr);
     HSL2_LAUNCH_CHECK();
   }
   return HSL2_OK;

@@ -133,7 +133,7 @@ __device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
 // A valid pixel's terms of one edge in its depth frame's sums (FP32): the depth diagonal
 // c = w Jz^2, the depth gradient b = w r Jz and the pose-i coupling ei = w Jz Ji (zero for an
 // edge from a frame to itself).  The one definition serves ba_linearize's per-edge planes and
-// ba_frame_sums, so both form the same values.
+// frame_sums_block, so both form the same values.
 __device__ __forceinline__ void depth_terms(const RowJac& ru, const RowJac& rv, bool self, float& c, float& b,
                                             float (&ei)[6]) {
 #pragma clang fp contract(off)
@@ -156,24 +156,22 @@ __device__ __forceinline__ void pose_j_terms(const RowJac& ru, const RowJac& rv,
 }
 
 // ---- 1. per-edge linearization ----------------------------------------------
-// With Cii == nullptr the per-edge planes Cii, bz and Eii are not written (ba_frame_sums forms
-// the frame sums from the pixels instead); E_ij goes to Eij either way.
-template <bool kSchur, int kSem, typename Acc = double>
-__global__ void __launch_bounds__(kLinThreads)
-ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
-             const float* __restrict__ intr, int intr_stride, const float* __restrict__ targets,
-             const float* __restrict__ weights, const int* __restrict__ edge_ii,
-             const int* __restrict__ edge_jj, int HW, int wd, int E,
-             Acc* __restrict__ Hs, Acc* __restrict__ vs, float* __restrict__ Eij,
-             float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz,
-             int slices = 1, Acc* __restrict__ Hpart = nullptr) {
+// With Cii == nullptr the per-edge planes Cii, bz and Eii are not written (frame_sums_block
+// forms the frame sums from the pixels instead); E_ij goes to Eij either way.
+// Block of edge e and pixel slice z (of `slices`): with several slices per edge the block
+// writes its 90 partial sums to Hpart [slices][E][90], which lin_reduce (or the assembly)
+// adds up in slice order.
+template <bool kSchur, int kSem, typename Acc>
+__device__ __forceinline__ void linearize_block(
+    const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
+    int intr_stride, const float* __restrict__ targets, const float* __restrict__ weights,
+    const int* __restrict__ edge_ii, const int* __restrict__ edge_jj, int HW, int wd, int E, Acc* __restrict__ Hs,
+    Acc* __restrict__ vs, float* __restrict__ Eij, float* __restrict__ Eii, float* __restrict__ Cii,
+    float* __restrict__ bz, int slices, Acc* __restrict__ Hpart, int e, int z) {
   __shared__ Acc lds[16 * kLinThreads];
   __shared__ Acc red[90];
-  const int e = blockIdx.x;
-  // pixel slice of this block (blockIdx.y of `slices`): with several slices per edge the block
-  // writes its 90 partial sums to Hpart [slices][E][90] and lin_reduce forms Hs and vs
   const int span = (HW + slices - 1) / slices;
-  const int k_begin = blockIdx.y * span, k_end = min(HW, k_begin + span);
+  const int k_begin = z * span, k_end = min(HW, k_begin + span);
   const int ix = edge_ii[e], jx = edge_jj[e];
   const bool self = ix == jx;
   EdgeGeom g;
@@ -244,7 +242,7 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   __syncthreads();
   const int t = threadIdx.x;
   if (slices > 1) {
-    if (t < 90) Hpart[((int64_t)blockIdx.y * E + e) * 90 + t] = red[t];
+    if (t < 90) Hpart[((int64_t)z * E + e) * 90 + t] = red[t];
     return;
   }
   if (t < 144) {
@@ -261,12 +259,25 @@ ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
   }
 }
 
+template <bool kSchur, int kSem, typename Acc = double>
+__global__ void __launch_bounds__(kLinThreads)
+ba_linearize(const float* __restrict__ poses, const float* __restrict__ disps,
+             const float* __restrict__ intr, int intr_stride, const float* __restrict__ targets,
+             const float* __restrict__ weights, const int* __restrict__ edge_ii,
+             const int* __restrict__ edge_jj, int HW, int wd, int E,
+             Acc* __restrict__ Hs, Acc* __restrict__ vs, float* __restrict__ Eij,
+             float* __restrict__ Eii, float* __restrict__ Cii, float* __restrict__ bz,
+             int slices = 1, Acc* __restrict__ Hpart = nullptr) {
+  linearize_block<kSchur, kSem, Acc>(poses, disps, intr, intr_stride, targets, weights, edge_ii, edge_jj, HW, wd, E,
+                                     Hs, vs, Eij, Eii, Cii, bz, slices, Hpart, static_cast<int>(blockIdx.x),
+                                     static_cast<int>(blockIdx.y));
+}
+
 // Hs [4, E, 6, 6] and vs [2, E, 6] from the slices' partial sums Hpart [slices][E][90] (the
 // packed lower triangle of the edge's 12 x 12 block, then its 12-vector), summed in slice order.
 template <typename Acc>
-__global__ void __launch_bounds__(256)
-lin_reduce(const Acc* __restrict__ Hpart, int slices, int E, Acc* __restrict__ Hs, Acc* __restrict__ vs) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void lin_reduce_elem(const Acc* __restrict__ Hpart, int slices, int E, Acc* __restrict__ Hs,
+                                                Acc* __restrict__ vs, int64_t idx) {
   if (idx >= (int64_t)E * 156) return;
   const int e = static_cast<int>(idx / 156), t = static_cast<int>(idx % 156);
   int l;
@@ -288,6 +299,12 @@ lin_reduce(const Acc* __restrict__ Hpart, int slices, int E, Acc* __restrict__ H
     const int k = t - 144;
     vs[((int64_t)(k / 6) * E + e) * 6 + (k % 6)] = v;
   }
+}
+
+template <typename Acc>
+__global__ void __launch_bounds__(256)
+lin_reduce(const Acc* __restrict__ Hpart, int slices, int E, Acc* __restrict__ Hs, Acc* __restrict__ vs) {
+  lin_reduce_elem(Hpart, slices, E, Hs, vs, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- 2. per-frame sums: C, w, Q and the pose-i Schur block Ei ----------------
@@ -391,20 +408,21 @@ inline void launch_frame_reduce(const float* Cii, const float* bz, const float* 
 // are summed.  The sensor-disparity prior is applied as in ba_frame_reduce.
 constexpr int kFsEdges = 64;
 
+// Block (fb, bx): pixels 256 bx .. of the fb-th frame of frame_order.
 template <int kSem, typename Acc>
-__global__ void __launch_bounds__(256)
-ba_frame_sums(const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
-              int intr_stride, const float* __restrict__ targets, const float* __restrict__ weights,
-              const int* __restrict__ edge_ii, const int* __restrict__ edge_jj, const float* __restrict__ eta,
-              int eta_rows, const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
-              const int* __restrict__ frame_pose, const int* __restrict__ frame_order,
-              const int* __restrict__ frame_kx, int HW, int wd, float c_eps, float* __restrict__ Q,
-              float* __restrict__ w, float* __restrict__ Eall, const float* __restrict__ disps_sens, float alpha) {
+__device__ __forceinline__ void frame_sums_block(
+    const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
+    int intr_stride, const float* __restrict__ targets, const float* __restrict__ weights,
+    const int* __restrict__ edge_ii, const int* __restrict__ edge_jj, const float* __restrict__ eta, int eta_rows,
+    const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges, const int* __restrict__ frame_pose,
+    const int* __restrict__ frame_order, const int* __restrict__ frame_kx, int HW, int wd, float c_eps,
+    float* __restrict__ Q, float* __restrict__ w, float* __restrict__ Eall, const float* __restrict__ disps_sens,
+    float alpha, int fb, int bx) {
   __shared__ EdgeGeom geo[kFsEdges];
   __shared__ int eid[kFsEdges];
   __shared__ int eself[kFsEdges];
-  const int f = frame_order[blockIdx.y];          // the frames with most edges first
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = frame_order[fb];                  // the frames with most edges first
+  const int k = bx * blockDim.x + threadIdx.x;
   const bool in = k < HW;
   const int e0 = frame_eptr[f], e1 = frame_eptr[f + 1];
   const int p = frame_pose[f];
@@ -485,6 +503,53 @@ ba_frame_sums(const float* __restrict__ poses, const float* __restrict__ disps, 
   }
 }
 
+// The linearisation (blocks [0, nlin): edge b % E, pixel slice b / E) and the frame sums (the
+// blocks after them: frame fb = (b - nlin) / nx, pixel block (b - nlin) % nx) in one launch.  The
+// frame sums recompute their rows from the pixels, so they need nothing from the linearisation;
+// the two write disjoint outputs (E_ij rows, Hessian blocks / the frames' Q, w and E_i rows), and
+// the frame sums use the memory bandwidth that the linearisation's arithmetic leaves free.
+template <int kSem, typename Acc>
+__global__ void __launch_bounds__(256)
+ba_lin_frame(const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
+             const float* __restrict__ targets, const float* __restrict__ weights, const int* __restrict__ edge_ii,
+             const int* __restrict__ edge_jj, int HW, int wd, int E, Acc* __restrict__ Hs, Acc* __restrict__ vs,
+             float* __restrict__ Eij, int slices, Acc* __restrict__ Hpart, int nlin, const float* __restrict__ eta,
+             int eta_rows, const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
+             const int* __restrict__ frame_pose, const int* __restrict__ frame_order,
+             const int* __restrict__ frame_kx, float* __restrict__ Q, float* __restrict__ w, float* __restrict__ Eall,
+             const float* __restrict__ disps_sens, float alpha, int nx) {
+  static_assert(kLinThreads == 256, "one block size for both parts");
+  static_assert(kSem == kSemCuda, "the ba_cuda loop: one intrinsics vector, Q = 1 / (C + eta)");
+  const int b = blockIdx.x;
+  if (b < nlin) {
+    linearize_block<true, kSem, Acc>(poses, disps, intr, 0, targets, weights, edge_ii, edge_jj, HW, wd, E, Hs, vs,
+                                     Eij, nullptr, nullptr, nullptr, slices, Hpart, b % E, b / E);
+    return;
+  }
+  const int fb = b - nlin;
+  frame_sums_block<kSem, Acc>(poses, disps, intr, 0, targets, weights, edge_ii, edge_jj, eta, eta_rows, frame_eptr,
+                              frame_edges, frame_pose, frame_order, frame_kx, HW, wd, 0.f, Q, w, Eall, disps_sens,
+                              alpha, fb / nx, fb % nx);
+}
+
+// The frame sums alone (grid: pixel blocks x frames), for graphs large enough to fill the GPU
+// with the linearisation by itself: a launch of their own keeps their occupancy (108 VGPRs)
+// instead of the linearisation's (212).
+template <int kSem, typename Acc>
+__global__ void __launch_bounds__(256)
+ba_frame_sums(const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
+              const float* __restrict__ targets, const float* __restrict__ weights, const int* __restrict__ edge_ii,
+              const int* __restrict__ edge_jj, int HW, int wd, const float* __restrict__ eta, int eta_rows,
+              const int* __restrict__ frame_eptr, const int* __restrict__ frame_edges,
+              const int* __restrict__ frame_pose, const int* __restrict__ frame_order,
+              const int* __restrict__ frame_kx, float* __restrict__ Q, float* __restrict__ w, float* __restrict__ Eall,
+              const float* __restrict__ disps_sens, float alpha) {
+  static_assert(kSem == kSemCuda, "the ba_cuda loop: one intrinsics vector, Q = 1 / (C + eta)");
+  frame_sums_block<kSem, Acc>(poses, disps, intr, 0, targets, weights, edge_ii, edge_jj, eta, eta_rows, frame_eptr,
+                              frame_edges, frame_pose, frame_order, frame_kx, HW, wd, 0.f, Q, w, Eall, disps_sens,
+                              alpha, static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.x));
+}
+
 // ---- 3. Ev rows, one wave per slot -------------------------------------------
 // Ev[s] = sum_px Q w E_s for the Ev tasks [task0, NT) of the plan's task list (task0 = NPAIR:
 // the pair blocks come from ba_schur_gram_lds).
@@ -562,11 +627,16 @@ ba_schur_gram_lds(const float* __restrict__ Eall, const float* __restrict__ Q,
                   const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
                   const int* __restrict__ frame_pair0, int HW, const float* __restrict__ wvec,
                   Acc* __restrict__ S, Acc* __restrict__ Ev, int slices, int64_t npair, int64_t part_stride,
-                  Acc* __restrict__ Spart) {
+                  Acc* __restrict__ Spart, const Acc* __restrict__ Hpart, int lin_slices, int E,
+                  Acc* __restrict__ Hs, Acc* __restrict__ vs) {
   __shared__ float Ls[2][kGramKC][kGramRS];
   __shared__ float qs[2][kGramKC], qws[2][kGramKC];
   __shared__ Acc evs[4][16 * kGramGroup];
   const int blk = blockIdx.x;
+  if (blk >= 8 * item_blocks8) {                  // lin_reduce of a sliced linearisation, alongside
+    lin_reduce_elem(Hpart, lin_slices, E, Hs, vs, (int64_t)(blk - 8 * item_blocks8) * blockDim.x + threadIdx.x);
+    return;
+  }
   const int idx = (blk % 8) * item_blocks8 + blk / 8;   // XCD-aware: runs of consecutive (item, slice)
   if (idx >= n_items * slices) return;
   const int item = idx / slices, slice = idx - item * slices;
@@ -793,35 +863,43 @@ inline void launch_schur_ev(const hsl2_ba_plan* p, const int* blob, const float*
 
 // Schur pair blocks and Ev rows (both from the Gram kernel; Ev alone when there are no
 // Gram items, i.e. motion-only).
+// With Hpart != nullptr (a linearisation cut into lin_slices > 1 pixel slices) the Gram launch
+// also carries lin_reduce (Hpart -> Hs, vs) in blocks after its own; without Gram items it runs
+// on its own.
 template <typename Acc>
 inline void launch_schur_gram(const hsl2_ba_plan* p, const int* blob, const float* Eall, const float* Q,
-                              const float* wvec, Acc* S, Acc* Ev, Acc* Spart, hipStream_t s) {
+                              const float* wvec, Acc* S, Acc* Ev, Acc* Spart, hipStream_t s,
+                              const Acc* Hpart = nullptr, Acc* Hs = nullptr, Acc* vs = nullptr) {
   const int HW = static_cast<int>(p->HW);
+  const int E = static_cast<int>(p->E);
+  const int nlr = Hpart ? static_cast<int>((E * 156 + 255) / 256) : 0;
   if (p->NGGI > 0) {
     const int ni = static_cast<int>(p->NGGI);
     const int z = p->gram_slices;
     const int ib8 = (ni * z + 7) / 8;
     const int64_t stride = p->NPAIR * 36 + p->NS * 6;
-    hipLaunchKernelGGL(ba_schur_gram_lds<Acc>, dim3(ib8 * 8), dim3(256), 0, s, Eall, Q, blob + p->o_gram_group_item,
-                       ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row, blob + p->o_frame_pair0, HW, wvec,
-                       S, Ev, z, p->NPAIR, stride, Spart);
+    hipLaunchKernelGGL(ba_schur_gram_lds<Acc>, dim3(ib8 * 8 + nlr), dim3(256), 0, s, Eall, Q,
+                       blob + p->o_gram_group_item, ni, ib8, blob + p->o_slot_ptr, blob + p->o_slot_row,
+                       blob + p->o_frame_pair0, HW, wvec, S, Ev, z, p->NPAIR, stride, Spart, Hpart, p->lin_slices, E,
+                       Hs, vs);
     if (z > 1)
       hipLaunchKernelGGL(gram_reduce<Acc>, dim3(static_cast<unsigned>((stride + 255) / 256)), dim3(256), 0, s, Spart,
                          z, p->NPAIR, stride, S, Ev);
   } else {
     launch_schur_ev(p, blob, Eall, Q, wvec, Ev, s);
+    if (nlr > 0)
+      hipLaunchKernelGGL(lin_reduce<Acc>, dim3(static_cast<unsigned>(nlr)), dim3(256), 0, s, Hpart, p->lin_slices, E,
+                         Hs, vs);
   }
 }
 
 // ---- 4. reduced camera system, dense column-major ----------------------------
 template <typename R>
-__global__ void __launch_bounds__(256)
-ba_assemble(const R* __restrict__ Hs, const R* __restrict__ S,
-            const int* __restrict__ blk_ptr, const int* __restrict__ blk_src, int E, int P,
-            int64_t n_pad, float lm, float ep, int damp_on_H, R* __restrict__ A,
-            int* __restrict__ info) {
+__device__ __forceinline__ void assemble_elem(const R* __restrict__ Hs, const R* __restrict__ S,
+                                              const int* __restrict__ blk_ptr, const int* __restrict__ blk_src,
+                                              int E, int P, int64_t n_pad, float lm, float ep, int damp_on_H,
+                                              R* __restrict__ A, int* __restrict__ info, int64_t idx) {
   const int64_t n = 6 * (int64_t)P;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx == 0) info[0] = 0;                     // this iteration's factorisation status
   if (idx >= n_pad * n_pad) return;
   const int64_t col = idx / n_pad;
@@ -875,12 +953,20 @@ ba_assemble(const R* __restrict__ Hs, const R* __restrict__ S,
 
 template <typename R>
 __global__ void __launch_bounds__(256)
-ba_assemble_rhs(const R* __restrict__ vs, const R* __restrict__ Ev,
-                const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
-                int n_pad, R* __restrict__ bvec) {
-  // one wave per pose (six rows): lanes take the pose's contributions in turn, each lane
-  // sums its share of the six components, then the wave reduces them (fixed order)
-  const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+ba_assemble(const R* __restrict__ Hs, const R* __restrict__ S,
+            const int* __restrict__ blk_ptr, const int* __restrict__ blk_src, int E, int P,
+            int64_t n_pad, float lm, float ep, int damp_on_H, R* __restrict__ A,
+            int* __restrict__ info) {
+  assemble_elem(Hs, S, blk_ptr, blk_src, E, P, n_pad, lm, ep, damp_on_H, A, info,
+                (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// one wave per pose (six rows): lanes take the pose's contributions in turn, each lane sums its
+// share of the six components, then the wave reduces them (fixed order).  Wave-uniform `a`.
+template <typename R>
+__device__ __forceinline__ void assemble_rhs_wave(const R* __restrict__ vs, const R* __restrict__ Ev,
+                                                  const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src,
+                                                  int E, int P, int n_pad, R* __restrict__ bvec, int a) {
   const int lane = threadIdx.x % kWave;
   if (6 * a >= n_pad) return;
   R v[6] = {0, 0, 0, 0, 0, 0};
@@ -903,16 +989,40 @@ ba_assemble_rhs(const R* __restrict__ vs, const R* __restrict__ Ev,
   }
 }
 
-// ---- 6. back-substitution and disparity update -------------------------------
 template <typename R>
 __global__ void __launch_bounds__(256)
-ba_backsub(const float* __restrict__ Eall, const float* __restrict__ Q,
-           const float* __restrict__ w, const R* __restrict__ x, const int* __restrict__ info,
-           const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
-           const int* __restrict__ slot_pose, const int* __restrict__ frame_kx, int HW,
-           int t0_skip, float* __restrict__ disps, float* __restrict__ dz) {
-  const int f = blockIdx.y;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+ba_assemble_rhs(const R* __restrict__ vs, const R* __restrict__ Ev,
+                const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P,
+                int n_pad, R* __restrict__ bvec) {
+  assemble_rhs_wave(vs, Ev, rhs_ptr, rhs_src, E, P, n_pad, bvec,
+                    static_cast<int>(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
+}
+
+// The reduced system and its right-hand side in one launch: blocks [0, nA) assemble A (one
+// element per thread), the blocks after them the rhs (one pose per wave).
+template <typename R>
+__global__ void __launch_bounds__(256)
+ba_assemble_sys(const R* __restrict__ Hs, const R* __restrict__ S, const int* __restrict__ blk_ptr,
+                const int* __restrict__ blk_src, const R* __restrict__ vs, const R* __restrict__ Ev,
+                const int* __restrict__ rhs_ptr, const int* __restrict__ rhs_src, int E, int P, int64_t n_pad,
+                float lm, float ep, R* __restrict__ A, R* __restrict__ bvec, int* __restrict__ info, int nA) {
+  const int b = blockIdx.x;
+  if (b < nA) {
+    assemble_elem(Hs, S, blk_ptr, blk_src, E, P, n_pad, lm, ep, 0, A, info, (int64_t)b * blockDim.x + threadIdx.x);
+    return;
+  }
+  assemble_rhs_wave(vs, Ev, rhs_ptr, rhs_src, E, P, static_cast<int>(n_pad), bvec,
+                    (b - nA) * static_cast<int>(blockDim.x / kWave) + static_cast<int>(threadIdx.x / kWave));
+}
+
+// ---- 6. back-substitution and disparity update -------------------------------
+template <typename R>
+__device__ __forceinline__ void backsub_px(const float* __restrict__ Eall, const float* __restrict__ Q,
+                                           const float* __restrict__ w, const R* __restrict__ x,
+                                           const int* __restrict__ info, const int* __restrict__ slot_ptr,
+                                           const int* __restrict__ slot_row, const int* __restrict__ slot_pose,
+                                           const int* __restrict__ frame_kx, int HW, int t0_skip,
+                                           float* __restrict__ disps, float* __restrict__ dz, int f, int k) {
   if (k >= HW) return;
   const bool solved = info[0] == 0;
   R acc = 0;
@@ -950,6 +1060,17 @@ ba_backsub(const float* __restrict__ Eall, const float* __restrict__ Q,
   const float d = static_cast<float>(static_cast<R>(qv) * (static_cast<R>(w[(int64_t)f * HW + k]) - acc));
   if (dz) dz[(int64_t)f * HW + k] = d;
   if (disps) disps[(int64_t)frame_kx[f] * HW + k] += d;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+ba_backsub(const float* __restrict__ Eall, const float* __restrict__ Q,
+           const float* __restrict__ w, const R* __restrict__ x, const int* __restrict__ info,
+           const int* __restrict__ slot_ptr, const int* __restrict__ slot_row,
+           const int* __restrict__ slot_pose, const int* __restrict__ frame_kx, int HW,
+           int t0_skip, float* __restrict__ disps, float* __restrict__ dz) {
+  backsub_px(Eall, Q, w, x, info, slot_ptr, slot_row, slot_pose, frame_kx, HW, t0_skip, disps, dz,
+             static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // ---- 7. pose retraction -------------------------------------------------------
@@ -1010,10 +1131,9 @@ __device__ void retr_se3(const float xi[6], float* pose, int full_v) {
 }
 
 template <typename R>
-__global__ void ba_pose_retr(float* __restrict__ poses, const R* __restrict__ x,
-                             int* __restrict__ info, int t0, int P, int full_v,
-                             float* __restrict__ dx) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pose_retr_one(float* __restrict__ poses, const R* __restrict__ x,
+                                              int* __restrict__ info, int t0, int P, int full_v,
+                                              float* __restrict__ dx, int a) {
   const bool solved = info[0] == 0;
   if (a == 0 && !solved) info[1] = 1;            // remember a failed solve
   if (a >= P) return;
@@ -1030,6 +1150,32 @@ __global__ void ba_pose_retr(float* __restrict__ poses, const R* __restrict__ x,
   retr_se3(xi, pose, full_v);
 #pragma unroll
   for (int n = 0; n < 7; ++n) poses[(int64_t)(t0 + a) * 7 + n] = pose[n];
+}
+
+template <typename R>
+__global__ void ba_pose_retr(float* __restrict__ poses, const R* __restrict__ x,
+                             int* __restrict__ info, int t0, int P, int full_v,
+                             float* __restrict__ dx) {
+  pose_retr_one(poses, x, info, t0, P, full_v, dx, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// The depth back-substitution with the disparity update (blocks [0, nbs): frame b / nx, pixel
+// block b % nx) and the pose retraction (the blocks after them, one pose per thread) in one
+// launch: both only read the solution and write disjoint outputs (disparities, poses).
+template <typename R>
+__global__ void __launch_bounds__(256)
+ba_update(const float* __restrict__ Eall, const float* __restrict__ Q, const float* __restrict__ w,
+          const R* __restrict__ x, int* __restrict__ info, const int* __restrict__ slot_ptr,
+          const int* __restrict__ slot_row, const int* __restrict__ slot_pose, const int* __restrict__ frame_kx,
+          int HW, float* __restrict__ disps, float* __restrict__ dz, int nx, int nbs, float* __restrict__ poses,
+          int t0, int P, float* __restrict__ dx) {
+  const int b = blockIdx.x;
+  if (b < nbs) {
+    backsub_px(Eall, Q, w, x, info, slot_ptr, slot_row, slot_pose, frame_kx, HW, 1, disps, dz, b / nx,
+               (b % nx) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x));
+    return;
+  }
+  pose_retr_one(poses, x, info, t0, P, 0, dx, (b - nbs) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x));
 }
 
 __global__ void ba_clear_flags(int* info) {

@@ -18,7 +18,8 @@
 //                  plus one block per tile row i > k: b_i -= L_ik y_k
 //   back-substitution, one chol_back_all launch: one block per tile row j applies
 //                  y_j -= L_kj^T x_k as each x_k (k > j) is published through an agent-scope
-//                  counter, then writes x_j = (L_jj^-1)^T y_j and publishes it
+//                  counter, then writes x_j = (L_jj^-1)^T y_j and publishes it (a single-tile
+//                  system: block 0's consumer wave forms x = (L_00^-1)^T y itself, one launch)
 //
 // A pivot that is not positive (or NaN) sets info = 1, as Eigen's LLT reports a
 // NumericalIssue; the caller then treats the solution as zero.  Templated on the element
@@ -431,7 +432,8 @@ __device__ bool invert_columns(PanelShared<R>& sh, R (*Xs)[kLd]) {
 // block barrier is the one after the counter is cleared.
 template <typename R>
 __device__ void panel_block(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k,
-                            int blk, int* __restrict__ info, PanelShared<R>& sh, R (*Xs)[kLd]) {
+                            int blk, int* __restrict__ info, PanelShared<R>& sh, R (*Xs)[kLd],
+                            R* __restrict__ xsol) {
   const int t = threadIdx.x;
   const int i = k + blk;
   const int64_t bk = (int64_t)k * kTile, bi = (int64_t)i * kTile;
@@ -460,6 +462,18 @@ __device__ void panel_block(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, 
 #pragma unroll 8
   for (int c = 0; c < kTile; ++c) acc += Xs[u][c] * sh.bk[c];
   b[bk + u] = acc;
+  if (xsol) {
+    // a single-tile system: the back-substitution x = L_kk^-T y here, in this wave (lane u:
+    // x_u = sum_c (L_kk^-1)[c][u] y_c), instead of a chol_back_all launch.  y replaces b_k in
+    // LDS: every read of b_k above has returned (its value was used), and the fence orders the
+    // stores before the loads of the other lanes
+    sh.bk[u] = acc;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    R xv = 0;
+#pragma unroll 8
+    for (int c = 0; c < kTile; ++c) xv += Xs[c][u] * sh.bk[c];
+    xsol[u] = xv;
+  }
 }
 
 // Trailing-update work of step k (256 threads): tile (k+1+ii, k+1+jj) -= L_{k+1+ii,k} L_{k+1+jj,k}^T.
@@ -541,7 +555,7 @@ template <typename R> union StepShared {
 template <typename R>
 __global__ void __launch_bounds__(kStepThreads)
 chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b, int k, int T,
-          int* __restrict__ info) {
+          int* __restrict__ info, R* __restrict__ xsol) {
   __shared__ StepShared<R> smem;
   if (k == 0 && blockIdx.x == 0)                 // clear the back-substitution's tile counters
     for (int i = threadIdx.x; i < T; i += blockDim.x) info[kBackFlags + i] = 0;
@@ -549,7 +563,7 @@ chol_step(R* __restrict__ A, int64_t lda, R* __restrict__ Ld, R* __restrict__ b,
   const int blk = blockIdx.x;
   if (blk < npanel) {
     if (threadIdx.x >= (kFactorWaves + 1) * kWave) return;
-    panel_block<R>(A, lda, Ld, b, k, blk, info, smem.panel.sh, smem.panel.Xs);
+    panel_block<R>(A, lda, Ld, b, k, blk, info, smem.panel.sh, smem.panel.Xs, xsol);
     return;
   }
   const int kp = k - 1, m1 = T - k - 1;           // sub-triangle size of the remaining tiles
@@ -700,11 +714,18 @@ template <typename R>
 int cholesky_solve(R* A, int64_t n_pad, R* Ld, R* b, R* x, int* info, hipStream_t s) {
   HSL2_REQUIRE(n_pad % kTile == 0, HSL2_ERR_SHAPE, "cholesky: size must be a multiple of 64");
   const int T = static_cast<int>(n_pad / kTile);
+  if (T == 1) {
+    // one tile: the panel block also back-substitutes (no chol_back_all launch)
+    hipLaunchKernelGGL(chol_step<R>, dim3(1), dim3(kStepThreads), 0, s, A, n_pad, Ld, b, 0, T, info, x);
+    HSL2_LAUNCH_CHECK();
+    return HSL2_OK;
+  }
   for (int k = 0; k < T; ++k) {
     // panel k, with the rest of step k - 1's trailing update alongside
     const int m1 = k > 0 ? T - k - 1 : 0;
     const int rest = k > 0 ? m1 * (m1 + 1) / 2 + m1 : 0;
-    hipLaunchKernelGGL(chol_step<R>, dim3(T - k + rest), dim3(kStepThreads), 0, s, A, n_pad, Ld, b, k, T, info);
+    hipLaunchKernelGGL(chol_step<R>, dim3(T - k + rest), dim3(kStepThreads), 0, s, A, n_pad, Ld, b, k, T, info,
+                       static_cast<R*>(nullptr));
     HSL2_LAUNCH_CHECK();
     const int m = T - k - 1;
     if (m > 0) {
