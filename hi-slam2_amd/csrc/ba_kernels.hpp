@@ -159,8 +159,8 @@ __device__ __forceinline__ void pose_j_terms(const RowJac& ru, const RowJac& rv,
 // With Cii == nullptr the per-edge planes Cii, bz and Eii are not written (frame_sums_block
 // forms the frame sums from the pixels instead); E_ij goes to Eij either way.
 // Block of edge e and pixel slice z (of `slices`): with several slices per edge the block
-// writes its 90 partial sums to Hpart [slices][E][90], which lin_reduce (or the assembly)
-// adds up in slice order.
+// writes its 90 partial sums to Hpart [slices][E][90], which lin_reduce adds up in slice
+// order.
 template <bool kSchur, int kSem, typename Acc>
 __device__ __forceinline__ void linearize_block(
     const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
