@@ -226,15 +226,29 @@ __device__ __forceinline__ void linearize_block(
         Jv[n] = rv.Ji[n];
         Jv[6 + n] = rv.Jj[n];
       }
+      // Ju[7] (the u row's Jj[1]) and Jv[6] (the v row's Jj[0]) are zero by construction
+      // (pixel_rows): the products with them only add zeros, so they are left out at compile
+      // time (26 of the 180 multiply-adds; every sum is unchanged for finite values)
+      constexpr int kZu = 7, kZv = 6;
 #pragma unroll
       for (int n = 0; n < 12; ++n) {
         const float wun = ru.w * Ju[n], wvn = rv.w * Jv[n];
 #pragma unroll
-        for (int m = 0; m <= n; ++m) acc[tri(n, m)] = fmaf(wvn, Jv[m], fmaf(wun, Ju[m], acc[tri(n, m)]));
+        for (int m = 0; m <= n; ++m) {
+          float v = acc[tri(n, m)];
+          if (n != kZu && m != kZu) v = fmaf(wun, Ju[m], v);
+          if (n != kZv && m != kZv) v = fmaf(wvn, Jv[m], v);
+          acc[tri(n, m)] = v;
+        }
       }
       const float wru = ru.w * ru.r, wrv = rv.w * rv.r;
 #pragma unroll
-      for (int n = 0; n < 12; ++n) acc[78 + n] = fmaf(wrv, Jv[n], fmaf(wru, Ju[n], acc[78 + n]));
+      for (int n = 0; n < 12; ++n) {
+        float v = acc[78 + n];
+        if (n != kZu) v = fmaf(wru, Ju[n], v);
+        if (n != kZv) v = fmaf(wrv, Jv[n], v);
+        acc[78 + n] = v;
+      }
     }
   }
 
