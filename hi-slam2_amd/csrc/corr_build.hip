@@ -102,14 +102,21 @@ __device__ __forceinline__ void store_nt(To* dst, const OutV<To, CW>& o) {
 // avg_pool2d does).  src holds the level above for the tile ([kER][RS][WS] floats, pitch PS per
 // p); the values are stored to `out` (layout `layout`, pyramid_offset) and returned in v, so
 // that the caller can place them in LDS for the next level once every thread has finished
-// reading src.  Thread chunk c covers CW consecutive x of one (p, y): with p the slowest index,
-// or, for pixel-interleaved levels, the fastest (the 32 p of one (y, x) segment are then one
-// contiguous run in memory); pool_stash uses the same split.
+// reading src.  Thread chunk c covers CW consecutive x of one (p, y), with p the slowest index;
+// for pixel-interleaved levels p is the fastest index (the p of one segment position are then
+// one contiguous run in memory), and in the 2 x 4 segment layout a chunk is one whole segment
+// (two rows of four x), so that every lane writes whole 16-element segments; pool_stash uses the
+// same split.
 template <int RS, int WS>
 struct Pool {
   static constexpr int RD = RS / 2, WD = WS / 2, CW = chunk_of(WD);
   static constexpr int NCH = kER * RD * WD / CW;     // chunks of the tile
   static constexpr int NIT = (NCH + kThreads - 1) / kThreads;
+  // kPlaneGroups24 levels: a chunk is one whole 2-row x 4-column segment
+  static constexpr bool SEG = RD % 2 == 0 && WD % 4 == 0;
+  static constexpr int NCH24 = SEG ? kER * RD * WD / 8 : 1;
+  static constexpr int NIT24 = (NCH24 + kThreads - 1) / kThreads;
+  static constexpr int VN = NIT * CW > NIT24 * 8 ? NIT * CW : NIT24 * 8;   // values per thread
   static constexpr int PD = RD * WD + 4;             // pitch per p of this level in LDS
 };
 
@@ -126,11 +133,38 @@ __device__ __forceinline__ void pool_split(int c, bool pfast, int& p, int& y, in
 template <typename To, int RS, int WS, int PS>
 __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* __restrict__ out,
                                              int64_t plane0, int64_t plane_stride, int gy0, int gx0,
-                                             int wl, int layout, float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
+                                             int wl, int layout, float (&v)[Pool<RS, WS>::VN]) {
   using PL = Pool<RS, WS>;
   constexpr int CW = PL::CW;
   const int t = threadIdx.x;
-  const bool pfast = layout == kPlaneGroups || layout == kPlaneGroups24;
+  if (layout == kPlaneGroups24) {
+    // one whole segment (output rows 2 yp, 2 yp + 1, columns x .. x + 3) per chunk, p fastest:
+    // the 16 p of a chunk row are one contiguous run of 16-element segments in memory
+    if constexpr (PL::SEG) {
+      constexpr int XS = PL::WD / 4;
+#pragma unroll
+      for (int it = 0; it < PL::NIT24; ++it) {
+        const int c = it * kThreads + t;
+        if (PL::NCH24 % kThreads == 0 || c < PL::NCH24) {
+          const int p = c % kER, r = c / kER;
+          const int yp = r / XS, x = (r % XS) * 4;
+          OutV<To, 8> o;
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float* a = src + p * PS + (2 * (2 * yp + dy)) * WS + 2 * (x + e);
+              const float s = 0.25f * ((a[0] + a[1]) + (a[WS] + a[WS + 1]));
+              v[it * 8 + dy * 4 + e] = s;
+              o.v[dy * 4 + e] = from_f<To>(s);
+            }
+          store_nt<To, 8>(out + pyramid_offset(layout, plane0 + p, gy0 + 2 * yp, gx0 + x, wl, plane_stride), o);
+        }
+      }
+    }
+    return;
+  }
+  const bool pfast = layout == kPlaneGroups;
 #pragma unroll
   for (int it = 0; it < PL::NIT; ++it) {
     const int c = it * kThreads + t;
@@ -141,36 +175,39 @@ __device__ __forceinline__ void pool_compute(const float* __restrict__ src, To* 
       OutV<To, CW> o;
 #pragma unroll
       for (int e = 0; e < CW; ++e) {
-        v[it][e] = 0.25f * ((a[2 * e] + a[2 * e + 1]) + (a[WS + 2 * e] + a[WS + 2 * e + 1]));
-        o.v[e] = from_f<To>(v[it][e]);
+        v[it * CW + e] = 0.25f * ((a[2 * e] + a[2 * e + 1]) + (a[WS + 2 * e] + a[WS + 2 * e + 1]));
+        o.v[e] = from_f<To>(v[it * CW + e]);
       }
-      if (layout == kPlaneGroups24) {
-        // CW is 4 or 8 here and x a multiple of CW: one 4-element half segment per 4 values
-        if constexpr (CW % 4 == 0) {
-#pragma unroll
-          for (int hseg = 0; hseg < CW / 4; ++hseg) {
-            OutV<To, 4> o4;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o4.v[e] = o.v[4 * hseg + e];
-            store_nt<To, 4>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x + 4 * hseg, wl, plane_stride),
-                            o4);
-          }
-        }
-      } else {
-        // a kPlaneGroups level has CW == 8 and gx a multiple of 8: the chunk is one segment
-        store_nt<To, CW>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl, plane_stride), o);
-      }
+      // a kPlaneGroups level has CW == 8 and gx a multiple of 8: the chunk is one segment
+      store_nt<To, CW>(out + pyramid_offset(layout, plane0 + p, gy0 + y, gx0 + x, wl, plane_stride), o);
     }
   }
 }
 
 template <int RS, int WS>
-__device__ __forceinline__ void pool_stash(float* __restrict__ dst, int layout,
-                                           const float (&v)[Pool<RS, WS>::NIT][Pool<RS, WS>::CW]) {
+__device__ __forceinline__ void pool_stash(float* __restrict__ dst, int layout, const float (&v)[Pool<RS, WS>::VN]) {
   using PL = Pool<RS, WS>;
   constexpr int WD = PL::WD, CW = PL::CW;
   const int t = threadIdx.x;
-  const bool pfast = layout == kPlaneGroups || layout == kPlaneGroups24;
+  if (layout == kPlaneGroups24) {
+    if constexpr (PL::SEG) {
+      constexpr int XS = WD / 4;
+#pragma unroll
+      for (int it = 0; it < PL::NIT24; ++it) {
+        const int c = it * kThreads + t;
+        if (PL::NCH24 % kThreads == 0 || c < PL::NCH24) {
+          const int p = c % kER, r = c / kER;
+          const int yp = r / XS, x = (r % XS) * 4;
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dst[p * PL::PD + (2 * yp + dy) * WD + x + e] = v[it * 8 + dy * 4 + e];
+        }
+      }
+    }
+    return;
+  }
+  const bool pfast = layout == kPlaneGroups;
 #pragma unroll
   for (int it = 0; it < PL::NIT; ++it) {
     const int c = it * kThreads + t;
@@ -178,7 +215,7 @@ __device__ __forceinline__ void pool_stash(float* __restrict__ dst, int layout,
       int p, y, x;
       pool_split<RS, WS>(c, pfast, p, y, x);
 #pragma unroll
-      for (int e = 0; e < CW; ++e) dst[p * PL::PD + y * WD + x + e] = v[it][e];
+      for (int e = 0; e < CW; ++e) dst[p * PL::PD + y * WD + x + e] = v[it * CW + e];
     }
   }
 }
@@ -220,7 +257,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   float* s1 = sout;
   float* s2 = s1 + kER * P1::PD;
   {
-    float v[P1::NIT][P1::CW];
+    float v[P1::VN];
     const int hl = h >> 1, wl = w >> 1, lay = pyramid_layout(1, HW, hl, wl, tiled);
     pool_compute<To, kRows, WT, SQ>(sout_in, l1, pl0, (int64_t)hl * wl, y0 >> 1, x0 >> 1, wl, lay, v);
     __syncthreads();                               // every read of the f32 tile is done
@@ -229,7 +266,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   }
   __syncthreads();
   {
-    float v[P2::NIT][P2::CW];
+    float v[P2::VN];
     const int hl = h >> 2, wl = w >> 2, lay = pyramid_layout(2, HW, hl, wl, tiled);
     pool_compute<To, kRows / 2, WT / 2, P1::PD>(s1, l2, pl0, (int64_t)hl * wl, y0 >> 2, x0 >> 2, wl, lay, v);
     if (levels < 4) {
@@ -240,7 +277,7 @@ __device__ void emit_rows(const float* __restrict__ sout_in, float* __restrict__
   }
   __syncthreads();
   {
-    float v[P3::NIT][P3::CW];
+    float v[P3::VN];
     const int hl = h >> 3, wl = w >> 3, lay = pyramid_layout(3, HW, hl, wl, tiled);
     pool_compute<To, kRows / 4, WT / 4, P2::PD>(s2, l3, pl0, (int64_t)hl * wl, y0 >> 3, x0 >> 3, wl, lay, v);
   }
