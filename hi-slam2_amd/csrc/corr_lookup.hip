@@ -839,14 +839,14 @@ int launch_bwd(const float* coords, const void* grad, void* vgrad, int64_t batch
 // window loads of all four levels are issued together (one memory latency per block), then per
 // level the taps go to LDS and the bilinear combine rounds each channel to bf16 (as the
 // lookup's own output) into an LDS row per pixel; then the 64 x C_out x 196 product on
-// v_mfma_f32_32x32x8bf16_1k (FP32 accumulation; each wave one 32-row slice of W, its fragments
-// read from L2 a step group ahead), bias, ReLU and
+// v_mfma_f32_32x32x16_bf16 (FP32 accumulation; each wave one 32-row slice of W, its fragments
+// read from L2 a step group ahead, 16 bytes per lane), bias, ReLU and
 // bf16 stores of 4 consecutive pixels.  bf16 pyramids of 4 levels, radius 3.
-typedef short conv_bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 conv_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float conv_acc16 __attribute__((ext_vector_type(16)));
 constexpr int kConvK = 196;                     // 4 levels x 49 channels
-constexpr int kConvKP = 200;                    // padded to whole MFMA steps of 8
-constexpr int kConvRow = kConvKP + 8;           // LDS row (bf16), 416 bytes
+constexpr int kConvKP = 208;                    // padded to whole MFMA steps of 16
+constexpr int kConvRow = kConvKP + 8;           // LDS row (bf16), 432 bytes
 static_assert(kThreads == 4 * kPix, "one thread per (level, pixel) for the window origins");
 
 template <int COUT>
@@ -924,38 +924,47 @@ corr_lookup_conv_kernel(LevelTable lv, const float* __restrict__ coords, int64_t
   static_assert(NOT * NP == kThreads / kWave && PH * NP == 2, "tile split");
   const int lane = t % kWave, wave = t / kWave;
   const int otile = wave % NOT, ph0 = (wave / NOT) * PH;
-  const int kh = 4 * (lane / 32);
+  const int kh = 8 * (lane / 32);              // lane half: k [8 kh', 8 kh' + 8) of each step
   conv_acc16 acc[PH];
 #pragma unroll
   for (int j = 0; j < PH; ++j)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[j][v] = 0.f;
-  // W's fragments come from L2: steps in groups of KG, the next group's loads issued before this
-  // group's MFMAs
-  constexpr int KS = kConvKP / 8, KG = 5, NG = KS / KG;
-  static_assert(KS % KG == 0, "whole step groups");
+  // W's fragments come from L2: steps in groups of KG (the last group partial), the next group's
+  // loads issued before this group's MFMAs.  W rows hold 196 values: the piece [192, 200) is
+  // read as four values and four zeros, pieces from 200 on are zeros (as A is there), so no
+  // load leaves the row
+  constexpr int KS = kConvKP / 16, KG = 4, NG = (KS + KG - 1) / KG;
+  static_assert(kConvK % 8 == 4 && kConvKP - kConvK < 16, "W row tail handling");
   const int oc_l = otile * 32 + lane % 32;
   const __hip_bfloat16* wrow = W + (int64_t)oc_l * kConvK + kh;
-  auto wload = [&](int grp, conv_bf16x4 (&b)[KG]) {
+  auto wload = [&](int grp, conv_bf16x8 (&b)[KG]) {
 #pragma unroll
     for (int q = 0; q < KG; ++q) {
-      const int k0 = (grp * KG + q) * 8;
-      b[q] = conv_bf16x4{0, 0, 0, 0};
-      if (k0 + kh < kConvK) b[q] = *reinterpret_cast<const conv_bf16x4*>(wrow + k0);
+      const int k0 = (grp * KG + q) * 16;
+      if (grp * KG + q >= KS) continue;
+      // two 8-byte loads: a W row is 392 bytes, so a piece is 8-byte but not always 16-byte aligned
+      typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+      const int kk = k0 + kh;
+      bf16x4 lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+      if (kk + 4 <= kConvK) lo = *reinterpret_cast<const bf16x4*>(wrow + k0);
+      if (kk + 8 <= kConvK) hi = *reinterpret_cast<const bf16x4*>(wrow + k0 + 4);
+      b[q] = conv_bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     }
   };
-  conv_bf16x4 wb[2][KG];
+  conv_bf16x8 wb[2][KG];
   wload(0, wb[0]);
 #pragma unroll
   for (int grp = 0; grp < NG; ++grp) {
     if (grp + 1 < NG) wload(grp + 1, wb[(grp + 1) & 1]);
 #pragma unroll
     for (int q = 0; q < KG; ++q) {
-      const int k0 = (grp * KG + q) * 8;
+      if (grp * KG + q >= KS) continue;
+      const int k0 = (grp * KG + q) * 16;
 #pragma unroll
       for (int j = 0; j < PH; ++j) {
-        const conv_bf16x4 a = *reinterpret_cast<const conv_bf16x4*>(&A[(ph0 + j) * 32 + lane % 32][k0 + kh]);
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wb[grp & 1][q], acc[j], 0, 0, 0);
+        const conv_bf16x8 a = *reinterpret_cast<const conv_bf16x8*>(&A[(ph0 + j) * 32 + lane % 32][k0 + kh]);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wb[grp & 1][q], acc[j], 0, 0, 0);
       }
     }
   }
