@@ -347,35 +347,55 @@ jdsa_reduce(const float* __restrict__ C, const float* __restrict__ w, const floa
         S[i][j] = S[j][i] = v;
       }
     for (int i = 0; i < kJdsaMaxD; ++i) b[i] = red[20 + i] - red[24 + i];
-    // Cholesky of the leading D x D block (padding rows of a smaller grid are unused)
+    // Cholesky of the leading D x D block (padding rows of a smaller grid are unused).  The loops
+    // run to the constant kJdsaMaxD with D as a guard, so every array index is a constant after
+    // unrolling and the arrays stay in registers (a loop bound of D would index them through
+    // scratch memory); the operations and their order are those of the plain D loops.
     double L[kJdsaMaxD][kJdsaMaxD] = {};
     int fail = 0;
-    for (int j = 0; j < D; ++j) {
-      double d = S[j][j];
-      for (int q = 0; q < j; ++q) d -= L[j][q] * L[j][q];
-      if (!(d > 0.0)) {
-        fail = 1;
-        break;
-      }
-      L[j][j] = sqrt(d);
-      for (int i = j + 1; i < D; ++i) {
-        double v = S[i][j];
-        for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
-        L[i][j] = v / L[j][j];
+#pragma unroll
+    for (int j = 0; j < kJdsaMaxD; ++j) {
+      if (j < D && !fail) {
+        double d = S[j][j];
+#pragma unroll
+        for (int q = 0; q < j; ++q) d -= L[j][q] * L[j][q];
+        if (!(d > 0.0)) {
+          fail = 1;
+        } else {
+          L[j][j] = sqrt(d);
+#pragma unroll
+          for (int i = j + 1; i < kJdsaMaxD; ++i) {
+            if (i < D) {
+              double v = S[i][j];
+#pragma unroll
+              for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
+              L[i][j] = v / L[j][j];
+            }
+          }
+        }
       }
     }
     double x[kJdsaMaxD] = {0.0, 0.0, 0.0, 0.0};
     if (!fail) {
-      double y[kJdsaMaxD];
-      for (int i = 0; i < D; ++i) {
-        double v = b[i];
-        for (int q = 0; q < i; ++q) v -= L[i][q] * y[q];
-        y[i] = v / L[i][i];
+      double y[kJdsaMaxD] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < kJdsaMaxD; ++i) {
+        if (i < D) {
+          double v = b[i];
+#pragma unroll
+          for (int q = 0; q < i; ++q) v -= L[i][q] * y[q];
+          y[i] = v / L[i][i];
+        }
       }
-      for (int i = D - 1; i >= 0; --i) {
-        double v = y[i];
-        for (int q = i + 1; q < D; ++q) v -= L[q][i] * x[q];
-        x[i] = v / L[i][i];
+#pragma unroll
+      for (int i = kJdsaMaxD - 1; i >= 0; --i) {
+        if (i < D) {
+          double v = y[i];
+#pragma unroll
+          for (int q = i + 1; q < kJdsaMaxD; ++q)
+            if (q < D) v -= L[q][i] * x[q];
+          x[i] = v / L[i][i];
+        }
       }
     }
     JdsaSlot& out = slots[s];
