@@ -547,10 +547,12 @@ ba_lin_frame(const float* __restrict__ poses, const float* __restrict__ disps, c
 }
 
 // The frame sums alone (grid: pixel blocks x frames), for graphs large enough to fill the GPU
-// with the linearisation by itself: a launch of their own keeps their occupancy (108 VGPRs)
-// instead of the linearisation's (212).
+// with the linearisation by itself: a launch of their own keeps their occupancy instead of the
+// linearisation's (212 VGPRs).  With FP64 sums the kernel is held to five waves per SIMD (93
+// VGPRs, no spills; 108 and four waves unconstrained), which measured slightly faster; the
+// FP32-sum instantiation would spill at five and keeps the compiler's choice.
 template <int kSem, typename Acc>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(Acc) == 8 ? 5 : 1)))
 ba_frame_sums(const float* __restrict__ poses, const float* __restrict__ disps, const float* __restrict__ intr,
               const float* __restrict__ targets, const float* __restrict__ weights, const int* __restrict__ edge_ii,
               const int* __restrict__ edge_jj, int HW, int wd, const float* __restrict__ eta, int eta_rows,
