@@ -127,6 +127,40 @@ def test_corrblock_bf16_all_edges_against_fixture(lookup_inputs):
     assert e <= 1e-2 and a <= 2e-2
 
 
+def test_lookup_conv_all_edges_matches_unfused(lookup_inputs):
+    """The lookup fused with the 1x1 convolution at the benchmarked size (3000 edges, every
+    output): against the unfused bf16 lookup of the same pyramid followed by a float32
+    convolution.  Tolerance: one bf16 rounding of the output (2^-7 of each value) plus 1e-5 for
+    the different FP32 accumulation order of the 196 products (the outputs are of order 0.1, so a
+    wrong pixel, edge or channel fails by orders of magnitude); no ReLU, so that values near zero
+    are compared as they are."""
+    import torch.nn.functional as F
+    from hislam2_amd import corr as hc
+    p, fm, coords, f = lookup_inputs
+    print('global300 lookup fused with a 1x1 conv: build 3000 edges, fused and unfused', flush=True)
+    fb = torch.from_numpy(fm).cuda().to(torch.bfloat16)
+    ii, jj = torch.from_numpy(p.ii).cuda(), torch.from_numpy(p.jj).cuda()
+    blk = hc.CorrBlock(fb[:, ii], fb[:, jj], dtype=torch.bfloat16)
+    c = torch.from_numpy(coords).cuda()
+    g = torch.Generator().manual_seed(11)
+    W = (torch.randn(128, 196, 1, 1, generator=g) / 14.0).cuda()
+    b = (torch.randn(128, generator=g) * 0.1).cuda()
+    out = blk.lookup_conv1x1(c, W, b, relu=False)                 # [1, 3000, 128, h, w] bf16
+    corr = blk(c)                                                 # [1, 3000, 196, h, w] bf16
+    del blk
+    torch.cuda.empty_cache()
+    n = c.shape[1]
+    Wf = W.to(torch.bfloat16).float()
+    worst = 0.0
+    for e0 in range(0, n, 250):
+        e1 = min(n, e0 + 250)
+        ref = F.conv2d(corr[0, e0:e1].float(), Wf, b)
+        diff = (out[0, e0:e1].float() - ref).abs()
+        worst = max(worst, float(diff.max()))
+        assert bool((diff <= ref.abs() * 2.0 ** -7 + 1e-5).all()), (e0, float(diff.max()))
+    print(f'fused lookup-conv (3000 edges): largest absolute difference {worst:.3g}', flush=True)
+
+
 @pytest.mark.parametrize('dtype', [torch.bfloat16, torch.float32])
 def test_altcorr_mfma_all_edges_against_fixture(lookup_inputs, dtype):
     from hislam2_amd import corr as hc
