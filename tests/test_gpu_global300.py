@@ -15,6 +15,8 @@ their checksums asserted first).
   (AltCorrBlock), compared on 7 edges spread over the list (the first, the middle and the last)
   at 128 seeded pixels each: 1e-2 relative to the largest reference value and 2e-2 absolute
   (the feature maps themselves are rounded to bf16); the float32 on-the-fly path 1e-5 relative.
+* the lookup fused with a 1x1 convolution, every output of all 3000 edges against the unfused
+  lookup of the same pyramid followed by a float32 convolution.
 """
 import os
 
